@@ -1,0 +1,387 @@
+"""cuspfft_amd -- MI355X-native binary-sparse 2-D R2C FFT.
+
+Python host mirror of the reference's entry points for this path
+(reference include/sparse_fft.h:7-13, 57, 70) over the C ABI in
+``include/spfft_c.h``, implemented by hand-written gfx950 HIP kernels in
+``cuspfft_amd/_lib/libspfft_hip.so``.  PyTorch is used only for device memory
+and streams.  There is no CPU fallback: if the HIP library is missing every
+call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libspfft_hip.so")
+CLI_PATH = os.path.join(_HERE, "_lib", "spfft_cli")
+
+SMOOTH_3 = 3
+SMOOTH_7 = 7
+PREC_FP32 = 0
+
+STATUS = {0: "OK", 1: "INVALID_ARG", 2: "HIP", 3: "ALLOC", 4: "UNSUPPORTED", 5: "BAD_INPUT"}
+
+
+class SpfftError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(msg)
+        self.status = status
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("length_rule", ctypes.c_int), ("u_tile", ctypes.c_int),
+                ("precision", ctypes.c_int), ("fft_len", ctypes.c_int), ("m1", ctypes.c_int),
+                ("rows_per_wg", ctypes.c_int), ("slab", ctypes.c_int),
+                ("conv_rows", ctypes.c_int)]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_int), ("cols", ctypes.c_int), ("out_cols", ctypes.c_int),
+                ("pitch", ctypes.c_int), ("fft_len", ctypes.c_int), ("m1", ctypes.c_int),
+                ("m2", ctypes.c_int), ("packed", ctypes.c_int), ("u_tile", ctypes.c_int),
+                ("rows_per_wg", ctypes.c_int), ("slab", ctypes.c_int),
+                ("conv_rows", ctypes.c_int), ("workspace_bytes", ctypes.c_size_t)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("d_output", ctypes.c_void_p), ("output_cols", ctypes.c_int),
+                ("ms", ctypes.c_float), ("ms_e2e", ctypes.c_float),
+                ("mem_bytes", ctypes.c_size_t)]
+
+
+class CscView(ctypes.Structure):
+    _fields_ = [("n_sparse_cols", ctypes.c_int), ("nnz", ctypes.c_int), ("packed", ctypes.c_int),
+                ("d_sparse_cols", ctypes.c_void_p), ("d_col_ptr", ctypes.c_void_p),
+                ("d_row_idx", ctypes.c_void_p), ("d_col_start", ctypes.c_void_p)]
+
+
+class Profile(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double * 4), ("launches", ctypes.c_int * 4)]
+
+
+PROFILE_STAGES = ("csc_build", "colpass_fft1", "conv_rows", "ifft1_finalize")
+
+
+class HostCoo(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_int), ("cols", ctypes.c_int), ("nnz", ctypes.c_int),
+                ("row_idx", ctypes.POINTER(ctypes.c_int)),
+                ("col_idx", ctypes.POINTER(ctypes.c_int))]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+
+
+def load():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"cuspfft_amd: HIP library not built ({LIB_PATH}); "
+                           "run `python -m cuspfft_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "spfft_opts_default": (None, [ctypes.POINTER(Opts)]),
+        "spfft_last_error": (ctypes.c_char_p, []),
+        "spfft_version": (ctypes.c_char_p, []),
+        "spfft_plan_create": (_i, [_i, _i, ctypes.POINTER(Opts), ctypes.POINTER(_vp)]),
+        "spfft_plan_destroy": (_i, [_vp]),
+        "spfft_plan_get_info": (_i, [_vp, ctypes.POINTER(PlanInfo)]),
+        "spfft_csc_build": (_i, [_vp, _vp, _vp, _i, _vp]),
+        "spfft_csc_get": (_i, [_vp, _vp, ctypes.POINTER(CscView)]),
+        "spfft_rowpass": (_i, [_vp, _vp, ctypes.c_int64, _vp]),
+        "spfft_execute": (_i, [_vp, _vp, _vp, _i, _vp, ctypes.c_int64, _vp]),
+        "spfft_check_input": (_i, [_vp, _vp]),
+        "spfft_csc_bluestein_mixed_radix": (_i, [_i, _i, _i, _vp, _vp, _i, ctypes.POINTER(Result)]),
+        "spfft_csc_bluestein_cufft_smooth": (_i, [_i, _i, _i, _vp, _vp, _i, ctypes.POINTER(Result)]),
+        "spfft_run_host_coo": (_i, [_i, _i, _i, _vp, _vp, ctypes.POINTER(Opts),
+                                    ctypes.POINTER(Result)]),
+        "spfft_free_output": (_i, [_vp]),
+        "spfft_profile_enable": (_i, [_vp, _i]),
+        "spfft_profile_read": (_i, [_vp, _vp, ctypes.POINTER(Profile)]),
+        "spfft_next_3_smooth": (_i, [_i]),
+        "spfft_next_7_smooth": (_i, [_i]),
+        "spfft_factor_3_smooth": (_i, [_i, _vp, _i]),
+        "spfft_coo_generate": (_i, [ctypes.c_char_p, _i, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                                    ctypes.POINTER(HostCoo)]),
+        "spfft_coo_free": (None, [ctypes.POINTER(HostCoo)]),
+        "spfft_read_mtx": (_i, [ctypes.c_char_p, ctypes.POINTER(HostCoo)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(status: int):
+    if status != 0:
+        msg = load().spfft_last_error().decode()
+        raise SpfftError(status, f"spfft {STATUS.get(status, status)}: {msg}")
+
+
+def next_3_smooth(n: int) -> int:
+    return load().spfft_next_3_smooth(int(n))
+
+
+def next_7_smooth(n: int) -> int:
+    return load().spfft_next_7_smooth(int(n))
+
+
+def factor_3_smooth(n: int) -> list[int]:
+    buf = (ctypes.c_int * 128)()
+    k = load().spfft_factor_3_smooth(int(n), ctypes.cast(buf, _vp), 128)
+    if k < 0:
+        raise RuntimeError("factor_3_smooth: not 3-smooth")
+    return list(buf[:k])
+
+
+# ---------------------------------------------------------------------------
+# COO input (the reference's COOMatrix, include/mtx_reader.h:7-13)
+# ---------------------------------------------------------------------------
+@dataclass
+class COOMatrix:
+    rows: int
+    cols: int
+    row_idx: np.ndarray
+    col_idx: np.ndarray
+
+    @property
+    def nnz(self) -> int:
+        return int(len(self.row_idx))
+
+
+def _from_host_coo(h: HostCoo) -> COOMatrix:
+    n = h.nnz
+    r = np.ctypeslib.as_array(h.row_idx, shape=(max(n, 1),))[:n].astype(np.int32).copy()
+    c = np.ctypeslib.as_array(h.col_idx, shape=(max(n, 1),))[:n].astype(np.int32).copy()
+    return COOMatrix(h.rows, h.cols, r, c)
+
+
+def generate(kind: str, n: int, p0: float, p1: float = 0.0, p2: float = 0.0, p3: float = 0.0,
+             seed: int = 0) -> COOMatrix:
+    """Seeded synthetic binary symmetric matrix (see spfft_coo_generate)."""
+    L = load()
+    h = HostCoo()
+    st = L.spfft_coo_generate(kind.encode(), int(n), float(p0), float(p1), float(p2),
+                              float(p3), int(seed), ctypes.byref(h))
+    try:
+        if st != 0:
+            raise SpfftError(st, f"generator {kind} failed")
+        return _from_host_coo(h)
+    finally:
+        L.spfft_coo_free(ctypes.byref(h))
+
+
+def read_mtx(path: str) -> COOMatrix:
+    L = load()
+    h = HostCoo()
+    st = L.spfft_read_mtx(path.encode(), ctypes.byref(h))
+    try:
+        if st != 0:
+            raise RuntimeError(f"Cannot read Matrix Market file: {path}")
+        return _from_host_coo(h)
+    finally:
+        L.spfft_coo_free(ctypes.byref(h))
+
+
+# Named workloads (sizes fixed by the benchmark configurations).
+CONFIGS = {
+    "c1": ("uniform", 2310, 8000, 0, 0, 0),
+    "c2": ("banded", 8219, 242700, 300, 1200, 0.02),
+    "c3": ("chunglu", 16706, 121250, 2.1, 360, 0),
+    "c5": ("chunglu", 40421, 351400, 2.2, 0, 0),
+}
+
+CONFIG_TEXT = {
+    "c1": "uniform symmetric 2310x2310, 8000 random i<j pairs mirrored (16000 nnz), seed 0",
+    "c2": "banded symmetric 8219x8219, p ~ exp(-|i-j|/300), cutoff 1200, +2% off-band, ~242700 nnz, seed 0",
+    "c3": "Chung-Lu power-law 16706x16706, exponent 2.1, max degree 360, ~121250 nnz, seed 0",
+    "c5": "Chung-Lu power-law 40421x40421, exponent 2.2, ~351400 nnz, seed 0",
+}
+
+
+def config_matrix(name: str, seed: int = 0) -> COOMatrix:
+    kind, n, p0, p1, p2, p3 = CONFIGS[name]
+    return generate(kind, n, p0, p1, p2, p3, seed)
+
+
+# ---------------------------------------------------------------------------
+# Plans and device execution (torch supplies device memory and the stream)
+# ---------------------------------------------------------------------------
+def _stream_handle(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class Plan:
+    """A planned transform for one (rows, cols) shape.  Mirrors the set-up half
+    of the reference drivers (src/sparse_fft_bluestein.cu:2256-2320)."""
+
+    def __init__(self, rows: int, cols: int, length_rule: int = SMOOTH_3, u_tile: int = 128,
+                 fft_len: int = 0, m1: int = 0, rows_per_wg: int = 0, slab: int = 0,
+                 conv_rows: int = 0, precision: int = PREC_FP32):
+        L = load()
+        o = Opts()
+        L.spfft_opts_default(ctypes.byref(o))
+        o.length_rule, o.u_tile, o.precision = length_rule, u_tile, precision
+        o.fft_len, o.m1, o.rows_per_wg, o.slab, o.conv_rows = fft_len, m1, rows_per_wg, slab, conv_rows
+        self._h = _vp()
+        _check(L.spfft_plan_create(int(rows), int(cols), ctypes.byref(o), ctypes.byref(self._h)))
+        inf = PlanInfo()
+        _check(L.spfft_plan_get_info(self._h, ctypes.byref(inf)))
+        self.info = {k: getattr(inf, k) for k, _ in PlanInfo._fields_}
+        self.rows, self.cols = rows, cols
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.spfft_plan_destroy(h)
+            self._h = None
+
+    @property
+    def pitch(self) -> int:
+        return self.info["pitch"]
+
+    def alloc_output(self):
+        import torch
+        return torch.empty((self.rows, self.pitch), dtype=torch.complex64, device="cuda")
+
+    def execute(self, d_row, d_col, d_out, stream=None):
+        """All three stages on device COO (int32 torch tensors) into d_out [rows, pitch]."""
+        self._check_io(d_row, d_col, d_out)
+        _check(load().spfft_execute(self._h, _vp(d_row.data_ptr()), _vp(d_col.data_ptr()),
+                                    int(d_row.numel()), _vp(d_out.data_ptr()), int(d_out.shape[1]),
+                                    _stream_handle(stream)))
+
+    def csc_build(self, d_row, d_col, stream=None):
+        self._check_coo(d_row, d_col)
+        _check(load().spfft_csc_build(self._h, _vp(d_row.data_ptr()), _vp(d_col.data_ptr()),
+                                      int(d_row.numel()), _stream_handle(stream)))
+
+    def rowpass(self, d_out, stream=None):
+        _check(load().spfft_rowpass(self._h, _vp(d_out.data_ptr()), int(d_out.shape[1]),
+                                    _stream_handle(stream)))
+
+    def check_input(self, stream=None):
+        _check(load().spfft_check_input(self._h, _stream_handle(stream)))
+
+    def profile(self, on: bool = True):
+        """Start (or stop) per-kernel event timing on the launch stream."""
+        _check(load().spfft_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self, stream=None) -> dict:
+        """{stage: (total_ms, launches)} since the last read."""
+        pr = Profile()
+        _check(load().spfft_profile_read(self._h, _stream_handle(stream), ctypes.byref(pr)))
+        return {name: (pr.ms[i], pr.launches[i]) for i, name in enumerate(PROFILE_STAGES)}
+
+    def csc_get(self, stream=None) -> dict:
+        """Copies of the device CSC arrays as numpy (sparse_cols, col_ptr, row_idx, col_start)."""
+        import torch
+        v = CscView()
+        _check(load().spfft_csc_get(self._h, _stream_handle(stream), ctypes.byref(v)))
+        it = np.uint16 if v.packed else np.int32
+        return {
+            "n_sparse_cols": v.n_sparse_cols,
+            "packed": bool(v.packed),
+            "sparse_cols": _d2h(v.d_sparse_cols, v.n_sparse_cols, it),
+            "col_ptr": _d2h(v.d_col_ptr, v.n_sparse_cols + 1, np.int32),
+            "row_idx": _d2h(v.d_row_idx, v.nnz, it),
+            "col_start": _d2h(v.d_col_start, self.cols + 1, np.int32),
+        }
+
+    def _check_coo(self, d_row, d_col):
+        import torch
+        for t in (d_row, d_col):
+            if t.dtype != torch.int32 or not t.is_cuda or not t.is_contiguous():
+                raise ValueError("COO arrays must be contiguous int32 CUDA tensors")
+        if d_row.numel() != d_col.numel():
+            raise ValueError("row and col arrays differ in length")
+
+    def _check_io(self, d_row, d_col, d_out):
+        import torch
+        self._check_coo(d_row, d_col)
+        if (d_out.dtype != torch.complex64 or not d_out.is_cuda or d_out.dim() != 2
+                or d_out.shape[0] != self.rows or d_out.shape[1] < self.info["out_cols"]
+                or not d_out.is_contiguous()):
+            raise ValueError("output must be a contiguous complex64 CUDA tensor [rows, >= cols/2+1]")
+
+
+_hip = None
+
+
+def _hip_lib():
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_int]
+        _hip.hipMemcpy.restype = ctypes.c_int
+    return _hip
+
+
+def _d2h(ptr, n: int, dtype) -> np.ndarray:
+    out = np.empty(max(n, 0), dtype=dtype)
+    if n > 0:
+        rc = _hip_lib().hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2)  # DeviceToHost
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed ({rc})")
+    return out
+
+
+def device_ptr_to_host(ptr, rows: int, pitch: int) -> np.ndarray:
+    """Copy a [rows, pitch] complex64 device array (raw pointer) to host."""
+    return _d2h(ptr, rows * pitch * 2, np.float32).view(np.complex64).reshape(rows, pitch)
+
+
+@dataclass
+class SparseFFTResult:
+    """Mirror of the reference's SparseFFTResult (include/sparse_fft.h:7-13);
+    d_output is a torch tensor [rows, output_cols] (pitch P)."""
+    d_output: object
+    output_cols: int
+    ms: float
+    mem_bytes: int
+    ms_e2e: float = 0.0
+
+
+def _run_reference_shaped(coo: COOMatrix, u_tile: int, length_rule: int) -> SparseFFTResult:
+    import torch
+    if u_tile <= 0:
+        raise RuntimeError("CSC tile size must be positive")
+    free0, _ = torch.cuda.mem_get_info()
+    plan = Plan(coo.rows, coo.cols, length_rule=length_rule, u_tile=u_tile)
+    d_row = torch.as_tensor(np.ascontiguousarray(coo.row_idx, np.int32)).cuda()
+    d_col = torch.as_tensor(np.ascontiguousarray(coo.col_idx, np.int32)).cuda()
+    out = plan.alloc_output()
+    torch.cuda.synchronize()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record()
+    plan.csc_build(d_row, d_col)
+    e1.record()
+    plan.rowpass(out)
+    e2.record()
+    e2.synchronize()
+    plan.check_input()
+    free1, _ = torch.cuda.mem_get_info()
+    return SparseFFTResult(out, plan.pitch, e1.elapsed_time(e2), max(free0 - free1, 0),
+                           e0.elapsed_time(e2))
+
+
+def sparse_fft_csc_bluestein_mixed_radix(coo: COOMatrix, u_tile: int = 128) -> SparseFFTResult:
+    """Bluestein at next_3_smooth(2*cols-1); reference src/sparse_fft_bluestein.cu:2256-2392."""
+    return _run_reference_shaped(coo, u_tile, SMOOTH_3)
+
+
+def sparse_fft_csc_bluestein_cufft_smooth(coo: COOMatrix, u_tile: int = 128) -> SparseFFTResult:
+    """Bluestein at next_7_smooth(2*cols-1); reference src/sparse_fft_bluestein.cu:2557-2719."""
+    return _run_reference_shaped(coo, u_tile, SMOOTH_7)

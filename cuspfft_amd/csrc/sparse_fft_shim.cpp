@@ -1,0 +1,56 @@
+// sparse_fft_shim.cpp -- the reference-shaped C++ entry points (sparse_fft.h,
+// mtx_reader.h) over the C ABI.  Errors become std::runtime_error, as the
+// reference's CUDA_CHECK / CUFFT_CHECK macros throw
+// (reference include/sparse_fft_internal.cuh:16-38).
+#include <stdexcept>
+#include <string>
+
+#include "../../include/sparse_fft.h"
+#include "../../include/spfft_c.h"
+
+namespace {
+
+SparseFFTResult run(const COOMatrix& coo, int u_tile, bool smooth7) {
+    if (coo.nnz < 0 || (size_t)coo.nnz > coo.row_idx.size() || (size_t)coo.nnz > coo.col_idx.size())
+        throw std::runtime_error("COOMatrix nnz does not match its index arrays");
+    spfft_result r;
+    const spfft_status st =
+        smooth7 ? spfft_csc_bluestein_cufft_smooth(coo.rows, coo.cols, coo.nnz, coo.row_idx.data(),
+                                                   coo.col_idx.data(), u_tile, &r)
+                : spfft_csc_bluestein_mixed_radix(coo.rows, coo.cols, coo.nnz, coo.row_idx.data(),
+                                                  coo.col_idx.data(), u_tile, &r);
+    if (st != SPFFT_OK) throw std::runtime_error(spfft_last_error());
+    SparseFFTResult out;
+    out.d_output = static_cast<hipFloatComplex*>(r.d_output);
+    out.output_cols = r.output_cols;
+    out.ms = r.ms;
+    out.mem_bytes = r.mem_bytes;
+    return out;
+}
+
+}  // namespace
+
+SparseFFTResult sparse_fft_csc_bluestein_mixed_radix(const COOMatrix& coo, int u_tile) {
+    return run(coo, u_tile, false);
+}
+
+SparseFFTResult sparse_fft_csc_bluestein_cufft_smooth(const COOMatrix& coo, int u_tile) {
+    return run(coo, u_tile, true);
+}
+
+COOMatrix read_mtx(const std::string& path) {
+    spfft_host_coo h;
+    const spfft_status st = spfft_read_mtx(path.c_str(), &h);
+    if (st != SPFFT_OK) {
+        spfft_coo_free(&h);
+        throw std::runtime_error("Cannot read Matrix Market file: " + path);
+    }
+    COOMatrix c;
+    c.rows = h.rows;
+    c.cols = h.cols;
+    c.nnz = h.nnz;
+    c.row_idx.assign(h.row_idx, h.row_idx + h.nnz);
+    c.col_idx.assign(h.col_idx, h.col_idx + h.nnz);
+    spfft_coo_free(&h);
+    return c;
+}

@@ -1,0 +1,43 @@
+// spfft_kernels.h -- host-visible launchers of the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "spfft_types.h"
+
+namespace spfft {
+
+// ---- COO -> compact CSC (csc_build.hip) ----------------------------------
+struct CscBuildArgs {
+    int R, C, nnz;
+    bool packed;
+    const int* row;          // device COO, nnz entries each
+    const int* col;
+    int* counts;             // C+1
+    int* colstart;           // C+1 dense column starts (output)
+    void* sparse_cols;       // n_sp uint16 (packed) or int32 (output)
+    int* col_ptr;            // n_sp+1 (output)
+    void* row_idx;           // nnz uint16 (packed) or int32 (output)
+    int* totals;             // [0] = valid nnz, [1] = n_sp (output)
+    int* err;                // nonzero if any index is out of range (output)
+    int2* scan_tmp;          // csc_scan_tmp_elems(max(C, 256*blocks))
+    int* hist;               // csc_sort_hist_elems(nnz)
+    int* hist_off;           // csc_sort_hist_elems(nnz) + 1
+    int* keys[2];            // nnz each (only used when more than one pass)
+    int* vals[2];
+};
+
+size_t csc_scan_tmp_elems(int n);
+size_t csc_sort_hist_elems(int nnz);
+int csc_sort_passes(int cols);
+hipError_t launch_csc_build(const CscBuildArgs& a, hipStream_t s);
+
+// ---- row pass (rowpass.hip) ----------------------------------------------
+size_t k1_lds_bytes(const RowPassArgs& a);
+size_t k2_lds_bytes(const RowPassArgs& a);
+size_t k3_lds_bytes(const RowPassArgs& a);
+// ev: NULL, or 4 events recorded before K1, after K1, after K2, after K3.
+hipError_t launch_rowpass_tile(const RowPassArgs& a, bool packed, hipStream_t s, hipEvent_t* ev);
+hipError_t set_rowpass_lds_limits(size_t max_bytes);
+
+}  // namespace spfft

@@ -1,0 +1,691 @@
+// spfft_plan.cpp -- host planner and C ABI (include/spfft_c.h).
+//
+// Planning mirrors the reference drivers' set-up (src/sparse_fft_bluestein.cu:
+// 2256-2320 for --csc-mixed-radix, :2557-2642 for --csc-cufft-smooth):
+// smooth Bluestein length M, chirp h, kernel b and B^ = DFT_M(b).  Here every
+// table is produced in FP64 on the host with exact integer reduction of the
+// angle (k^2 mod 2C, k*n mod M) and rounded once to FP32; B^ comes from an
+// FP64 FFT and has the 1/M scale folded in.  The device work (CSC build, pass
+// 1, row pass) is enqueued on the caller's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/spfft_c.h"
+#include "spfft_kernels.h"
+#include "spfft_types.h"
+
+using spfft::FftStages;
+using spfft::RowPassArgs;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+spfft_status fail(spfft_status st, const std::string& msg) {
+    g_last_error = msg;
+    return st;
+}
+
+spfft_status hip_fail(hipError_t e, const char* what) {
+    return fail(e == hipErrorOutOfMemory ? SPFFT_ERR_ALLOC : SPFFT_ERR_HIP,
+                std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+#define SPFFT_HIP(call, what)                                   \
+    do {                                                        \
+        hipError_t e__ = (call);                                \
+        if (e__ != hipSuccess) return hip_fail(e__, what);      \
+    } while (0)
+
+const double kPi = 3.14159265358979323846264338327950288;
+typedef std::complex<double> cd;
+
+// exp(sign * 2 pi i * k / n), k reduced exactly mod n first.
+cd unit_root(long long k, long long n, int sign) {
+    long long kk = k % n;
+    if (kk < 0) kk += n;
+    const double a = 2.0 * kPi * (double)kk / (double)n;
+    return cd(std::cos(a), sign * std::sin(a));
+}
+
+float2 to_f2(cd z) { return make_float2((float)z.real(), (float)z.imag()); }
+
+int next_smooth(int n, const int* primes, int np) {
+    if (n < 1) n = 1;
+    for (int m = n;; m++) {
+        int x = m;
+        for (int i = 0; i < np; i++)
+            while (x % primes[i] == 0) x /= primes[i];
+        if (x == 1) return m;
+    }
+}
+
+bool is_7_smooth(int n) {
+    if (n < 1) return false;
+    for (int p : {2, 3, 5, 7})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+// Radix sequence for a 7-smooth length: 8s, then one 4 or 2, then 7s, 5s, 3s.
+bool plan_stages(int L, FftStages& f) {
+    f.L = L;
+    f.nstages = 0;
+    int n = L, c2 = 0;
+    while (n % 2 == 0) { n /= 2; c2++; }
+    std::vector<int> r;
+    while (c2 >= 3) { r.push_back(8); c2 -= 3; }
+    if (c2 == 2) r.push_back(4);
+    if (c2 == 1) r.push_back(2);
+    for (int p : {7, 5, 3})
+        while (n % p == 0) { r.push_back(p); n /= p; }
+    if (n != 1 || (int)r.size() > spfft::kMaxStages) return false;
+    for (size_t i = 0; i < r.size(); i++) f.radix[i] = r[i];
+    f.nstages = (int)r.size();
+    return true;
+}
+
+// FP64 Stockham FFT (same autosort form as the device kernels), any 7-smooth n.
+void fft64(std::vector<cd>& x, int sign) {
+    const int n = (int)x.size();
+    FftStages f;
+    if (n <= 1 || !plan_stages(n, f)) return;
+    std::vector<cd> y(n), tw(n);
+    for (int i = 0; i < n; i++) tw[i] = unit_root(i, n, sign);
+    int Ns = 1;
+    for (int s = 0; s < f.nstages; s++) {
+        const int R = f.radix[s], nb = n / R, step = n / (Ns * R);
+        std::vector<cd> F(R * R);
+        for (int a = 0; a < R; a++)
+            for (int b = 0; b < R; b++) F[a * R + b] = unit_root((long long)a * b, R, sign);
+        cd v[8], o[8];
+        for (int j = 0; j < nb; j++) {
+            const int k = j % Ns;
+            for (int r = 0; r < R; r++) v[r] = x[j + r * nb] * tw[(long long)r * k * step % n];
+            for (int p = 0; p < R; p++) {
+                cd acc = 0;
+                for (int q = 0; q < R; q++) acc += F[p * R + q] * v[q];
+                o[p] = acc;
+            }
+            const int d = (j / Ns) * Ns * R + k;
+            for (int r = 0; r < R; r++) y[d + r * Ns] = o[r];
+        }
+        x.swap(y);
+        Ns *= R;
+    }
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        reset();
+        if (n == 0) n = 16;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) { p = nullptr; return e; }
+        bytes = n;
+        return hipSuccess;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+template <class T>
+hipError_t upload(DevBuf& b, const std::vector<T>& v) {
+    hipError_t e = b.alloc(v.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (v.empty()) return hipSuccess;
+    return hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+size_t g_lds_limit = 0;
+std::mutex g_lds_mutex;
+
+spfft_status ensure_lds_limit(size_t& budget) {
+    std::lock_guard<std::mutex> lk(g_lds_mutex);
+    if (g_lds_limit == 0) {
+        int dev = 0, v = 0;
+        SPFFT_HIP(hipGetDevice(&dev), "hipGetDevice");
+        SPFFT_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
+                  "hipDeviceGetAttribute");
+        g_lds_limit = (size_t)v;
+    }
+    budget = std::min<size_t>(g_lds_limit, 65536);
+    return SPFFT_OK;
+}
+
+}  // namespace
+
+struct spfft_plan_s {
+    int R = 0, C = 0, V = 0, P = 0, M = 0, M1 = 0, M2 = 0;
+    spfft_opts opts{};
+    bool packed = false;
+    RowPassArgs base{};  // kernel argument template (pointers to the tables filled in)
+    DevBuf h, wr_hi, wr_lo, tw1, tw2, twM, bhat, T;
+    // CSC workspace
+    int nnz_cap = -1;
+    int last_nnz = 0;
+    bool have_csc = false;
+    DevBuf counts, colstart, sparse_cols, col_ptr, totals, err, scan_tmp, hist, hist_off;
+    DevBuf keys0, keys1, vals0, vals1, row_idx;
+    size_t table_bytes = 0;
+    // optional per-kernel event timing (spfft_profile_*)
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    // event groups recorded since the last read: kind 0 = CSC build (2 events),
+    // kind 1 = row-pass tile (4 events)
+    std::vector<std::pair<int, size_t>> ev_groups;
+    ~spfft_plan_s() {
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    }
+    hipEvent_t* take_events(int n) {
+        while (ev_pool.size() < ev_used + n) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            ev_pool.push_back(e);
+        }
+        hipEvent_t* r = &ev_pool[ev_used];
+        ev_used += n;
+        return r;
+    }
+};
+
+namespace {
+
+spfft_status choose_split(spfft_plan_s& p, int m1_req) {
+    const int M = p.M;
+    if (m1_req > 0) {
+        if (M % m1_req != 0) return fail(SPFFT_ERR_INVALID_ARG, "m1 must divide the Bluestein length");
+        p.M1 = m1_req;
+    } else {
+        int best = 1;
+        for (int d = 1; (long long)d * d <= M; d++)
+            if (M % d == 0 && d <= 512) best = d;
+        p.M1 = best;
+    }
+    p.M2 = M / p.M1;
+    if (p.M1 > 1024 || p.M2 > 2048)
+        return fail(SPFFT_ERR_UNSUPPORTED, "Bluestein length " + std::to_string(M) +
+                                               " does not split into M1 <= 1024 and M2 <= 2048");
+    if (!plan_stages(p.M1, p.base.f1) || !plan_stages(p.M2, p.base.f2))
+        return fail(SPFFT_ERR_UNSUPPORTED, "Bluestein length must be 7-smooth");
+    return SPFFT_OK;
+}
+
+spfft_status choose_tiling(spfft_plan_s& p, size_t budget) {
+    RowPassArgs& a = p.base;
+    auto pow2_ok = [](int x) { return x > 0 && x <= 64 && (x & (x - 1)) == 0; };
+    const int js[] = {64, 32, 16, 8, 4, 2, 1};
+    const int us[] = {8, 4, 2, 1};
+    if (p.opts.slab && !pow2_ok(p.opts.slab))
+        return fail(SPFFT_ERR_INVALID_ARG, "slab must be a power of two <= 64");
+    if (p.opts.rows_per_wg && !(p.opts.rows_per_wg == 1 || p.opts.rows_per_wg == 2 ||
+                                p.opts.rows_per_wg == 4 || p.opts.rows_per_wg == 8))
+        return fail(SPFFT_ERR_INVALID_ARG, "rows_per_wg must be 1, 2, 4 or 8");
+    bool found = false;
+    int jdef = 1;
+    while (jdef < 16 && jdef < p.M2) jdef <<= 1;
+    for (int J : js) {
+        if (p.opts.slab ? J != p.opts.slab : J > jdef) continue;
+        for (int U : us) {
+            if (p.opts.rows_per_wg ? U != p.opts.rows_per_wg : false) continue;
+            a.J = J;
+            a.U = U;
+            if (spfft::k1_lds_bytes(a) <= budget && spfft::k3_lds_bytes(a) <= budget) { found = true; break; }
+        }
+        if (found) break;
+    }
+    if (!found) return fail(SPFFT_ERR_UNSUPPORTED, "sub-FFT length M1 does not fit in LDS");
+    const int gs[] = {16, 8, 4, 2, 1};
+    found = false;
+    for (int G : gs) {
+        if (p.opts.conv_rows ? G != p.opts.conv_rows : false) continue;
+        a.G2 = G;
+        if (spfft::k2_lds_bytes(a) <= budget) { found = true; break; }
+    }
+    if (!found) return fail(SPFFT_ERR_UNSUPPORTED, "sub-FFT length M2 does not fit in LDS");
+    return SPFFT_OK;
+}
+
+spfft_status ensure_csc_capacity(spfft_plan_s& p, int nnz) {
+    if (nnz <= p.nnz_cap) return SPFFT_OK;
+    const int cap = std::max(nnz, 1);
+    const size_t scan_n = std::max(spfft::csc_scan_tmp_elems(p.C),
+                                   spfft::csc_scan_tmp_elems((int)spfft::csc_sort_hist_elems(cap)));
+    const size_t ib = p.packed ? 2 : 4;
+    SPFFT_HIP(p.scan_tmp.alloc(scan_n * sizeof(int2)), "hipMalloc(scan)");
+    SPFFT_HIP(p.hist.alloc(spfft::csc_sort_hist_elems(cap) * sizeof(int)), "hipMalloc(hist)");
+    SPFFT_HIP(p.hist_off.alloc((spfft::csc_sort_hist_elems(cap) + 1) * sizeof(int)), "hipMalloc(hist)");
+    SPFFT_HIP(p.row_idx.alloc((size_t)cap * ib), "hipMalloc(row_idx)");
+    if (spfft::csc_sort_passes(p.C) > 1) {
+        SPFFT_HIP(p.keys0.alloc((size_t)cap * 4), "hipMalloc(keys)");
+        SPFFT_HIP(p.keys1.alloc((size_t)cap * 4), "hipMalloc(keys)");
+        SPFFT_HIP(p.vals0.alloc((size_t)cap * 4), "hipMalloc(vals)");
+        SPFFT_HIP(p.vals1.alloc((size_t)cap * 4), "hipMalloc(vals)");
+    }
+    p.nnz_cap = cap;
+    return SPFFT_OK;
+}
+
+spfft_status do_csc_build(spfft_plan_s& p, const int* d_row, const int* d_col, int nnz, hipStream_t s) {
+    if (nnz < 0) return fail(SPFFT_ERR_INVALID_ARG, "nnz must be >= 0");
+    if (nnz > 0 && (!d_row || !d_col)) return fail(SPFFT_ERR_INVALID_ARG, "NULL COO pointer");
+    spfft_status st = ensure_csc_capacity(p, nnz);
+    if (st != SPFFT_OK) return st;
+    spfft::CscBuildArgs a{};
+    a.R = p.R;
+    a.C = p.C;
+    a.nnz = nnz;
+    a.packed = p.packed;
+    a.row = d_row;
+    a.col = d_col;
+    a.counts = p.counts.as<int>();
+    a.colstart = p.colstart.as<int>();
+    a.sparse_cols = p.sparse_cols.p;
+    a.col_ptr = p.col_ptr.as<int>();
+    a.row_idx = p.row_idx.p;
+    a.totals = p.totals.as<int>();
+    a.err = p.err.as<int>();
+    a.scan_tmp = p.scan_tmp.as<int2>();
+    a.hist = p.hist.as<int>();
+    a.hist_off = p.hist_off.as<int>();
+    a.keys[0] = p.keys0.as<int>();
+    a.keys[1] = p.keys1.as<int>();
+    a.vals[0] = p.vals0.as<int>();
+    a.vals[1] = p.vals1.as<int>();
+    hipEvent_t* ev = nullptr;
+    if (p.profiling) {
+        const size_t at = p.ev_used;
+        ev = p.take_events(2);
+        if (!ev) return fail(SPFFT_ERR_HIP, "hipEventCreate failed");
+        p.ev_groups.push_back({0, at});
+        SPFFT_HIP(hipEventRecord(ev[0], s), "hipEventRecord");
+    }
+    SPFFT_HIP(spfft::launch_csc_build(a, s), "csc build launch");
+    if (ev) SPFFT_HIP(hipEventRecord(ev[1], s), "hipEventRecord");
+    p.last_nnz = nnz;
+    p.have_csc = true;
+    return SPFFT_OK;
+}
+
+spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t s) {
+    if (!p.have_csc) return fail(SPFFT_ERR_INVALID_ARG, "no CSC built yet");
+    if (!d_out) return fail(SPFFT_ERR_INVALID_ARG, "NULL output pointer");
+    if (pitch < p.V) return fail(SPFFT_ERR_INVALID_ARG, "output pitch smaller than cols/2+1");
+    RowPassArgs a = p.base;
+    a.P = (int)pitch;
+    a.colstart = p.colstart.as<int>();
+    a.row_idx = p.row_idx.p;
+    a.out = static_cast<float2*>(d_out);
+    const int tile = p.opts.u_tile;
+    for (int u0 = 0; u0 < p.R; u0 += tile) {
+        a.u_base = u0;
+        a.tile_rows = std::min(tile, p.R - u0);
+        hipEvent_t* ev = nullptr;
+        if (p.profiling) {
+            const size_t at = p.ev_used;
+            ev = p.take_events(4);
+            if (!ev) return fail(SPFFT_ERR_HIP, "hipEventCreate failed");
+            p.ev_groups.push_back({1, at});
+        }
+        SPFFT_HIP(spfft::launch_rowpass_tile(a, p.packed, s, ev), "row pass launch");
+    }
+    return SPFFT_OK;
+}
+
+bool check_host_coo(int rows, int cols, int nnz, const int* r, const int* c) {
+    for (int i = 0; i < nnz; i++)
+        if (r[i] < 0 || r[i] >= rows || c[i] < 0 || c[i] >= cols) return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* spfft_last_error(void) { return g_last_error.c_str(); }
+const char* spfft_version(void) { return "spfft-mi355x 0.1 (gfx950)"; }
+
+void spfft_opts_default(spfft_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->length_rule = SPFFT_SMOOTH_3;
+    o->u_tile = 128;
+    o->precision = SPFFT_PREC_FP32;
+}
+
+int spfft_next_3_smooth(int n) {
+    static const int p[] = {2, 3};
+    return next_smooth(n, p, 2);
+}
+
+int spfft_next_7_smooth(int n) {
+    static const int p[] = {2, 3, 5, 7};
+    return next_smooth(n, p, 4);
+}
+
+int spfft_factor_3_smooth(int n, int* out, int cap) {
+    int c3 = 0, k = 0;
+    if (n < 1) return -1;
+    while (n % 3 == 0) { c3++; n /= 3; }
+    for (int i = 0; i < c3 / 2; i++) { if (out && k < cap) out[k] = 9; k++; }
+    if (c3 % 2) { if (out && k < cap) out[k] = 3; k++; }
+    while (n % 2 == 0) { if (out && k < cap) out[k] = 2; k++; n /= 2; }
+    return n == 1 ? k : -1;
+}
+
+spfft_status spfft_plan_create(int rows, int cols, const spfft_opts* opts_in, spfft_plan* out) {
+    if (!out) return fail(SPFFT_ERR_INVALID_ARG, "NULL plan pointer");
+    *out = nullptr;
+    spfft_opts o;
+    if (opts_in) o = *opts_in; else spfft_opts_default(&o);
+    if (rows <= 0 || cols <= 0) return fail(SPFFT_ERR_INVALID_ARG, "rows and cols must be positive");
+    if (o.u_tile <= 0) return fail(SPFFT_ERR_INVALID_ARG, "CSC tile size must be positive");
+    if (o.precision != SPFFT_PREC_FP32) return fail(SPFFT_ERR_UNSUPPORTED, "unknown precision form");
+    if (rows > (1 << 22)) return fail(SPFFT_ERR_UNSUPPORTED, "rows > 2^22 not supported");
+    if (cols > (1 << 20)) return fail(SPFFT_ERR_UNSUPPORTED, "cols > 2^20 not supported");
+    int M;
+    if (o.fft_len > 0) {
+        if (o.fft_len < 2 * cols - 1 || !is_7_smooth(o.fft_len))
+            return fail(SPFFT_ERR_INVALID_ARG, "fft_len must be 7-smooth and >= 2*cols-1");
+        M = o.fft_len;
+    } else if (o.length_rule == SPFFT_SMOOTH_3) {
+        M = spfft_next_3_smooth(2 * cols - 1);
+    } else if (o.length_rule == SPFFT_SMOOTH_7) {
+        M = spfft_next_7_smooth(2 * cols - 1);
+    } else {
+        return fail(SPFFT_ERR_INVALID_ARG, "length_rule must be 3 or 7");
+    }
+    spfft_plan_s* p = new (std::nothrow) spfft_plan_s();
+    if (!p) return fail(SPFFT_ERR_ALLOC, "host allocation failed");
+    std::unique_ptr<spfft_plan_s> guard(p);
+    p->R = rows;
+    p->C = cols;
+    p->V = cols / 2 + 1;
+    p->P = (p->V + 3) / 4 * 4;
+    p->M = M;
+    p->opts = o;
+    p->packed = rows <= 65536 && cols <= 65536;
+    spfft_status st = choose_split(*p, o.m1);
+    if (st != SPFFT_OK) return st;
+    size_t budget = 0;
+    st = ensure_lds_limit(budget);
+    if (st != SPFFT_OK) return st;
+    RowPassArgs& a = p->base;
+    a.R = rows;
+    a.C = cols;
+    a.V = p->V;
+    a.P = p->P;
+    a.M = M;
+    a.M1 = p->M1;
+    a.M2 = p->M2;
+    int hs = 8;
+    while ((1LL << (2 * hs)) < rows) hs++;
+    a.hi_shift = hs;
+    a.n_hi = (rows + (1 << hs) - 1) >> hs;
+    a.r_magic = ~0ULL / (uint64_t)rows + 1;
+    st = choose_tiling(*p, budget);
+    if (st != SPFFT_OK) return st;
+
+    // ---- tables (FP64 -> FP32) ----
+    std::vector<float2> h(cols), lo(1 << hs), hi(a.n_hi), t1(p->M1), t2(p->M2), tm((size_t)M), bh((size_t)M);
+    for (int k = 0; k < cols; k++) {
+        const long long k2 = (long long)k * k % (2LL * cols);
+        h[k] = to_f2(unit_root(k2, 2LL * cols, -1));  // exp(-i pi k^2 / C)
+    }
+    for (int l = 0; l < (1 << hs); l++) lo[l] = to_f2(unit_root(l, rows, -1));
+    for (int j = 0; j < a.n_hi; j++) hi[j] = to_f2(unit_root((long long)j << hs, rows, -1));
+    for (int i = 0; i < p->M1; i++) t1[i] = to_f2(unit_root(i, p->M1, -1));
+    for (int i = 0; i < p->M2; i++) t2[i] = to_f2(unit_root(i, p->M2, -1));
+    for (int k1 = 0; k1 < p->M1; k1++)
+        for (int n2 = 0; n2 < p->M2; n2++)
+            tm[(size_t)k1 * p->M2 + n2] = to_f2(unit_root((long long)k1 * n2, M, -1));
+    {
+        // b[0] = 1, b[k] = b[M-k] = exp(+i pi k^2 / C) for 1 <= k < C
+        std::vector<cd> b((size_t)M, cd(0, 0));
+        b[0] = cd(1, 0);
+        for (int k = 1; k < cols; k++) {
+            const long long k2 = (long long)k * k % (2LL * cols);
+            const cd z = unit_root(k2, 2LL * cols, +1);
+            b[k] = z;
+            b[M - k] = z;
+        }
+        fft64(b, -1);
+        const double inv_m = 1.0 / (double)M;
+        for (int k1 = 0; k1 < p->M1; k1++)
+            for (int k2 = 0; k2 < p->M2; k2++)
+                bh[(size_t)k1 * p->M2 + k2] = to_f2(b[(size_t)k1 + (size_t)p->M1 * k2] * inv_m);
+    }
+    SPFFT_HIP(upload(p->h, h), "upload chirp");
+    SPFFT_HIP(upload(p->wr_lo, lo), "upload twiddles");
+    SPFFT_HIP(upload(p->wr_hi, hi), "upload twiddles");
+    SPFFT_HIP(upload(p->tw1, t1), "upload twiddles");
+    SPFFT_HIP(upload(p->tw2, t2), "upload twiddles");
+    SPFFT_HIP(upload(p->twM, tm), "upload twiddles");
+    SPFFT_HIP(upload(p->bhat, bh), "upload B^");
+    // one tile is at most 65535 workgroup rows of the pass-1 grid
+    p->opts.u_tile = std::min(o.u_tile, 65535);
+    const int tile_rows = std::min(p->opts.u_tile, rows);
+    SPFFT_HIP(p->T.alloc((size_t)tile_rows * M * sizeof(float2)), "hipMalloc(tile buffer)");
+    SPFFT_HIP(p->counts.alloc((size_t)(cols + 1) * 4), "hipMalloc(counts)");
+    SPFFT_HIP(p->colstart.alloc((size_t)(cols + 1) * 4), "hipMalloc(colstart)");
+    SPFFT_HIP(p->sparse_cols.alloc((size_t)cols * (p->packed ? 2 : 4)), "hipMalloc(sparse_cols)");
+    SPFFT_HIP(p->col_ptr.alloc((size_t)(cols + 1) * 4), "hipMalloc(col_ptr)");
+    SPFFT_HIP(p->totals.alloc(2 * 4), "hipMalloc(totals)");
+    SPFFT_HIP(p->err.alloc(4), "hipMalloc(err)");
+    a.h = p->h.as<float2>();
+    a.wr_hi = p->wr_hi.as<float2>();
+    a.wr_lo = p->wr_lo.as<float2>();
+    a.tw1 = p->tw1.as<float2>();
+    a.tw2 = p->tw2.as<float2>();
+    a.twM = p->twM.as<float2>();
+    a.bhat = p->bhat.as<float2>();
+    a.T = p->T.as<float2>();
+    p->table_bytes = p->h.bytes + p->wr_lo.bytes + p->wr_hi.bytes + p->tw1.bytes + p->tw2.bytes +
+                     p->twM.bytes + p->bhat.bytes + p->T.bytes;
+    *out = guard.release();
+    return SPFFT_OK;
+}
+
+spfft_status spfft_plan_destroy(spfft_plan p) {
+    delete p;
+    return SPFFT_OK;
+}
+
+spfft_status spfft_plan_get_info(spfft_plan p, spfft_plan_info* info) {
+    if (!p || !info) return fail(SPFFT_ERR_INVALID_ARG, "NULL argument");
+    info->rows = p->R;
+    info->cols = p->C;
+    info->out_cols = p->V;
+    info->pitch = p->P;
+    info->fft_len = p->M;
+    info->m1 = p->M1;
+    info->m2 = p->M2;
+    info->packed = p->packed ? 1 : 0;
+    info->u_tile = p->opts.u_tile;
+    info->rows_per_wg = p->base.U;
+    info->slab = p->base.J;
+    info->conv_rows = p->base.G2;
+    info->workspace_bytes = p->table_bytes;
+    return SPFFT_OK;
+}
+
+spfft_status spfft_csc_build(spfft_plan p, const int* d_row, const int* d_col, int nnz, void* stream) {
+    if (!p) return fail(SPFFT_ERR_INVALID_ARG, "NULL plan");
+    return do_csc_build(*p, d_row, d_col, nnz, (hipStream_t)stream);
+}
+
+spfft_status spfft_check_input(spfft_plan p, void* stream) {
+    if (!p) return fail(SPFFT_ERR_INVALID_ARG, "NULL plan");
+    if (!p->have_csc) return fail(SPFFT_ERR_INVALID_ARG, "no CSC built yet");
+    SPFFT_HIP(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+    int err = 0;
+    SPFFT_HIP(hipMemcpy(&err, p->err.p, 4, hipMemcpyDeviceToHost), "hipMemcpy(err)");
+    if (err) return fail(SPFFT_ERR_BAD_INPUT, "COO index outside [0, rows) x [0, cols)");
+    return SPFFT_OK;
+}
+
+spfft_status spfft_csc_get(spfft_plan p, void* stream, spfft_csc_view* v) {
+    if (!v) return fail(SPFFT_ERR_INVALID_ARG, "NULL view");
+    spfft_status st = spfft_check_input(p, stream);
+    if (st != SPFFT_OK) return st;
+    int tot[2] = {0, 0};
+    SPFFT_HIP(hipMemcpy(tot, p->totals.p, 8, hipMemcpyDeviceToHost), "hipMemcpy(totals)");
+    v->nnz = tot[0];
+    v->n_sparse_cols = tot[1];
+    v->packed = p->packed ? 1 : 0;
+    v->d_sparse_cols = p->sparse_cols.p;
+    v->d_col_ptr = p->col_ptr.as<int>();
+    v->d_row_idx = p->row_idx.p;
+    v->d_col_start = p->colstart.as<int>();
+    return SPFFT_OK;
+}
+
+spfft_status spfft_rowpass(spfft_plan p, void* d_out, int64_t pitch, void* stream) {
+    if (!p) return fail(SPFFT_ERR_INVALID_ARG, "NULL plan");
+    return do_rowpass(*p, d_out, pitch, (hipStream_t)stream);
+}
+
+spfft_status spfft_execute(spfft_plan p, const int* d_row, const int* d_col, int nnz, void* d_out,
+                           int64_t pitch, void* stream) {
+    if (!p) return fail(SPFFT_ERR_INVALID_ARG, "NULL plan");
+    spfft_status st = do_csc_build(*p, d_row, d_col, nnz, (hipStream_t)stream);
+    if (st != SPFFT_OK) return st;
+    return do_rowpass(*p, d_out, pitch, (hipStream_t)stream);
+}
+
+spfft_status spfft_profile_enable(spfft_plan p, int on) {
+    if (!p) return fail(SPFFT_ERR_INVALID_ARG, "NULL plan");
+    p->profiling = on != 0;
+    p->ev_used = 0;
+    p->ev_groups.clear();
+    return SPFFT_OK;
+}
+
+spfft_status spfft_profile_read(spfft_plan p, void* stream, spfft_profile* out) {
+    if (!p || !out) return fail(SPFFT_ERR_INVALID_ARG, "NULL argument");
+    std::memset(out, 0, sizeof(*out));
+    SPFFT_HIP(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+    for (const auto& g : p->ev_groups) {
+        hipEvent_t* ev = &p->ev_pool[g.second];
+        float ms = 0.f;
+        if (g.first == 0) {
+            SPFFT_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]), "hipEventElapsedTime");
+            out->ms[0] += ms;
+            out->launches[0] += 1;
+        } else {
+            for (int k = 0; k < 3; k++) {
+                SPFFT_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]), "hipEventElapsedTime");
+                out->ms[1 + k] += ms;
+                out->launches[1 + k] += 1;
+            }
+        }
+    }
+    p->ev_used = 0;
+    p->ev_groups.clear();
+    return SPFFT_OK;
+}
+
+spfft_status spfft_free_output(void* d) {
+    if (d) SPFFT_HIP(hipFree(d), "hipFree");
+    return SPFFT_OK;
+}
+
+spfft_status spfft_run_host_coo(int rows, int cols, int nnz, const int* h_row, const int* h_col,
+                                const spfft_opts* opts, spfft_result* res) {
+    if (!res) return fail(SPFFT_ERR_INVALID_ARG, "NULL result");
+    std::memset(res, 0, sizeof(*res));
+    spfft_opts o;
+    if (opts) o = *opts; else spfft_opts_default(&o);
+    if (o.u_tile <= 0) return fail(SPFFT_ERR_INVALID_ARG, "CSC tile size must be positive");
+    if (rows <= 0 || cols <= 0 || nnz < 0) return fail(SPFFT_ERR_INVALID_ARG, "bad matrix shape");
+    if (nnz > 0 && (!h_row || !h_col)) return fail(SPFFT_ERR_INVALID_ARG, "NULL COO pointer");
+    if (!check_host_coo(rows, cols, nnz, h_row, h_col))
+        return fail(SPFFT_ERR_BAD_INPUT, "COO index outside [0, rows) x [0, cols)");
+    size_t free0 = 0, total = 0;
+    SPFFT_HIP(hipMemGetInfo(&free0, &total), "hipMemGetInfo");
+    spfft_plan p = nullptr;
+    spfft_status st = spfft_plan_create(rows, cols, &o, &p);
+    if (st != SPFFT_OK) return st;
+    std::unique_ptr<spfft_plan_s> pg(p);
+    DevBuf drow, dcol, dout;
+    SPFFT_HIP(drow.alloc((size_t)nnz * 4), "hipMalloc(COO)");
+    SPFFT_HIP(dcol.alloc((size_t)nnz * 4), "hipMalloc(COO)");
+    if (nnz > 0) {
+        SPFFT_HIP(hipMemcpy(drow.p, h_row, (size_t)nnz * 4, hipMemcpyHostToDevice), "hipMemcpy(COO)");
+        SPFFT_HIP(hipMemcpy(dcol.p, h_col, (size_t)nnz * 4, hipMemcpyHostToDevice), "hipMemcpy(COO)");
+    }
+    SPFFT_HIP(dout.alloc((size_t)rows * p->P * sizeof(float2)), "hipMalloc(output)");
+    st = ensure_csc_capacity(*p, nnz);
+    if (st != SPFFT_OK) return st;
+    SPFFT_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    hipEvent_t e0, e1, e2;
+    SPFFT_HIP(hipEventCreate(&e0), "hipEventCreate");
+    SPFFT_HIP(hipEventCreate(&e1), "hipEventCreate");
+    SPFFT_HIP(hipEventCreate(&e2), "hipEventCreate");
+    struct EvGuard {
+        hipEvent_t a, b, c;
+        ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); (void)hipEventDestroy(c); }
+    } evg{e0, e1, e2};
+    SPFFT_HIP(hipEventRecord(e0, 0), "hipEventRecord");
+    st = do_csc_build(*p, drow.as<int>(), dcol.as<int>(), nnz, 0);
+    if (st != SPFFT_OK) return st;
+    SPFFT_HIP(hipEventRecord(e1, 0), "hipEventRecord");
+    st = do_rowpass(*p, dout.p, p->P, 0);
+    if (st != SPFFT_OK) return st;
+    SPFFT_HIP(hipEventRecord(e2, 0), "hipEventRecord");
+    SPFFT_HIP(hipEventSynchronize(e2), "hipEventSynchronize");
+    float ms_loop = 0.f, ms_all = 0.f;
+    SPFFT_HIP(hipEventElapsedTime(&ms_loop, e1, e2), "hipEventElapsedTime");
+    SPFFT_HIP(hipEventElapsedTime(&ms_all, e0, e2), "hipEventElapsedTime");
+    st = spfft_check_input(p, 0);
+    if (st != SPFFT_OK) return st;
+    size_t free1 = 0;
+    SPFFT_HIP(hipMemGetInfo(&free1, &total), "hipMemGetInfo");
+    res->d_output = dout.p;
+    dout.p = nullptr;  // ownership passes to the caller
+    res->output_cols = p->P;
+    res->ms = ms_loop;
+    res->ms_e2e = ms_all;
+    res->mem_bytes = free1 < free0 ? free0 - free1 : 0;
+    return SPFFT_OK;
+}
+
+spfft_status spfft_csc_bluestein_mixed_radix(int rows, int cols, int nnz, const int* h_row,
+                                             const int* h_col, int u_tile, spfft_result* res) {
+    spfft_opts o;
+    spfft_opts_default(&o);
+    o.length_rule = SPFFT_SMOOTH_3;
+    o.u_tile = u_tile;
+    return spfft_run_host_coo(rows, cols, nnz, h_row, h_col, &o, res);
+}
+
+spfft_status spfft_csc_bluestein_cufft_smooth(int rows, int cols, int nnz, const int* h_row,
+                                              const int* h_col, int u_tile, spfft_result* res) {
+    spfft_opts o;
+    spfft_opts_default(&o);
+    o.length_rule = SPFFT_SMOOTH_7;
+    o.u_tile = u_tile;
+    return spfft_run_host_coo(rows, cols, nnz, h_row, h_col, &o, res);
+}
+
+}  // extern "C"

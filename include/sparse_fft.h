@@ -1,0 +1,25 @@
+// sparse_fft.h -- drop-in C++ entry points for the two reported variants of the
+// reference (include/sparse_fft.h:7-13, 57, 70), implemented over the C ABI in
+// spfft_c.h and hand-written gfx950 kernels.
+#pragma once
+#include <hip/hip_complex.h>
+#include <vector>
+
+#include "mtx_reader.h"
+
+struct SparseFFTResult {
+    hipFloatComplex* d_output;  // device pointer, rows * output_cols; caller frees with hipFree
+    int output_cols;            // row pitch P = 4 * ceil((cols/2 + 1) / 4)
+    float ms;                   // GPU time of the tile loop (the reference's timed region)
+    size_t mem_bytes;           // device free-memory delta over the call
+    std::vector<hipFloatComplex> h_output;  // unused (no streamed variant)
+};
+
+// Bluestein length next_3_smooth(2*cols - 1).  Replaces
+// sparse_fft_csc_bluestein_mixed_radix (reference src/sparse_fft_bluestein.cu:2256-2392).
+SparseFFTResult sparse_fft_csc_bluestein_mixed_radix(const COOMatrix& coo, int u_tile = 128);
+
+// Bluestein length next_7_smooth(2*cols - 1), hand-written radix-2/3/4/5/7/8
+// sub-FFTs in place of cuFFT.  Replaces sparse_fft_csc_bluestein_cufft_smooth
+// (reference src/sparse_fft_bluestein.cu:2557-2719).
+SparseFFTResult sparse_fft_csc_bluestein_cufft_smooth(const COOMatrix& coo, int u_tile = 128);

@@ -1,0 +1,174 @@
+/* spfft_c.h -- C ABI of the MI355X (gfx950) binary-sparse 2-D R2C FFT.
+ *
+ * Plain C types only (pointers, ints, sizes); device pointers are HIP device
+ * addresses and `stream` is a hipStream_t passed as void* (NULL = the legacy
+ * default stream).  Every function returns an spfft_status; on failure
+ * spfft_last_error() returns a message for the calling thread.
+ *
+ * Output layout (all entry points): X[u][v] = sum over nonzeros (r, c) of
+ * exp(-2 pi i (u r / rows + v c / cols)), u < rows, v <= cols/2, complex64
+ * row-major with pitch P = 4 * ceil((cols/2 + 1) / 4) elements -- the layout
+ * of the reference's SparseFFTResult (src/sparse_fft_bluestein.cu:2259-2260,
+ * :2391).  Padding columns v in [cols/2+1, P) are not written.
+ */
+#ifndef SPFFT_C_H
+#define SPFFT_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum spfft_status {
+    SPFFT_OK = 0,
+    SPFFT_ERR_INVALID_ARG = 1, /* bad size, tile, option or NULL pointer       */
+    SPFFT_ERR_HIP = 2,         /* a HIP runtime call failed                    */
+    SPFFT_ERR_ALLOC = 3,       /* device or host allocation failed             */
+    SPFFT_ERR_UNSUPPORTED = 4, /* shape outside what the kernels support       */
+    SPFFT_ERR_BAD_INPUT = 5    /* COO index outside [0, rows) x [0, cols)      */
+} spfft_status;
+
+/* Bluestein length rule.  SMOOTH_3: M = next 2^a 3^b >= 2*cols-1 (the
+ * reference's --csc-mixed-radix, src/sparse_fft_bluestein.cu:2261).
+ * SMOOTH_7: M = next 2^a 3^b 5^c 7^d >= 2*cols-1 (--csc-cufft-smooth, :2562). */
+enum { SPFFT_SMOOTH_3 = 3, SPFFT_SMOOTH_7 = 7 };
+
+/* Arithmetic form of the row pass. */
+enum { SPFFT_PREC_FP32 = 0 };
+
+typedef struct spfft_opts {
+    int length_rule; /* SPFFT_SMOOTH_3 or SPFFT_SMOOTH_7                      */
+    int u_tile;      /* output rows per tile (reference --csc-tile), > 0       */
+    int precision;   /* SPFFT_PREC_*                                           */
+    int fft_len;     /* 0: from length_rule; else an explicit 7-smooth M >= 2*cols-1 */
+    int m1;          /* 0: automatic; else the first factor of M = m1 * m2     */
+    int rows_per_wg; /* 0: automatic; else 1, 2, 4 or 8 (pass-1 rows per workgroup) */
+    int slab;        /* 0: automatic; else a power of two <= 64 (n2 columns per workgroup) */
+    int conv_rows;   /* 0: automatic; else rows per workgroup of the convolution kernel */
+} spfft_opts;
+
+typedef struct spfft_plan_info {
+    int rows, cols;
+    int out_cols;      /* cols/2 + 1                                   */
+    int pitch;         /* P, elements                                  */
+    int fft_len;       /* M                                            */
+    int m1, m2;        /* M = m1 * m2                                  */
+    int packed;        /* 1 if CSC indices are stored as uint16        */
+    int u_tile;
+    int rows_per_wg, slab, conv_rows;
+    size_t workspace_bytes; /* device bytes held by the plan (tables + tile buffer) */
+} spfft_plan_info;
+
+typedef struct spfft_plan_s* spfft_plan;
+
+/* Result of the reference-shaped one-shot entry points (mirrors SparseFFTResult,
+ * include/sparse_fft.h:7-13).  d_output is owned by the caller: release it with
+ * spfft_free_output().  ms = GPU time of the tile loop only (the reference's
+ * timed region, src/sparse_fft_bluestein.cu:2322-2377); ms_e2e = GPU time of
+ * the CSC build plus the tile loop. */
+typedef struct spfft_result {
+    void* d_output;
+    int output_cols; /* pitch P */
+    float ms;
+    float ms_e2e;
+    size_t mem_bytes; /* device free-memory delta over the call */
+} spfft_result;
+
+/* Device view of the compact CSC built by the last execute / csc_build call
+ * (the reference's CompactCSC after packing, include/sparse_fft_internal.cuh:
+ * 193-222 and :146-151). */
+typedef struct spfft_csc_view {
+    int n_sparse_cols;
+    int nnz;
+    int packed;                /* 1: uint16 arrays, 0: int32 arrays  */
+    const void* d_sparse_cols; /* n_sparse_cols entries               */
+    const int* d_col_ptr;      /* n_sparse_cols + 1 entries           */
+    const void* d_row_idx;     /* nnz entries                         */
+    const int* d_col_start;    /* cols + 1 dense column starts        */
+} spfft_csc_view;
+
+void spfft_opts_default(spfft_opts* opts);
+const char* spfft_last_error(void);
+const char* spfft_version(void);
+
+/* Planning: tables (chirp, B^, twiddles) in FP64 rounded to FP32, device
+ * workspace for one tile. */
+spfft_status spfft_plan_create(int rows, int cols, const spfft_opts* opts, spfft_plan* plan);
+spfft_status spfft_plan_destroy(spfft_plan plan);
+spfft_status spfft_plan_get_info(spfft_plan plan, spfft_plan_info* info);
+
+/* Stage 1 only: device COO (int32 rows, cols) -> compact CSC held by the plan. */
+spfft_status spfft_csc_build(spfft_plan plan, const int* d_row, const int* d_col, int nnz, void* stream);
+/* Synchronises `stream`, then fills `view` (and checks the input range flag). */
+spfft_status spfft_csc_get(spfft_plan plan, void* stream, spfft_csc_view* view);
+/* Stages 2 + 3 on the CSC of the last spfft_csc_build: writes d_out (pitch in elements). */
+spfft_status spfft_rowpass(spfft_plan plan, void* d_out, int64_t out_pitch, void* stream);
+/* All three stages, asynchronous on `stream`.  Inputs and output in device memory. */
+spfft_status spfft_execute(spfft_plan plan, const int* d_row, const int* d_col, int nnz, void* d_out,
+                           int64_t out_pitch, void* stream);
+/* Synchronises `stream` and reports SPFFT_ERR_BAD_INPUT if the last CSC build
+ * saw an index outside the matrix. */
+spfft_status spfft_check_input(spfft_plan plan, void* stream);
+
+/* Per-kernel GPU time from HIP events recorded on the launch stream around
+ * the CSC build and around each row-pass kernel of every tile (the role of the
+ * reference's CUSPFFT_BREAKDOWN timing, src/sparse_fft_bluestein.cu:1752-1760).
+ * Index 0 = CSC build, 1 = pass 1 + first sub-FFTs, 2 = convolution rows,
+ * 3 = inverse sub-FFTs + finalize.  spfft_profile_read synchronises `stream`,
+ * returns the sums since the last read and resets them. */
+typedef struct spfft_profile {
+    double ms[4];
+    int launches[4];
+} spfft_profile;
+spfft_status spfft_profile_enable(spfft_plan plan, int on);
+spfft_status spfft_profile_read(spfft_plan plan, void* stream, spfft_profile* out);
+
+/* Reference-shaped synchronous entry points: host COO in, device output out.
+ *   spfft_csc_bluestein_mixed_radix   replaces sparse_fft_csc_bluestein_mixed_radix
+ *                                     (include/sparse_fft.h:57, src/sparse_fft_bluestein.cu:2256-2392)
+ *   spfft_csc_bluestein_cufft_smooth  replaces sparse_fft_csc_bluestein_cufft_smooth
+ *                                     (include/sparse_fft.h:70, src/sparse_fft_bluestein.cu:2557-2719)
+ * u_tile <= 0 is rejected with SPFFT_ERR_INVALID_ARG ("CSC tile size must be
+ * positive", as the reference throws at :2264-2265). */
+spfft_status spfft_csc_bluestein_mixed_radix(int rows, int cols, int nnz, const int* h_row,
+                                             const int* h_col, int u_tile, spfft_result* result);
+spfft_status spfft_csc_bluestein_cufft_smooth(int rows, int cols, int nnz, const int* h_row,
+                                              const int* h_col, int u_tile, spfft_result* result);
+/* General form of the two above (any opts). */
+spfft_status spfft_run_host_coo(int rows, int cols, int nnz, const int* h_row, const int* h_col,
+                                const spfft_opts* opts, spfft_result* result);
+spfft_status spfft_free_output(void* d_output);
+
+/* Smooth lengths and the radix order of the reference planner
+ * (src/sparse_fft_bluestein.cu:1975-2011). */
+int spfft_next_3_smooth(int n);
+int spfft_next_7_smooth(int n);
+/* Writes the factor_3_smooth radix order into out (cap entries); returns the
+ * count, or -1 if n is not 3-smooth. */
+int spfft_factor_3_smooth(int n, int* out, int cap);
+
+/* Seeded synthetic binary matrices (host memory, canonical COO: deduplicated,
+ * sorted by (row, col), zero diagonal, symmetric).  Release with spfft_coo_free. */
+typedef struct spfft_host_coo {
+    int rows, cols, nnz;
+    int* row_idx;
+    int* col_idx;
+} spfft_host_coo;
+
+/* kind: "uniform" (p0 = number of distinct i<j pairs),
+ *       "banded"  (p0 = target nnz, p1 = decay scale, p2 = band cutoff, p3 = off-band fraction),
+ *       "chunglu" (p0 = target nnz, p1 = degree exponent, p2 = max expected degree, 0 = sqrt(target)). */
+spfft_status spfft_coo_generate(const char* kind, int n, double p0, double p1, double p2, double p3,
+                                uint64_t seed, spfft_host_coo* out);
+void spfft_coo_free(spfft_host_coo* coo);
+/* Matrix Market reader with the reference's semantics (src/mtx_reader.cu:7-77):
+ * symmetric entries mirrored, 1-based -> 0-based, duplicates removed, sorted by (row, col). */
+spfft_status spfft_read_mtx(const char* path, spfft_host_coo* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPFFT_C_H */

@@ -1,0 +1,259 @@
+"""GPU parity tests of the hot path, through the C ABI (cuspfft_amd -> libspfft_hip.so).
+
+Bars:
+  * CSC build: bit-exact against the oracle's restatement of make_compact_csc
+    (reference include/sparse_fft_internal.cuh:200-222) and its uint16 packing.
+  * Plain FP32 transform: relative L2 error <= 9e-6 against the FP64 oracle.
+    FFTW single precision is ~1e-7 from exact, so by the triangle inequality
+    this keeps the plain form within 1e-5 of FFTW's output.
+  * Full-size configurations: spot rows against the FP64 oracle plus
+    size-independent invariants (X[0][0] = nnz, Parseval, X[u][v] = X[v][u]
+    for symmetric inputs).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REL_L2_PLAIN = 9e-6
+
+
+def _dev(a):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a, np.int32)).cuda()
+
+
+def _run(sp, rows, cols, r, c, **kw):
+    import torch
+    plan = sp.Plan(rows, cols, **kw)
+    out = plan.alloc_output()
+    plan.execute(_dev(r), _dev(c), out)
+    torch.cuda.synchronize()
+    plan.check_input()
+    return plan, out[:, : cols // 2 + 1].cpu().numpy()
+
+
+def _random_coo(rng, rows, cols, nnz, shuffle=True):
+    nnz = min(nnz, rows * cols)
+    idx = rng.choice(rows * cols, nnz, replace=False)
+    if not shuffle:
+        idx = np.sort(idx)
+    return (idx // cols).astype(np.int32), (idx % cols).astype(np.int32)
+
+
+def _csc_case(sp, oracle_mod, rows, cols, r, c):
+    import torch
+    plan = sp.Plan(rows, cols)
+    plan.csc_build(_dev(r), _dev(c))
+    got = plan.csc_get()
+    sc, cp, ri = oracle_mod.compact_csc(rows, cols, r, c)
+    assert got["packed"] == oracle_mod.can_pack_u16(rows, cols)
+    assert got["n_sparse_cols"] == len(sc)
+    np.testing.assert_array_equal(got["sparse_cols"].astype(np.int64), sc)
+    np.testing.assert_array_equal(got["col_ptr"], cp)
+    np.testing.assert_array_equal(got["row_idx"].astype(np.int64), ri)
+    counts = np.bincount(c, minlength=cols)
+    np.testing.assert_array_equal(got["col_start"], np.concatenate([[0], np.cumsum(counts)]))
+    del plan
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("case", ["c1", "c1_shuffled", "c3", "small_cols", "empty_cols",
+                                  "full_column", "single", "edge_65536", "int32_path"])
+def test_csc_bit_exact(gpu, oracle_mod, case):
+    sp = gpu
+    rng = np.random.default_rng(7)
+    print(f"[csc] case={case}", flush=True)
+    if case in ("c1", "c1_shuffled", "c3"):
+        m = sp.config_matrix("c3" if case == "c3" else "c1")
+        r, c = m.row_idx, m.col_idx
+        if case == "c1_shuffled":
+            p = rng.permutation(len(r))
+            r, c = r[p], c[p]
+        _csc_case(sp, oracle_mod, m.rows, m.cols, r, c)
+    elif case == "small_cols":
+        r, c = _random_coo(rng, 3000, 200, 20000)
+        _csc_case(sp, oracle_mod, 3000, 200, r, c)
+    elif case == "empty_cols":
+        r, c = _random_coo(rng, 100, 1000, 300)
+        c = (c // 7) * 7  # only every 7th column can be active
+        key = np.unique(r.astype(np.int64) * 1000 + c)
+        r, c = (key // 1000).astype(np.int32), (key % 1000).astype(np.int32)
+        p = rng.permutation(len(r))
+        _csc_case(sp, oracle_mod, 100, 1000, r[p], c[p])
+    elif case == "full_column":
+        rows, cols = 5000, 40
+        r = np.arange(rows, dtype=np.int32)[::-1].copy()
+        c = np.full(rows, 17, np.int32)
+        r2, c2 = _random_coo(rng, rows, cols, 3000)
+        keep = c2 != 17
+        r = np.concatenate([r, r2[keep]])
+        c = np.concatenate([c, c2[keep]])
+        p = rng.permutation(len(r))
+        _csc_case(sp, oracle_mod, rows, cols, r[p], c[p])
+    elif case == "single":
+        _csc_case(sp, oracle_mod, 9, 11, np.array([4], np.int32), np.array([10], np.int32))
+    elif case == "edge_65536":
+        rows, cols = 8, 65536
+        r, c = _random_coo(rng, rows, cols, 5000)
+        c[:3] = [0, 65535, 32768]
+        key = np.unique(r.astype(np.int64) * cols + c)
+        r, c = (key // cols).astype(np.int32), (key % cols).astype(np.int32)
+        _csc_case(sp, oracle_mod, rows, cols, r, c)
+    elif case == "int32_path":
+        rows, cols = 6, 70000
+        r, c = _random_coo(rng, rows, cols, 4000)
+        _csc_case(sp, oracle_mod, rows, cols, r, c)
+
+
+@pytest.mark.parametrize("rule", [3, 7])
+def test_c1_full_vs_fp64(gpu, oracle_mod, rule):
+    sp = gpu
+    m = sp.config_matrix("c1")
+    print(f"[fft] c1 rule={rule}", flush=True)
+    plan, X = _run(sp, m.rows, m.cols, m.row_idx, m.col_idx, length_rule=rule)
+    ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, nthreads=8)
+    err = oracle_mod.rel_l2(X, ref)
+    print(f"c1 rule={rule} M={plan.info['fft_len']} rel_l2={err:.3e}")
+    assert err <= REL_L2_PLAIN
+    assert abs(X[0, 0] - m.nnz) <= 1e-6 * m.nnz * 10
+
+
+SHAPES = [(1, 1, 1), (1, 7, 3), (7, 1, 4), (3, 5, 6), (17, 31, 100), (64, 64, 700),
+          (127, 128, 2000), (130, 257, 3000), (200, 3, 250), (33, 1000, 5000)]
+
+
+@pytest.mark.parametrize("rows,cols,nnz", SHAPES)
+@pytest.mark.parametrize("tile", [1, 7, 128])
+def test_small_shapes(gpu, oracle_mod, rows, cols, nnz, tile):
+    sp = gpu
+    rng = np.random.default_rng(rows * 1000 + cols)
+    r, c = _random_coo(rng, rows, cols, nnz)
+    print(f"[fft] shape {rows}x{cols} nnz={len(r)} tile={tile}", flush=True)
+    for rule in (3, 7):
+        _, X = _run(sp, rows, cols, r, c, length_rule=rule, u_tile=tile)
+        ref = oracle_mod.sparse_rfft2_f64(rows, cols, r, c)
+        assert oracle_mod.rel_l2(X, ref) <= REL_L2_PLAIN, (rows, cols, rule)
+
+
+def test_empty_matrix(gpu):
+    sp = gpu
+    z = np.zeros(0, np.int32)
+    _, X = _run(sp, 50, 60, z, z)
+    assert np.all(X == 0)
+
+
+@pytest.mark.parametrize("rows,cols,r0,c0", [(2310, 2310, 17, 1999), (31, 8219, 30, 8218),
+                                             (1000, 999, 0, 0)])
+def test_single_nonzero_closed_form(gpu, rows, cols, r0, c0):
+    sp = gpu
+    _, X = _run(sp, rows, cols, np.array([r0], np.int32), np.array([c0], np.int32))
+    u = np.arange(rows)[:, None]
+    v = np.arange(cols // 2 + 1)[None, :]
+    ph = ((u * r0) % rows) / rows + ((v * c0) % cols) / cols
+    ref = np.exp(-2j * np.pi * ph)
+    assert np.abs(X - ref).max() < 2e-5
+
+
+def _invariants(X_dev, m):
+    """X[0][0] = nnz, half-spectrum Parseval, and symmetry X[u][v] = X[v][u]."""
+    import torch
+    V = m.cols // 2 + 1
+    Xv = X_dev[:, :V]
+    assert abs(complex(Xv[0, 0].item()) - m.nnz) <= 1e-5 * m.nnz
+    w = torch.full((V,), 2.0, dtype=torch.float64, device=Xv.device)
+    w[0] = 1.0
+    if m.cols % 2 == 0:
+        w[-1] = 1.0
+    pars = (Xv.abs().double() ** 2 * w[None, :]).sum().item()
+    expect = float(m.rows) * m.cols * m.nnz
+    assert abs(pars / expect - 1) < 1e-5, pars / expect
+    n = min(V, m.rows)
+    S = Xv[:n, :n]
+    sym = ((S - S.transpose(0, 1)).abs().double().norm() / S.abs().double().norm()).item()
+    assert sym < 1e-5, sym
+
+
+@pytest.mark.parametrize("cfg,rule", [("c2", 3), ("c2", 7), ("c3", 3), ("c3", 7)])
+def test_full_config_spot_rows_and_invariants(gpu, oracle_mod, cfg, rule):
+    import torch
+    sp = gpu
+    m = sp.config_matrix(cfg)
+    print(f"[fft] {cfg} rule={rule}", flush=True)
+    plan = sp.Plan(m.rows, m.cols, length_rule=rule, u_tile=256)
+    out = plan.alloc_output()
+    plan.execute(_dev(m.row_idx), _dev(m.col_idx), out)
+    torch.cuda.synchronize()
+    plan.check_input()
+    rng = np.random.default_rng(3)
+    rows = np.unique(np.concatenate([[0, 1, m.rows - 1], rng.choice(m.rows, 29, replace=False)]))
+    ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, u_list=rows, nthreads=8)
+    got = out[torch.as_tensor(rows).cuda(), : m.cols // 2 + 1].cpu().numpy()
+    err = oracle_mod.rel_l2(got, ref)
+    print(f"{cfg} rule={rule} M={plan.info['fft_len']} spot rel_l2={err:.3e}")
+    assert err <= REL_L2_PLAIN
+    _invariants(out, m)
+
+
+def test_tile_invariance(gpu):
+    import torch
+    sp = gpu
+    m = sp.config_matrix("c1")
+    outs = []
+    for tile in (1, 100, 2310, 5000):
+        _, X = _run(sp, m.rows, m.cols, m.row_idx, m.col_idx, u_tile=tile)
+        outs.append(X)
+    for X in outs[1:]:
+        np.testing.assert_array_equal(X, outs[0])
+
+
+def test_errors(gpu):
+    sp = gpu
+    m = sp.config_matrix("c1")
+    with pytest.raises(RuntimeError, match="CSC tile size must be positive"):
+        sp.sparse_fft_csc_bluestein_mixed_radix(m, 0)
+    with pytest.raises(sp.SpfftError) as ei:
+        sp.Plan(10, 10, u_tile=-3)
+    assert ei.value.status == 1
+    bad_r = np.array([0, 5, 10], np.int32)
+    bad_c = np.array([0, 1, 2], np.int32)
+    plan = sp.Plan(10, 10)
+    out = plan.alloc_output()
+    plan.execute(_dev(bad_r), _dev(bad_c), out)
+    with pytest.raises(sp.SpfftError) as ei:
+        plan.check_input()
+    assert ei.value.status == 5
+
+
+def test_reference_shaped_entry_points(gpu, oracle_mod):
+    """The one-shot C entry points: host COO in, caller-owned device output out."""
+    sp = gpu
+    L = sp.load()
+    m = sp.config_matrix("c1")
+    ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, nthreads=8)
+    for fn in (L.spfft_csc_bluestein_mixed_radix, L.spfft_csc_bluestein_cufft_smooth):
+        res = sp.Result()
+        st = fn(m.rows, m.cols, m.nnz, m.row_idx.ctypes.data, m.col_idx.ctypes.data, 128,
+                ctypes.byref(res))
+        assert st == 0, L.spfft_last_error()
+        assert res.output_cols == (m.cols // 2 + 1 + 3) // 4 * 4
+        assert res.ms > 0 and res.ms_e2e >= res.ms
+        X = sp.device_ptr_to_host(res.d_output, m.rows, res.output_cols)[:, : m.cols // 2 + 1]
+        assert L.spfft_free_output(res.d_output) == 0
+        assert oracle_mod.rel_l2(X, ref) <= REL_L2_PLAIN
+    res = sp.Result()
+    st = L.spfft_csc_bluestein_mixed_radix(m.rows, m.cols, m.nnz, m.row_idx.ctypes.data,
+                                           m.col_idx.ctypes.data, 0, ctypes.byref(res))
+    assert st == 1 and b"CSC tile size must be positive" in L.spfft_last_error()
+
+
+def test_python_mirror(gpu, oracle_mod):
+    sp = gpu
+    m = sp.config_matrix("c1")
+    r = sp.sparse_fft_csc_bluestein_cufft_smooth(m, 64)
+    X = r.d_output[:, : m.cols // 2 + 1].cpu().numpy()
+    ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, nthreads=8)
+    assert r.output_cols == (m.cols // 2 + 1 + 3) // 4 * 4
+    assert oracle_mod.rel_l2(X, ref) <= REL_L2_PLAIN
