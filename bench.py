@@ -37,7 +37,22 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows sampled by the CPU baseline")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration(mib: int):
+    """A plain device-to-device copy of known size, so that the FETCH_SIZE / WRITE_SIZE
+    counters of the profiled kernels can be scaled to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "launches": 3}), flush=True)
 
 
 def cpu_baseline(m, rule: int, rows_sample: int):
@@ -64,6 +79,9 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the row-tile sharding across GPUs is a design note, not implemented", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calib:
+        copy_calibration(args.copy_calib)
+        return
     import numpy as np
     import torch
 
@@ -81,7 +99,6 @@ def main():
     for _ in range(args.warmup):
         plan.execute(d_row, d_col, out)
     torch.cuda.synchronize()
-    plan.check_input()
 
     plan.profile(True)
     torch.cuda.synchronize()
@@ -92,6 +109,7 @@ def main():
     t1 = time.perf_counter()
     prof = plan.profile_read(stream)
     plan.profile(False)
+    plan.check_input()
 
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     value = 1000.0 / ms_per_step
