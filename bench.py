@@ -156,8 +156,8 @@ def main():
             "rows": m.rows, "cols": m.cols, "nnz": m.nnz,
             "form": "plain-fp32",
             "length_rule": args.rule, "fft_len": info["fft_len"], "m1": info["m1"],
-            "m2": info["m2"], "u_tile": info["u_tile"], "rows_per_wg": info["rows_per_wg"],
-            "slab": info["slab"], "conv_rows": info["conv_rows"],
+            "m2": info["m2"], "u_tile": info["u_tile"], "conv_rows": info["conv_rows"],
+            "conv_radix": info["conv_radix"],
         },
         "rel_l2_vs_fp64": rel,
         "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 5),

@@ -35,7 +35,6 @@ class SpfftError(RuntimeError):
 class Opts(ctypes.Structure):
     _fields_ = [("length_rule", ctypes.c_int), ("u_tile", ctypes.c_int),
                 ("precision", ctypes.c_int), ("fft_len", ctypes.c_int), ("m1", ctypes.c_int),
-                ("rows_per_wg", ctypes.c_int), ("slab", ctypes.c_int),
                 ("conv_rows", ctypes.c_int)]
 
 
@@ -43,8 +42,9 @@ class PlanInfo(ctypes.Structure):
     _fields_ = [("rows", ctypes.c_int), ("cols", ctypes.c_int), ("out_cols", ctypes.c_int),
                 ("pitch", ctypes.c_int), ("fft_len", ctypes.c_int), ("m1", ctypes.c_int),
                 ("m2", ctypes.c_int), ("packed", ctypes.c_int), ("u_tile", ctypes.c_int),
-                ("rows_per_wg", ctypes.c_int), ("slab", ctypes.c_int),
-                ("conv_rows", ctypes.c_int), ("workspace_bytes", ctypes.c_size_t)]
+                ("conv_rows", ctypes.c_int), ("conv_fast", ctypes.c_int),
+                ("conv_stages", ctypes.c_int), ("conv_radix", ctypes.c_int * 8),
+                ("workspace_bytes", ctypes.c_size_t)]
 
 
 class Result(ctypes.Structure):
@@ -60,10 +60,10 @@ class CscView(ctypes.Structure):
 
 
 class Profile(ctypes.Structure):
-    _fields_ = [("ms", ctypes.c_double * 4), ("launches", ctypes.c_int * 4)]
+    _fields_ = [("ms", ctypes.c_double * 5), ("launches", ctypes.c_int * 5)]
 
 
-PROFILE_STAGES = ("csc_build", "colpass_fft1", "conv_rows", "ifft1_finalize")
+PROFILE_STAGES = ("csc_build", "colpass", "fft1", "conv", "ifft1_finalize")
 
 
 class HostCoo(ctypes.Structure):
@@ -93,6 +93,7 @@ def load():
         "spfft_plan_create": (_i, [_i, _i, ctypes.POINTER(Opts), ctypes.POINTER(_vp)]),
         "spfft_plan_destroy": (_i, [_vp]),
         "spfft_plan_get_info": (_i, [_vp, ctypes.POINTER(PlanInfo)]),
+        "spfft_plan_describe": (_i, [_i, _i, ctypes.POINTER(Opts), ctypes.POINTER(PlanInfo)]),
         "spfft_csc_build": (_i, [_vp, _vp, _vp, _i, _vp]),
         "spfft_csc_get": (_i, [_vp, _vp, ctypes.POINTER(CscView)]),
         "spfft_rowpass": (_i, [_vp, _vp, ctypes.c_int64, _vp]),
@@ -223,23 +224,41 @@ def _stream_handle(stream=None):
     return ctypes.c_void_p(s.cuda_stream)
 
 
+def _opts(length_rule=SMOOTH_3, u_tile=128, fft_len=0, m1=0, conv_rows=0, precision=PREC_FP32):
+    o = Opts()
+    load().spfft_opts_default(ctypes.byref(o))
+    o.length_rule, o.u_tile, o.precision = length_rule, u_tile, precision
+    o.fft_len, o.m1, o.conv_rows = fft_len, m1, conv_rows
+    return o
+
+
+def _info_dict(inf: PlanInfo) -> dict:
+    d = {k: getattr(inf, k) for k, _ in PlanInfo._fields_}
+    d["conv_radix"] = list(inf.conv_radix)[: inf.conv_stages]
+    return d
+
+
+def describe_plan(rows: int, cols: int, **kw) -> dict:
+    """Host-only shape plan (M, M1 x M2 split, convolution kernel) -- no GPU needed."""
+    inf = PlanInfo()
+    _check(load().spfft_plan_describe(int(rows), int(cols), ctypes.byref(_opts(**kw)),
+                                      ctypes.byref(inf)))
+    return _info_dict(inf)
+
+
 class Plan:
     """A planned transform for one (rows, cols) shape.  Mirrors the set-up half
     of the reference drivers (src/sparse_fft_bluestein.cu:2256-2320)."""
 
     def __init__(self, rows: int, cols: int, length_rule: int = SMOOTH_3, u_tile: int = 128,
-                 fft_len: int = 0, m1: int = 0, rows_per_wg: int = 0, slab: int = 0,
-                 conv_rows: int = 0, precision: int = PREC_FP32):
+                 fft_len: int = 0, m1: int = 0, conv_rows: int = 0, precision: int = PREC_FP32):
         L = load()
-        o = Opts()
-        L.spfft_opts_default(ctypes.byref(o))
-        o.length_rule, o.u_tile, o.precision = length_rule, u_tile, precision
-        o.fft_len, o.m1, o.rows_per_wg, o.slab, o.conv_rows = fft_len, m1, rows_per_wg, slab, conv_rows
+        o = _opts(length_rule, u_tile, fft_len, m1, conv_rows, precision)
         self._h = _vp()
         _check(L.spfft_plan_create(int(rows), int(cols), ctypes.byref(o), ctypes.byref(self._h)))
         inf = PlanInfo()
         _check(L.spfft_plan_get_info(self._h, ctypes.byref(inf)))
-        self.info = {k: getattr(inf, k) for k, _ in PlanInfo._fields_}
+        self.info = _info_dict(inf)
         self.rows, self.cols = rows, cols
 
     def __del__(self):

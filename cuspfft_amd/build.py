@@ -28,8 +28,10 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
           "-Wno-unused-result", "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
-LIB_SOURCES = ["csc_build.hip", "rowpass.hip", "spfft_plan.cpp", "spfft_gen.cpp",
-               "sparse_fft_shim.cpp"]
+LIB_SOURCES = ["csc_build.hip", "rowpass.hip", "rowpass_conv_a.hip", "rowpass_conv_b.hip",
+               "rowpass_conv_c.hip", "rowpass_conv_d.hip", "rowpass_conv_e.hip",
+               "rowpass_fft1_a.hip", "rowpass_fft1_b.hip", "rowpass_fft1_c.hip", "spfft_plan.cpp",
+               "spfft_gen.cpp", "sparse_fft_shim.cpp"]
 CLI_SOURCES = ["cli_main.cpp"]
 
 

@@ -1,36 +1,36 @@
 // spfft_device.h -- device building blocks for the gfx950 kernels:
-// complex helpers, exact integer phase reduction, radix-2/3/4/5/7/8 DFT
-// butterflies and a batched Stockham FFT that runs inside LDS.
+// complex helpers, exact integer phase reduction and the radix-2/3/4/5/7/8
+// DFT butterflies (host+device, so the codelets can be checked on the CPU).
 //
-// The Stockham stage follows the autosort ("expand") form: for stage radix R
-// and Ns = product of the radices already applied,
+// Every FFT in this library uses the Stockham autosort ("expand") stage: for
+// stage radix R and Ns = product of the radices already applied,
 //   butterfly j in [0, L/R):  k = j mod Ns
 //     v[r] = x[j + r*L/R] * W_{Ns*R}^{r*k}          (r < R)
 //     v    = DFT_R(v)
 //     y[(j/Ns)*Ns*R + k + r*Ns] = v[r]
-// which leaves the result in natural order after the last stage.  Every
-// twiddle W_{Ns*R}^{r*k} = W_L^{r*k*L/(Ns*R)} is read from one length-L table
-// generated in FP64 on the host; no trigonometry runs on the device.
+// which leaves the result in natural order after the last stage.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "spfft_types.h"
 
+#define SPFFT_HD __host__ __device__ __forceinline__
+
 namespace spfft {
 
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+SPFFT_HD float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+SPFFT_HD float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+SPFFT_HD float2 cmul(float2 a, float2 b) {
     return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
 }
 // a * conj(b)
-__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+SPFFT_HD float2 cmulc(float2 a, float2 b) {
     return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -a.x * b.y));
 }
-__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+SPFFT_HD float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 // multiply by -i (forward) or +i (inverse)
-__device__ __forceinline__ float2 mul_mi(float2 a, bool inv) {
+SPFFT_HD float2 mul_mi(float2 a, bool inv) {
     return inv ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
 
@@ -54,7 +54,7 @@ __device__ __forceinline__ int fdiv_small(int a, int d, float inv_d) {
 template <int R> struct Dft;
 
 template <> struct Dft<2> {
-    static __device__ __forceinline__ void run(float2* v, bool) {
+    static SPFFT_HD void run(float2* v, bool) {
         const float2 a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
@@ -62,7 +62,7 @@ template <> struct Dft<2> {
 };
 
 template <> struct Dft<4> {
-    static __device__ __forceinline__ void run(float2* v, bool inv) {
+    static SPFFT_HD void run(float2* v, bool inv) {
         const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
         const float2 s13 = cadd(v[1], v[3]), d13 = mul_mi(csub(v[1], v[3]), inv);
         v[0] = cadd(s02, s13);
@@ -73,7 +73,7 @@ template <> struct Dft<4> {
 };
 
 template <> struct Dft<8> {
-    static __device__ __forceinline__ void run(float2* v, bool inv) {
+    static SPFFT_HD void run(float2* v, bool inv) {
         const float h = 0.70710678118654752f;
         float2 e[4] = {v[0], v[2], v[4], v[6]};
         float2 o[4] = {v[1], v[3], v[5], v[7]};
@@ -102,7 +102,7 @@ template <> struct Dft<8> {
 //   A_p = v_0 + sum_q s_q C_{pq},  B_p = sum_q d_q S_{pq}
 //   forward: y_p = A_p - i B_p, y_{R-p} = A_p + i B_p (inverse swaps the signs).
 template <int R, int H>
-__device__ __forceinline__ void dft_odd(float2* v, bool inv, const float* Cs, const float* Sn) {
+SPFFT_HD void dft_odd(float2* v, bool inv, const float* Cs, const float* Sn) {
     float2 s[H], d[H];
 #pragma unroll
     for (int q = 1; q <= H; q++) {
@@ -134,7 +134,7 @@ __device__ __forceinline__ void dft_odd(float2* v, bool inv, const float* Cs, co
 }
 
 template <> struct Dft<3> {
-    static __device__ __forceinline__ void run(float2* v, bool inv) {
+    static SPFFT_HD void run(float2* v, bool inv) {
         const float Cs[3] = {1.f, -0.5f, -0.5f};
         const float Sn[3] = {0.f, 0.86602540378443865f, -0.86602540378443865f};
         dft_odd<3, 1>(v, inv, Cs, Sn);
@@ -142,7 +142,7 @@ template <> struct Dft<3> {
 };
 
 template <> struct Dft<5> {
-    static __device__ __forceinline__ void run(float2* v, bool inv) {
+    static SPFFT_HD void run(float2* v, bool inv) {
         const float Cs[5] = {1.f, 0.30901699437494745f, -0.80901699437494742f,
                              -0.80901699437494742f, 0.30901699437494745f};
         const float Sn[5] = {0.f, 0.95105651629515357f, 0.58778525229247314f,
@@ -152,7 +152,7 @@ template <> struct Dft<5> {
 };
 
 template <> struct Dft<7> {
-    static __device__ __forceinline__ void run(float2* v, bool inv) {
+    static SPFFT_HD void run(float2* v, bool inv) {
         const float Cs[7] = {1.f, 0.62348980185873353f, -0.22252093395631440f, -0.90096886790241913f,
                              -0.90096886790241913f, -0.22252093395631440f, 0.62348980185873353f};
         const float Sn[7] = {0.f, 0.78183148246802981f, 0.97492791218182361f, 0.43388373911755812f,
@@ -160,64 +160,5 @@ template <> struct Dft<7> {
         dft_odd<7, 3>(v, inv, Cs, Sn);
     }
 };
-
-// One Stockham stage over G sequences of length L stored back to back in LDS
-// (sequence g at src + g*L).  tw: W_L^i, i < L (forward; conjugated for inverse).
-template <int R>
-__device__ __forceinline__ void lds_stage(const float2* __restrict__ src, float2* __restrict__ dst,
-                                          int L, int G, int Ns, const float2* __restrict__ tw,
-                                          bool inv) {
-    const int nb = L / R;
-    const int total = G * nb;
-    const int step = L / (Ns * R);
-    const float inv_nb = 1.f / (float)nb;
-    const float inv_ns = 1.f / (float)Ns;
-    for (int t = threadIdx.x; t < total; t += blockDim.x) {
-        const int g = fdiv_small(t, nb, inv_nb);
-        const int j = t - g * nb;
-        const int jq = fdiv_small(j, Ns, inv_ns);
-        const int k = j - jq * Ns;
-        const float2* s = src + g * L;
-        float2 v[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) v[r] = s[j + r * nb];
-        if (k != 0) {
-#pragma unroll
-            for (int r = 1; r < R; r++) {
-                const float2 w = tw[r * k * step];
-                v[r] = inv ? cmulc(v[r], w) : cmul(v[r], w);
-            }
-        }
-        Dft<R>::run(v, inv);
-        float2* d = dst + g * L + jq * Ns * R + k;
-#pragma unroll
-        for (int r = 0; r < R; r++) d[r * Ns] = v[r];
-    }
-}
-
-// Batched LDS FFT: G sequences of length f.L in buffer a (scratch b of the
-// same size).  Caller must __syncthreads() before.  Returns the buffer holding
-// the result (a or b); ends with a barrier.
-__device__ __forceinline__ float2* lds_fft(float2* a, float2* b, int G, const FftStages& f,
-                                           const float2* tw, bool inv) {
-    int Ns = 1;
-    for (int s = 0; s < f.nstages; s++) {
-        const int R = f.radix[s];
-        switch (R) {
-            case 2: lds_stage<2>(a, b, f.L, G, Ns, tw, inv); break;
-            case 3: lds_stage<3>(a, b, f.L, G, Ns, tw, inv); break;
-            case 4: lds_stage<4>(a, b, f.L, G, Ns, tw, inv); break;
-            case 5: lds_stage<5>(a, b, f.L, G, Ns, tw, inv); break;
-            case 7: lds_stage<7>(a, b, f.L, G, Ns, tw, inv); break;
-            default: lds_stage<8>(a, b, f.L, G, Ns, tw, inv); break;
-        }
-        __syncthreads();
-        float2* t = a;
-        a = b;
-        b = t;
-        Ns *= R;
-    }
-    return a;
-}
 
 }  // namespace spfft

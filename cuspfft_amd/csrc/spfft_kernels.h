@@ -33,11 +33,12 @@ int csc_sort_passes(int cols);
 hipError_t launch_csc_build(const CscBuildArgs& a, hipStream_t s);
 
 // ---- row pass (rowpass.hip) ----------------------------------------------
-size_t k1_lds_bytes(const RowPassArgs& a);
-size_t k2_lds_bytes(const RowPassArgs& a);
-size_t k3_lds_bytes(const RowPassArgs& a);
-// ev: NULL, or 4 events recorded before K1, after K1, after K2, after K3.
+bool rowpass_supports_m1(int m1);        // k_fft1 / k_ifft1 instantiated for this M1
+bool conv_fast_supported(int r1, int r2); // k_conv_fast<r1, r2> instantiated (r2 = 1: one stage)
+int conv_max_butterflies(int r);         // butterflies per thread per k_conv_fast stage
+size_t colpass_lds_bytes(const RowPassArgs& a);
+size_t conv_lds_bytes(const RowPassArgs& a);
+// ev: NULL, or 5 events recorded before k_colpass and after each of the 4 kernels.
 hipError_t launch_rowpass_tile(const RowPassArgs& a, bool packed, hipStream_t s, hipEvent_t* ev);
-hipError_t set_rowpass_lds_limits(size_t max_bytes);
 
 }  // namespace spfft

@@ -98,13 +98,19 @@ bool plan_stages(int L, FftStages& f) {
 // FP64 Stockham FFT (same autosort form as the device kernels), any 7-smooth n.
 void fft64(std::vector<cd>& x, int sign) {
     const int n = (int)x.size();
-    FftStages f;
-    if (n <= 1 || !plan_stages(n, f)) return;
+    if (n <= 1) return;
+    std::vector<int> rad;
+    {
+        int m = n;
+        for (int p : {8, 4, 2, 7, 5, 3})
+            while (m % p == 0) { rad.push_back(p); m /= p; }
+        if (m != 1) return;
+    }
     std::vector<cd> y(n), tw(n);
     for (int i = 0; i < n; i++) tw[i] = unit_root(i, n, sign);
     int Ns = 1;
-    for (int s = 0; s < f.nstages; s++) {
-        const int R = f.radix[s], nb = n / R, step = n / (Ns * R);
+    for (int R : rad) {
+        const int nb = n / R, step = n / (Ns * R);
         std::vector<cd> F(R * R);
         for (int a = 0; a < R; a++)
             for (int b = 0; b < R; b++) F[a * R + b] = unit_root((long long)a * b, R, sign);
@@ -179,7 +185,7 @@ struct spfft_plan_s {
     spfft_opts opts{};
     bool packed = false;
     RowPassArgs base{};  // kernel argument template (pointers to the tables filled in)
-    DevBuf h, wr_hi, wr_lo, tw1, tw2, twM, bhat, T;
+    DevBuf h, wr_hi, wr_lo, tw2, twM, bhat, X0, T;
     // CSC workspace
     int nnz_cap = -1;
     int last_nnz = 0;
@@ -211,58 +217,97 @@ struct spfft_plan_s {
 
 namespace {
 
-spfft_status choose_split(spfft_plan_s& p, int m1_req) {
-    const int M = p.M;
-    if (m1_req > 0) {
-        if (M % m1_req != 0) return fail(SPFFT_ERR_INVALID_ARG, "m1 must divide the Bluestein length");
-        p.M1 = m1_req;
-    } else {
-        int best = 1;
-        for (int d = 1; (long long)d * d <= M; d++)
-            if (M % d == 0 && d <= 512) best = d;
-        p.M1 = best;
+// k_conv_fast plan for a length-L FFT: one stage if L itself is an
+// instantiated radix, else the most balanced instantiated pair r1 * r2 = L.
+bool fast_conv_plan(int L, FftStages& f) {
+    f.L = L;
+    if (spfft::conv_fast_supported(L, 1)) {
+        f.nstages = 1;
+        f.radix[0] = L;
+        f.ns_before[0] = 1;
+        return true;
     }
-    p.M2 = M / p.M1;
-    if (p.M1 > 1024 || p.M2 > 2048)
+    for (int r1 = 8; r1 <= 64; r1++) {
+        if (L % r1 || L / r1 > r1 || !spfft::conv_fast_supported(r1, L / r1)) continue;
+        // smallest r1 >= r2 is the most balanced pair
+        f.nstages = 2;
+        f.radix[0] = r1;
+        f.radix[1] = L / r1;
+        f.ns_before[0] = 1;
+        f.ns_before[1] = r1;
+        return true;
+    }
+    return false;
+}
+
+spfft_status choose_split(spfft_plan_s& p, int m1_req, size_t budget) {
+    const int M = p.M;
+    RowPassArgs& a = p.base;
+    if (M == 1) {  // cols == 1: nothing to convolve beyond the 1-point product
+        p.M1 = 1;
+        p.M2 = 1;
+        a.conv_kind = 0;
+        a.f2.L = 1;
+        a.f2.nstages = 0;
+        return SPFFT_OK;
+    }
+    // score: fast convolution kernel first, then fewer stages, then M1 near 40
+    int best = -1;
+    double best_score = 1e18;
+    for (int d = 1; d <= 64; d++) {
+        if (M % d || !spfft::rowpass_supports_m1(d)) continue;
+        if (m1_req > 0 && d != m1_req) continue;
+        const int m2 = M / d;
+        FftStages f;
+        double score;
+        // LDS of one row: fast kernel (table + row), generic (table + 2 rows)
+        if (fast_conv_plan(m2, f) && (size_t)m2 * 16 + 4 <= budget) score = f.nstages * 10.0;
+        else if (plan_stages(m2, f) && (size_t)m2 * 24 + 4 <= budget) score = 1000.0 + f.nstages;
+        else continue;
+        score += std::fabs(std::log((double)d / 40.0));
+        if (score < best_score) {
+            best_score = score;
+            best = d;
+        }
+    }
+    if (best < 0) {
+        if (m1_req > 0)
+            return fail(SPFFT_ERR_UNSUPPORTED, "m1 = " + std::to_string(m1_req) +
+                                                   " gives no supported split of M = " + std::to_string(M));
         return fail(SPFFT_ERR_UNSUPPORTED, "Bluestein length " + std::to_string(M) +
-                                               " does not split into M1 <= 1024 and M2 <= 2048");
-    if (!plan_stages(p.M1, p.base.f1) || !plan_stages(p.M2, p.base.f2))
-        return fail(SPFFT_ERR_UNSUPPORTED, "Bluestein length must be 7-smooth");
+                                               " has no supported M1 x M2 split");
+    }
+    p.M1 = best;
+    p.M2 = M / best;
+    if (fast_conv_plan(p.M2, a.f2)) {
+        a.conv_kind = 1;
+    } else {
+        a.conv_kind = 0;
+        plan_stages(p.M2, a.f2);
+    }
     return SPFFT_OK;
 }
 
 spfft_status choose_tiling(spfft_plan_s& p, size_t budget) {
     RowPassArgs& a = p.base;
-    auto pow2_ok = [](int x) { return x > 0 && x <= 64 && (x & (x - 1)) == 0; };
-    const int js[] = {64, 32, 16, 8, 4, 2, 1};
-    const int us[] = {8, 4, 2, 1};
-    if (p.opts.slab && !pow2_ok(p.opts.slab))
-        return fail(SPFFT_ERR_INVALID_ARG, "slab must be a power of two <= 64");
-    if (p.opts.rows_per_wg && !(p.opts.rows_per_wg == 1 || p.opts.rows_per_wg == 2 ||
-                                p.opts.rows_per_wg == 4 || p.opts.rows_per_wg == 8))
-        return fail(SPFFT_ERR_INVALID_ARG, "rows_per_wg must be 1, 2, 4 or 8");
-    bool found = false;
-    int jdef = 1;
-    while (jdef < 16 && jdef < p.M2) jdef <<= 1;
-    for (int J : js) {
-        if (p.opts.slab ? J != p.opts.slab : J > jdef) continue;
-        for (int U : us) {
-            if (p.opts.rows_per_wg ? U != p.opts.rows_per_wg : false) continue;
-            a.J = J;
-            a.U = U;
-            if (spfft::k1_lds_bytes(a) <= budget && spfft::k3_lds_bytes(a) <= budget) { found = true; break; }
-        }
-        if (found) break;
-    }
-    if (!found) return fail(SPFFT_ERR_UNSUPPORTED, "sub-FFT length M1 does not fit in LDS");
-    const int gs[] = {16, 8, 4, 2, 1};
-    found = false;
-    for (int G : gs) {
-        if (p.opts.conv_rows ? G != p.opts.conv_rows : false) continue;
+    const int req = p.opts.conv_rows;
+    if (req < 0 || req > 64) return fail(SPFFT_ERR_INVALID_ARG, "conv_rows must be in [0, 64]");
+    int best = 0;
+    const int gmax = a.conv_kind == 1 ? 64 : 16;
+    for (int G = 1; G <= gmax; G++) {
         a.G2 = G;
-        if (spfft::k2_lds_bytes(a) <= budget) { found = true; break; }
+        bool ok = spfft::conv_lds_bytes(a) <= budget;
+        for (int s = 0; ok && a.conv_kind == 1 && s < a.f2.nstages; s++) {
+            const int R = a.f2.radix[s];
+            ok = (long)G * (a.M2 / R) <= 256L * spfft::conv_max_butterflies(R);
+        }
+        if (!ok) break;
+        best = G;
+        if (req && G == req) break;
     }
-    if (!found) return fail(SPFFT_ERR_UNSUPPORTED, "sub-FFT length M2 does not fit in LDS");
+    if (best == 0 || (req && best != req))
+        return fail(SPFFT_ERR_UNSUPPORTED, "convolution rows do not fit in LDS / registers");
+    a.G2 = best;
     return SPFFT_OK;
 }
 
@@ -343,7 +388,7 @@ spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t
         hipEvent_t* ev = nullptr;
         if (p.profiling) {
             const size_t at = p.ev_used;
-            ev = p.take_events(4);
+            ev = p.take_events(5);
             if (!ev) return fail(SPFFT_ERR_HIP, "hipEventCreate failed");
             p.ev_groups.push_back({1, at});
         }
@@ -393,9 +438,13 @@ int spfft_factor_3_smooth(int n, int* out, int cap) {
     return n == 1 ? k : -1;
 }
 
-spfft_status spfft_plan_create(int rows, int cols, const spfft_opts* opts_in, spfft_plan* out) {
-    if (!out) return fail(SPFFT_ERR_INVALID_ARG, "NULL plan pointer");
-    *out = nullptr;
+}  // extern "C"
+
+namespace {
+
+// Shape planning only (no device allocation): M, the M1 x M2 split, the
+// convolution kernel and its rows per workgroup.
+spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t budget, spfft_plan_s* p) {
     spfft_opts o;
     if (opts_in) o = *opts_in; else spfft_opts_default(&o);
     if (rows <= 0 || cols <= 0) return fail(SPFFT_ERR_INVALID_ARG, "rows and cols must be positive");
@@ -415,20 +464,16 @@ spfft_status spfft_plan_create(int rows, int cols, const spfft_opts* opts_in, sp
     } else {
         return fail(SPFFT_ERR_INVALID_ARG, "length_rule must be 3 or 7");
     }
-    spfft_plan_s* p = new (std::nothrow) spfft_plan_s();
-    if (!p) return fail(SPFFT_ERR_ALLOC, "host allocation failed");
-    std::unique_ptr<spfft_plan_s> guard(p);
     p->R = rows;
     p->C = cols;
     p->V = cols / 2 + 1;
     p->P = (p->V + 3) / 4 * 4;
     p->M = M;
     p->opts = o;
+    // one tile is at most 65535 workgroup rows of the length-M1 kernels' grid
+    p->opts.u_tile = std::min(o.u_tile, 65535);
     p->packed = rows <= 65536 && cols <= 65536;
-    spfft_status st = choose_split(*p, o.m1);
-    if (st != SPFFT_OK) return st;
-    size_t budget = 0;
-    st = ensure_lds_limit(budget);
+    spfft_status st = choose_split(*p, o.m1, budget);
     if (st != SPFFT_OK) return st;
     RowPassArgs& a = p->base;
     a.R = rows;
@@ -438,23 +483,70 @@ spfft_status spfft_plan_create(int rows, int cols, const spfft_opts* opts_in, sp
     a.M = M;
     a.M1 = p->M1;
     a.M2 = p->M2;
+    a.ldx = (cols + 63) / 64 * 64;
     int hs = 8;
     while ((1LL << (2 * hs)) < rows) hs++;
     a.hi_shift = hs;
     a.n_hi = (rows + (1 << hs) - 1) >> hs;
     a.r_magic = ~0ULL / (uint64_t)rows + 1;
-    st = choose_tiling(*p, budget);
+    a.inv_r = 1.0f / (float)rows;
+    return choose_tiling(*p, budget);
+}
+
+void fill_info(const spfft_plan_s* p, spfft_plan_info* info) {
+    info->rows = p->R;
+    info->cols = p->C;
+    info->out_cols = p->V;
+    info->pitch = p->P;
+    info->fft_len = p->M;
+    info->m1 = p->M1;
+    info->m2 = p->M2;
+    info->packed = p->packed ? 1 : 0;
+    info->u_tile = p->opts.u_tile;
+    info->conv_rows = p->base.G2;
+    info->conv_fast = p->base.conv_kind;
+    info->conv_stages = p->base.f2.nstages;
+    for (int i = 0; i < 8; i++) info->conv_radix[i] = i < p->base.f2.nstages ? p->base.f2.radix[i] : 0;
+    info->workspace_bytes = p->table_bytes;
+}
+
+}  // namespace
+
+extern "C" {
+
+spfft_status spfft_plan_describe(int rows, int cols, const spfft_opts* opts, spfft_plan_info* info) {
+    if (!info) return fail(SPFFT_ERR_INVALID_ARG, "NULL info");
+    spfft_plan_s p;
+    spfft_status st = plan_shape(rows, cols, opts, 65536, &p);
     if (st != SPFFT_OK) return st;
+    fill_info(&p, info);
+    return SPFFT_OK;
+}
+
+spfft_status spfft_plan_create(int rows, int cols, const spfft_opts* opts_in, spfft_plan* out) {
+    if (!out) return fail(SPFFT_ERR_INVALID_ARG, "NULL plan pointer");
+    *out = nullptr;
+    spfft_plan_s* p = new (std::nothrow) spfft_plan_s();
+    if (!p) return fail(SPFFT_ERR_ALLOC, "host allocation failed");
+    std::unique_ptr<spfft_plan_s> guard(p);
+    size_t budget = 0;
+    spfft_status st = ensure_lds_limit(budget);
+    if (st != SPFFT_OK) return st;
+    st = plan_shape(rows, cols, opts_in, budget, p);
+    if (st != SPFFT_OK) return st;
+    RowPassArgs& a = p->base;
+    const int M = p->M;
+    const int hs = a.hi_shift;
+    const spfft_opts& o = p->opts;
 
     // ---- tables (FP64 -> FP32) ----
-    std::vector<float2> h(cols), lo(1 << hs), hi(a.n_hi), t1(p->M1), t2(p->M2), tm((size_t)M), bh((size_t)M);
+    std::vector<float2> h(cols), lo(1 << hs), hi(a.n_hi), t2(p->M2), tm((size_t)M), bh((size_t)M);
     for (int k = 0; k < cols; k++) {
         const long long k2 = (long long)k * k % (2LL * cols);
         h[k] = to_f2(unit_root(k2, 2LL * cols, -1));  // exp(-i pi k^2 / C)
     }
     for (int l = 0; l < (1 << hs); l++) lo[l] = to_f2(unit_root(l, rows, -1));
     for (int j = 0; j < a.n_hi; j++) hi[j] = to_f2(unit_root((long long)j << hs, rows, -1));
-    for (int i = 0; i < p->M1; i++) t1[i] = to_f2(unit_root(i, p->M1, -1));
     for (int i = 0; i < p->M2; i++) t2[i] = to_f2(unit_root(i, p->M2, -1));
     for (int k1 = 0; k1 < p->M1; k1++)
         for (int n2 = 0; n2 < p->M2; n2++)
@@ -478,14 +570,12 @@ spfft_status spfft_plan_create(int rows, int cols, const spfft_opts* opts_in, sp
     SPFFT_HIP(upload(p->h, h), "upload chirp");
     SPFFT_HIP(upload(p->wr_lo, lo), "upload twiddles");
     SPFFT_HIP(upload(p->wr_hi, hi), "upload twiddles");
-    SPFFT_HIP(upload(p->tw1, t1), "upload twiddles");
     SPFFT_HIP(upload(p->tw2, t2), "upload twiddles");
     SPFFT_HIP(upload(p->twM, tm), "upload twiddles");
     SPFFT_HIP(upload(p->bhat, bh), "upload B^");
-    // one tile is at most 65535 workgroup rows of the pass-1 grid
-    p->opts.u_tile = std::min(o.u_tile, 65535);
-    const int tile_rows = std::min(p->opts.u_tile, rows);
+    const int tile_rows = std::min(o.u_tile, rows);
     SPFFT_HIP(p->T.alloc((size_t)tile_rows * M * sizeof(float2)), "hipMalloc(tile buffer)");
+    SPFFT_HIP(p->X0.alloc((size_t)tile_rows * a.ldx * sizeof(float2)), "hipMalloc(pass-1 buffer)");
     SPFFT_HIP(p->counts.alloc((size_t)(cols + 1) * 4), "hipMalloc(counts)");
     SPFFT_HIP(p->colstart.alloc((size_t)(cols + 1) * 4), "hipMalloc(colstart)");
     SPFFT_HIP(p->sparse_cols.alloc((size_t)cols * (p->packed ? 2 : 4)), "hipMalloc(sparse_cols)");
@@ -495,13 +585,13 @@ spfft_status spfft_plan_create(int rows, int cols, const spfft_opts* opts_in, sp
     a.h = p->h.as<float2>();
     a.wr_hi = p->wr_hi.as<float2>();
     a.wr_lo = p->wr_lo.as<float2>();
-    a.tw1 = p->tw1.as<float2>();
     a.tw2 = p->tw2.as<float2>();
     a.twM = p->twM.as<float2>();
     a.bhat = p->bhat.as<float2>();
     a.T = p->T.as<float2>();
-    p->table_bytes = p->h.bytes + p->wr_lo.bytes + p->wr_hi.bytes + p->tw1.bytes + p->tw2.bytes +
-                     p->twM.bytes + p->bhat.bytes + p->T.bytes;
+    a.X0 = p->X0.as<float2>();
+    p->table_bytes = p->h.bytes + p->wr_lo.bytes + p->wr_hi.bytes + p->tw2.bytes + p->twM.bytes +
+                     p->bhat.bytes + p->T.bytes + p->X0.bytes;
     *out = guard.release();
     return SPFFT_OK;
 }
@@ -513,19 +603,7 @@ spfft_status spfft_plan_destroy(spfft_plan p) {
 
 spfft_status spfft_plan_get_info(spfft_plan p, spfft_plan_info* info) {
     if (!p || !info) return fail(SPFFT_ERR_INVALID_ARG, "NULL argument");
-    info->rows = p->R;
-    info->cols = p->C;
-    info->out_cols = p->V;
-    info->pitch = p->P;
-    info->fft_len = p->M;
-    info->m1 = p->M1;
-    info->m2 = p->M2;
-    info->packed = p->packed ? 1 : 0;
-    info->u_tile = p->opts.u_tile;
-    info->rows_per_wg = p->base.U;
-    info->slab = p->base.J;
-    info->conv_rows = p->base.G2;
-    info->workspace_bytes = p->table_bytes;
+    fill_info(p, info);
     return SPFFT_OK;
 }
 
@@ -593,7 +671,7 @@ spfft_status spfft_profile_read(spfft_plan p, void* stream, spfft_profile* out) 
             out->ms[0] += ms;
             out->launches[0] += 1;
         } else {
-            for (int k = 0; k < 3; k++) {
+            for (int k = 0; k < 4; k++) {
                 SPFFT_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]), "hipEventElapsedTime");
                 out->ms[1 + k] += ms;
                 out->launches[1 + k] += 1;

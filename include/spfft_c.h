@@ -43,10 +43,8 @@ typedef struct spfft_opts {
     int u_tile;      /* output rows per tile (reference --csc-tile), > 0       */
     int precision;   /* SPFFT_PREC_*                                           */
     int fft_len;     /* 0: from length_rule; else an explicit 7-smooth M >= 2*cols-1 */
-    int m1;          /* 0: automatic; else the first factor of M = m1 * m2     */
-    int rows_per_wg; /* 0: automatic; else 1, 2, 4 or 8 (pass-1 rows per workgroup) */
-    int slab;        /* 0: automatic; else a power of two <= 64 (n2 columns per workgroup) */
-    int conv_rows;   /* 0: automatic; else rows per workgroup of the convolution kernel */
+    int m1;          /* 0: automatic; else the first factor of M = m1 * m2 (a 7-smooth m1 <= 64) */
+    int conv_rows;   /* 0: automatic; else rows per workgroup of the convolution kernel (<= 64) */
 } spfft_opts;
 
 typedef struct spfft_plan_info {
@@ -57,7 +55,10 @@ typedef struct spfft_plan_info {
     int m1, m2;        /* M = m1 * m2                                  */
     int packed;        /* 1 if CSC indices are stored as uint16        */
     int u_tile;
-    int rows_per_wg, slab, conv_rows;
+    int conv_rows;          /* rows per workgroup of the convolution kernel            */
+    int conv_fast;          /* 1: register-codelet kernel, 0: generic LDS fallback     */
+    int conv_stages;        /* radix stages of the length-m2 FFT                       */
+    int conv_radix[8];      /* their radices                                           */
     size_t workspace_bytes; /* device bytes held by the plan (tables + tile buffer) */
 } spfft_plan_info;
 
@@ -97,6 +98,9 @@ const char* spfft_version(void);
  * workspace for one tile. */
 spfft_status spfft_plan_create(int rows, int cols, const spfft_opts* opts, spfft_plan* plan);
 spfft_status spfft_plan_destroy(spfft_plan plan);
+/* Shape planning only, on the host (no device needed): fills `info` (workspace_bytes = 0)
+ * for the plan spfft_plan_create would build with a 64 KiB LDS budget. */
+spfft_status spfft_plan_describe(int rows, int cols, const spfft_opts* opts, spfft_plan_info* info);
 spfft_status spfft_plan_get_info(spfft_plan plan, spfft_plan_info* info);
 
 /* Stage 1 only: device COO (int32 rows, cols) -> compact CSC held by the plan. */
@@ -115,12 +119,13 @@ spfft_status spfft_check_input(spfft_plan plan, void* stream);
 /* Per-kernel GPU time from HIP events recorded on the launch stream around
  * the CSC build and around each row-pass kernel of every tile (the role of the
  * reference's CUSPFFT_BREAKDOWN timing, src/sparse_fft_bluestein.cu:1752-1760).
- * Index 0 = CSC build, 1 = pass 1 + first sub-FFTs, 2 = convolution rows,
- * 3 = inverse sub-FFTs + finalize.  spfft_profile_read synchronises `stream`,
- * returns the sums since the last read and resets them. */
+ * Index 0 = CSC build, 1 = column pass, 2 = length-m1 sub-FFTs, 3 = convolution
+ * rows (length-m2 FFT, B^ product, inverse), 4 = inverse length-m1 sub-FFTs +
+ * finalize.  spfft_profile_read synchronises `stream`, returns the sums since
+ * the last read and resets them. */
 typedef struct spfft_profile {
-    double ms[4];
-    int launches[4];
+    double ms[5];
+    int launches[5];
 } spfft_profile;
 spfft_status spfft_profile_enable(spfft_plan plan, int on);
 spfft_status spfft_profile_read(spfft_plan plan, void* stream, spfft_profile* out);

@@ -1,0 +1,136 @@
+"""CPU tests of the C ABI library: it loads without a GPU, exports every symbol the
+headers declare, and its host-only functions (planner, smooth lengths, generators,
+Matrix Market reader) behave as documented.  No device code runs here."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_c_functions():
+    txt = open(os.path.join(ROOT, "include", "spfft_c.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(spfft_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    import cuspfft_amd as sp
+    sp.load()
+    out = subprocess.run(["nm", "-D", "--defined-only", sp.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (\S+)", out))
+    missing = [f for f in _declared_c_functions() if f not in exported]
+    assert not missing, missing
+    # the C++ drop-in entry points of sparse_fft.h / mtx_reader.h
+    dem = subprocess.run(["nm", "-DC", "--defined-only", sp.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    for sym in ("sparse_fft_csc_bluestein_mixed_radix(COOMatrix const&, int)",
+                "sparse_fft_csc_bluestein_cufft_smooth(COOMatrix const&, int)",
+                "read_mtx("):
+        assert sym in dem, sym
+
+
+def test_smooth_lengths_match_oracle(oracle_mod):
+    import cuspfft_amd as sp
+    for n in list(range(1, 400)) + [4619, 16411, 33411, 80841]:
+        assert sp.next_3_smooth(n) == oracle_mod.next_3_smooth(n)
+        assert sp.next_7_smooth(n) == oracle_mod.next_7_smooth(n)
+    for m in (1, 6, 5184, 17496, 34992, 82944):
+        assert sp.factor_3_smooth(m) == oracle_mod.factor_3_smooth(m)
+
+
+def test_planner_covers_many_shapes():
+    import cuspfft_amd as sp
+    for cols in list(range(1, 200)) + list(range(200, 70000, 997)):
+        for rule in (3, 7):
+            info = sp.describe_plan(17, cols, length_rule=rule)
+            M = info["fft_len"]
+            assert M >= 2 * cols - 1
+            assert info["m1"] * info["m2"] == M
+            assert 1 <= info["m1"] <= 64
+            assert int(np.prod(info["conv_radix"])) == info["m2"] or info["m2"] == 1
+            assert info["pitch"] % 4 == 0 and info["pitch"] >= cols // 2 + 1
+
+
+def test_planner_config_choices():
+    import cuspfft_amd as sp
+    for cols, m3, m7 in [(2310, 5184, 4704), (8219, 17496, 16464), (16706, 34992, 33600)]:
+        a = sp.describe_plan(cols, cols, length_rule=3)
+        b = sp.describe_plan(cols, cols, length_rule=7)
+        assert a["fft_len"] == m3 and b["fft_len"] == m7
+        assert a["conv_fast"] == 1 and b["conv_fast"] == 1
+
+
+def test_planner_errors():
+    import cuspfft_amd as sp
+    with pytest.raises(sp.SpfftError) as e:
+        sp.describe_plan(10, 10, u_tile=0)
+    assert "CSC tile size must be positive" in str(e.value)
+    with pytest.raises(sp.SpfftError):
+        sp.describe_plan(10, 10, fft_len=11)  # not 7-smooth
+    with pytest.raises(sp.SpfftError):
+        sp.describe_plan(10, 10, fft_len=18)  # < 2*cols - 1
+    with pytest.raises(sp.SpfftError):
+        sp.describe_plan(10, 10, fft_len=20, m1=3)  # m1 does not divide M
+    with pytest.raises(sp.SpfftError):
+        sp.describe_plan(0, 10)
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "c3"])
+def test_generators_deterministic_symmetric(name):
+    import cuspfft_amd as sp
+    a = sp.config_matrix(name)
+    b = sp.config_matrix(name)
+    np.testing.assert_array_equal(a.row_idx, b.row_idx)
+    np.testing.assert_array_equal(a.col_idx, b.col_idx)
+    assert not np.any(a.row_idx == a.col_idx)  # zero diagonal
+    key = a.row_idx.astype(np.int64) * a.cols + a.col_idx
+    assert np.all(np.diff(key) > 0)  # canonical: sorted by (row, col), no duplicates
+    tkey = np.sort(a.col_idx.astype(np.int64) * a.cols + a.row_idx)
+    np.testing.assert_array_equal(key, tkey)  # symmetric
+    c = sp.config_matrix(name, seed=1)
+    assert len(c.row_idx) != len(a.row_idx) or np.any(c.row_idx != a.row_idx)
+
+
+def test_generator_sizes():
+    import cuspfft_amd as sp
+    assert sp.config_matrix("c1").nnz == 16000
+    assert abs(sp.config_matrix("c2").nnz / 242700 - 1) < 0.01
+    m3 = sp.config_matrix("c3")
+    assert abs(m3.nnz / 121250 - 1) < 0.02
+    assert np.bincount(m3.col_idx).max() <= 400
+
+
+def test_read_mtx(tmp_path):
+    import cuspfft_amd as sp
+    p = tmp_path / "m.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate pattern symmetric\n% comment\n4 4 4\n"
+                 "2 1\n3 3\n4 2\n2 1\n")
+    m = sp.read_mtx(str(p))
+    assert (m.rows, m.cols) == (4, 4)
+    got = list(zip(m.row_idx.tolist(), m.col_idx.tolist()))
+    assert got == [(0, 1), (1, 0), (1, 3), (2, 2), (3, 1)]
+    q = tmp_path / "g.mtx"
+    q.write_text("%%MatrixMarket matrix coordinate real general\n3 5 2\n3 5 7.5\n1 1 -2\n")
+    m = sp.read_mtx(str(q))
+    assert list(zip(m.row_idx.tolist(), m.col_idx.tolist())) == [(0, 0), (2, 4)]
+    with pytest.raises(RuntimeError):
+        sp.read_mtx(str(tmp_path / "missing.mtx"))
+
+
+def test_codelets_on_host(tmp_path):
+    """Every register codelet RegFft<L> (L 7-smooth <= 64) against a direct FP64 DFT,
+    compiled with hipcc and run on the host."""
+    exe = tmp_path / "codelet_check"
+    subprocess.run(["/opt/rocm/bin/hipcc", "-O2", "-std=c++17", "-I",
+                    os.path.join(ROOT, "cuspfft_amd", "csrc"), "--offload-arch=gfx950",
+                    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                    os.path.join(ROOT, "tests", "cpp", "codelet_check.hip"), "-o", str(exe)],
+                   check=True, capture_output=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout
+    assert out.stdout.strip().endswith("OK")

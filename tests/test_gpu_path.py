@@ -257,3 +257,39 @@ def test_python_mirror(gpu, oracle_mod):
     ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, nthreads=8)
     assert r.output_cols == (m.cols // 2 + 1 + 3) // 4 * 4
     assert oracle_mod.rel_l2(X, ref) <= REL_L2_PLAIN
+
+
+def _split_cases():
+    """(cols, fft_len, m1) combinations covering many register codelets (k_fft1/k_ifft1),
+    fast convolution pairs and the generic convolution fallback."""
+    cases = []
+    for m1 in (2, 3, 4, 6, 8, 9, 12, 16, 18, 24, 27, 32, 36, 48, 54, 64):
+        cases.append((2310, 5184, m1))
+    cases += [(20, 64, 1), (3, 5, 1)]  # M1 = 1: the whole row in the convolution kernel
+    for m1 in (5, 7, 10, 14, 15, 20, 21, 25, 28, 30, 35, 40, 42, 45, 49, 56, 60, 63):
+        cases.append((2310, 0, m1))  # 7-smooth M chosen below per m1
+    cases += [(300, 686, 0), (300, 625, 0), (1000, 2401, 0), (100, 3125, 0), (5000, 10125, 0)]
+    return cases
+
+
+@pytest.mark.parametrize("cols,fft_len,m1", _split_cases())
+def test_m1_splits(gpu, oracle_mod, cols, fft_len, m1):
+    sp = gpu
+    if fft_len == 0:
+        # smallest 7-smooth M >= 2*cols-1 divisible by m1
+        M = 2 * cols - 1
+        while M % m1 or sp.next_7_smooth(M) != M:
+            M += 1
+        fft_len = M
+    rows = 97
+    rng = np.random.default_rng(cols + fft_len + m1)
+    r, c = _random_coo(rng, rows, cols, 3000)
+    print(f"[split] cols={cols} M={fft_len} m1={m1}", flush=True)
+    plan, X = _run(sp, rows, cols, r, c, fft_len=fft_len, m1=m1, u_tile=50)
+    assert plan.info["fft_len"] == fft_len
+    if m1:
+        assert plan.info["m1"] == m1
+    ref = oracle_mod.sparse_rfft2_f64(rows, cols, r, c)
+    err = oracle_mod.rel_l2(X, ref)
+    print(f"  plan {plan.info} rel_l2={err:.3e}")
+    assert err <= REL_L2_PLAIN
