@@ -34,6 +34,9 @@ def parse():
     ap.add_argument("--config", default="c3")
     ap.add_argument("--rule", type=int, default=3)
     ap.add_argument("--tile", type=int, default=0, help="u-tile rows (0 = tuned default)")
+    ap.add_argument("--m1", type=int, default=0)
+    ap.add_argument("--colpass-rows", type=int, default=0)
+    ap.add_argument("--colpass-table", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows sampled by the CPU baseline")
     ap.add_argument("--no-check", action="store_true")
@@ -90,7 +93,8 @@ def main():
     dev = torch.device("cuda:0")
     m = sp.config_matrix(args.config)
     tile = args.tile if args.tile > 0 else m.rows
-    plan = sp.Plan(m.rows, m.cols, length_rule=args.rule, u_tile=tile)
+    plan = sp.Plan(m.rows, m.cols, length_rule=args.rule, u_tile=tile, m1=args.m1,
+                   colpass_rows=args.colpass_rows, colpass_table=args.colpass_table)
     d_row = torch.as_tensor(m.row_idx).to(dev)
     d_col = torch.as_tensor(m.col_idx).to(dev)
     out = plan.alloc_output()
@@ -157,7 +161,8 @@ def main():
             "form": "plain-fp32",
             "length_rule": args.rule, "fft_len": info["fft_len"], "m1": info["m1"],
             "m2": info["m2"], "u_tile": info["u_tile"], "conv_rows": info["conv_rows"],
-            "conv_radix": info["conv_radix"],
+            "conv_radix": info["conv_radix"], "colpass_rows": info["colpass_rows"],
+            "colpass_full_table": info["colpass_full_table"],
         },
         "rel_l2_vs_fp64": rel,
         "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 5),

@@ -35,7 +35,8 @@ class SpfftError(RuntimeError):
 class Opts(ctypes.Structure):
     _fields_ = [("length_rule", ctypes.c_int), ("u_tile", ctypes.c_int),
                 ("precision", ctypes.c_int), ("fft_len", ctypes.c_int), ("m1", ctypes.c_int),
-                ("conv_rows", ctypes.c_int)]
+                ("conv_rows", ctypes.c_int), ("colpass_rows", ctypes.c_int),
+                ("colpass_table", ctypes.c_int)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -44,6 +45,7 @@ class PlanInfo(ctypes.Structure):
                 ("m2", ctypes.c_int), ("packed", ctypes.c_int), ("u_tile", ctypes.c_int),
                 ("conv_rows", ctypes.c_int), ("conv_fast", ctypes.c_int),
                 ("conv_stages", ctypes.c_int), ("conv_radix", ctypes.c_int * 8),
+                ("colpass_rows", ctypes.c_int), ("colpass_full_table", ctypes.c_int),
                 ("workspace_bytes", ctypes.c_size_t)]
 
 
@@ -224,11 +226,13 @@ def _stream_handle(stream=None):
     return ctypes.c_void_p(s.cuda_stream)
 
 
-def _opts(length_rule=SMOOTH_3, u_tile=128, fft_len=0, m1=0, conv_rows=0, precision=PREC_FP32):
+def _opts(length_rule=SMOOTH_3, u_tile=128, fft_len=0, m1=0, conv_rows=0, precision=PREC_FP32,
+          colpass_rows=0, colpass_table=0):
     o = Opts()
     load().spfft_opts_default(ctypes.byref(o))
     o.length_rule, o.u_tile, o.precision = length_rule, u_tile, precision
     o.fft_len, o.m1, o.conv_rows = fft_len, m1, conv_rows
+    o.colpass_rows, o.colpass_table = colpass_rows, colpass_table
     return o
 
 
@@ -251,9 +255,11 @@ class Plan:
     of the reference drivers (src/sparse_fft_bluestein.cu:2256-2320)."""
 
     def __init__(self, rows: int, cols: int, length_rule: int = SMOOTH_3, u_tile: int = 128,
-                 fft_len: int = 0, m1: int = 0, conv_rows: int = 0, precision: int = PREC_FP32):
+                 fft_len: int = 0, m1: int = 0, conv_rows: int = 0, precision: int = PREC_FP32,
+                 colpass_rows: int = 0, colpass_table: int = 0):
         L = load()
-        o = _opts(length_rule, u_tile, fft_len, m1, conv_rows, precision)
+        o = _opts(length_rule, u_tile, fft_len, m1, conv_rows, precision, colpass_rows,
+                  colpass_table)
         self._h = _vp()
         _check(L.spfft_plan_create(int(rows), int(cols), ctypes.byref(o), ctypes.byref(self._h)))
         inf = PlanInfo()

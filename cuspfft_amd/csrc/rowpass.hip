@@ -24,6 +24,12 @@
 //   k_ifft1<M1> one thread per (u, n2): times W_M^{-k1 n2}, inverse length-M1
 //              codelet, chirp post-multiply, and the half-spectrum write of
 //              X[u][v] for v = M2 n1 + n2 <= C/2 at pitch P.
+//
+// Only rows u <= R/2 go through the four kernels: the input is real, so each
+// column DFT is Hermitian and X[R-u][v] = conj(X[u][(C - v) mod C]).  The
+// Bluestein convolution yields the full length-C spectrum of row u, and
+// k_ifft1 writes both row u and its mirror row R - u from it -- half of the
+// reference's pass-1 and row-FFT work for the same output.
 #include "rowpass_internal.h"
 
 namespace spfft {
@@ -31,58 +37,69 @@ namespace spfft {
 namespace {
 
 // ---------------------------------------------------------------------------
-// k_colpass
+// k_colpass: 64 columns x 64*U rows per workgroup.  Wave w takes columns
+// c0 + w + 4i; lane l takes rows u0 + l + 64t (t < U).  For a nonzero row r:
+//   ph_0 = (u0 r + l r) mod R     (scalar fastmod of u0 r, then l r added and
+//                                  reduced with a float estimate fixed in integers)
+//   ph_t = ph_{t-1} + (64 r mod R), one conditional subtraction
+// W_R^ph comes from one LDS table of R entries (FULL) or from the two-level
+// table hi[ph >> s] * lo[ph & (2^s - 1)].
 // ---------------------------------------------------------------------------
-template <bool PACKED>
+template <bool PACKED, bool FULL, int U>
 __global__ __launch_bounds__(kThreads) void k_colpass(RowPassArgs a) {
     extern __shared__ float2 smem[];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int c0 = blockIdx.x * 64;
-    const int u0l = blockIdx.y * 64;
+    const int u0l = blockIdx.y * 64 * U;
+    const uint32_t R = (uint32_t)a.R;
     const int nlo = 1 << a.hi_shift;
-    float2* s_lo = smem;
+    float2* s_lo = smem;                       // FULL: the whole W_R table
     float2* s_hi = s_lo + nlo;
-    float2* s_x = s_hi + a.n_hi;  // [64 rows][kPadX]
-    for (int i = threadIdx.x; i < nlo; i += kThreads) s_lo[i] = a.wr_lo[i];
-    for (int i = threadIdx.x; i < a.n_hi; i += kThreads) s_hi[i] = a.wr_hi[i];
+    if (FULL) {
+        for (int i = threadIdx.x; i < (int)R; i += kThreads) s_lo[i] = a.wr_full[i];
+    } else {
+        for (int i = threadIdx.x; i < nlo; i += kThreads) s_lo[i] = a.wr_lo[i];
+        for (int i = threadIdx.x; i < a.n_hi; i += kThreads) s_hi[i] = a.wr_hi[i];
+    }
     __syncthreads();
 
-    const uint32_t R = (uint32_t)a.R;
     const uint32_t mask = (uint32_t)nlo - 1u;
     const int hs = a.hi_shift;
     const uint32_t u0 = (uint32_t)(a.u_base + u0l);
     for (int i = 0; i < 16; i++) {
-        const int cl = wid + 4 * i;
-        const int c = c0 + cl;
-        float2 acc = make_float2(0.f, 0.f);
-        if (c < a.C) {
-            const int beg = a.colstart[c], end = a.colstart[c + 1];
-            for (int p = beg; p < end; p++) {
-                uint32_t r = PACKED ? (uint32_t)((const uint16_t*)a.row_idx)[p]
-                                    : (uint32_t)((const int*)a.row_idx)[p];
-                if (r >= R) r = fastmod_u32(r, a.r_magic, R);
-                const uint32_t base = PACKED ? fastmod_u32(u0 * r, a.r_magic, R)
-                                             : (uint32_t)(((uint64_t)u0 * r) % (uint64_t)R);
-                // (u0 + lane) * r mod R = (base + lane * r) mod R; x < 64 R
-                const uint32_t x = base + __umul24((uint32_t)lane, r);
-                const uint32_t q = (uint32_t)((float)x * a.inv_r);
-                int ph = (int)(x - __umul24(q, R));
-                if (ph < 0) ph += (int)R;
-                else if (ph >= (int)R) ph -= (int)R;
-                const float2 w = cmul(s_hi[(uint32_t)ph >> hs], s_lo[(uint32_t)ph & mask]);
-                acc.x += w.x;
-                acc.y += w.y;
+        const int c = c0 + wid + 4 * i;
+        if (c >= a.C) break;
+        float2 acc[U];
+#pragma unroll
+        for (int t = 0; t < U; t++) acc[t] = make_float2(0.f, 0.f);
+        const int beg = a.colstart[c], end = a.colstart[c + 1];
+        for (int p = beg; p < end; p++) {
+            uint32_t r = PACKED ? (uint32_t)((const uint16_t*)a.row_idx)[p]
+                                : (uint32_t)((const int*)a.row_idx)[p];
+            if (r >= R) r = fastmod_u32(r, a.r_magic, R);
+            const uint32_t base = PACKED ? fastmod_u32(u0 * r, a.r_magic, R)
+                                         : (uint32_t)(((uint64_t)u0 * r) % (uint64_t)R);
+            const uint32_t d64 = fastmod_u32(r << 6, a.r_magic, R);
+            // (u0 + lane) * r mod R = (base + lane * r) mod R with base + lane * r < 64 R
+            const uint32_t x = base + __umul24((uint32_t)lane, r);
+            const uint32_t q = (uint32_t)((float)x * a.inv_r);
+            int ph = (int)(x - __umul24(q, R));
+            if (ph < 0) ph += (int)R;
+            else if (ph >= (int)R) ph -= (int)R;
+#pragma unroll
+            for (int t = 0; t < U; t++) {
+                const float2 w = FULL ? s_lo[ph] : cmul(s_hi[(uint32_t)ph >> hs], s_lo[(uint32_t)ph & mask]);
+                acc[t].x += w.x;
+                acc[t].y += w.y;
+                ph += (int)d64;
+                if (ph >= (int)R) ph -= (int)R;
             }
-            acc = cmul(acc, a.h[c]);
         }
-        s_x[lane * kPadX + cl] = acc;
-    }
-    __syncthreads();
-    const int c = c0 + lane;
-    if (c < a.C) {
-        for (int i = 0; i < 16; i++) {
-            const int ul = wid + 4 * i;
-            if (u0l + ul < a.tile_rows) a.X0[(size_t)(u0l + ul) * a.ldx + c] = s_x[ul * kPadX + lane];
+        const float2 hc = a.h[c];
+#pragma unroll
+        for (int t = 0; t < U; t++) {
+            const int ul = u0l + lane + 64 * t;
+            if (ul < a.tile_rows) a.X0[(size_t)ul * a.ldx + c] = cmul(acc[t], hc);
         }
     }
 }
@@ -99,7 +116,20 @@ bool rowpass_supports_m1(int m1) {
 }
 
 size_t colpass_lds_bytes(const RowPassArgs& a) {
-    return ((size_t)(1 << a.hi_shift) + a.n_hi + 64 * kPadX) * sizeof(float2);
+    if (a.wr_full) return (size_t)a.R * sizeof(float2);
+    return ((size_t)(1 << a.hi_shift) + a.n_hi) * sizeof(float2);
+}
+
+template <bool PACKED, bool FULL>
+static void launch_colpass_u(const RowPassArgs& a, hipStream_t s) {
+    const int U = a.colpass_rows;
+    const dim3 gp((a.C + 63) / 64, (a.tile_rows + 64 * U - 1) / (64 * U));
+    const size_t lds = colpass_lds_bytes(a);
+    switch (U) {
+        case 1: hipLaunchKernelGGL((k_colpass<PACKED, FULL, 1>), gp, dim3(kThreads), lds, s, a); break;
+        case 2: hipLaunchKernelGGL((k_colpass<PACKED, FULL, 2>), gp, dim3(kThreads), lds, s, a); break;
+        default: hipLaunchKernelGGL((k_colpass<PACKED, FULL, 4>), gp, dim3(kThreads), lds, s, a); break;
+    }
 }
 
 static hipError_t launch_conv(const RowPassArgs& a, hipStream_t s) {
@@ -125,9 +155,13 @@ hipError_t launch_rowpass_tile(const RowPassArgs& a, bool packed, hipStream_t s,
     if (a.tile_rows <= 0) return hipSuccess;
     hipError_t e;
     if (ev && (e = hipEventRecord(ev[0], s)) != hipSuccess) return e;
-    const dim3 gp((a.C + 63) / 64, (a.tile_rows + 63) / 64);
-    if (packed) hipLaunchKernelGGL((k_colpass<true>), gp, dim3(kThreads), colpass_lds_bytes(a), s, a);
-    else hipLaunchKernelGGL((k_colpass<false>), gp, dim3(kThreads), colpass_lds_bytes(a), s, a);
+    if (packed) {
+        if (a.wr_full) launch_colpass_u<true, true>(a, s);
+        else launch_colpass_u<true, false>(a, s);
+    } else {
+        if (a.wr_full) launch_colpass_u<false, true>(a, s);
+        else launch_colpass_u<false, false>(a, s);
+    }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev[1], s)) != hipSuccess) return e;
     const dim3 g1((a.M2 + 63) / 64, (a.tile_rows + 3) / 4);

@@ -42,11 +42,21 @@ __global__ __launch_bounds__(kThreads) void k_ifft1(RowPassArgs a) {
 #pragma unroll
     for (int k1 = 1; k1 < M1; k1++) v[k1] = cmulc(t[(size_t)k1 * a.M2], tw[(size_t)k1 * a.M2]);
     RegFft<M1>::run(v, true);
-    float2* o = a.out + (size_t)(a.u_base + ul) * a.P;
+    // X[u][w] = h[w] y[w] for every w < C.  The input is real, so
+    // X[R-u][v] = conj(X[u][(C - v) mod C]): row u also yields its mirror row.
+    const int u = a.u_base + ul;
+    const bool mirror = u > 0 && a.R - u > u;
+    float2* o = a.out + (size_t)u * a.P;
+    float2* om = a.out + (size_t)(mirror ? a.R - u : u) * a.P;
 #pragma unroll
     for (int n1 = 0; n1 < M1; n1++) {
-        const int vi = n2 + a.M2 * n1;
-        if (vi < a.V) o[vi] = cmul(v[n1], a.h[vi]);
+        const int w = n2 + a.M2 * n1;
+        if (w < a.C) {
+            const float2 x = cmul(v[n1], a.h[w]);
+            if (w < a.V) o[w] = x;
+            const int vm = w == 0 ? 0 : a.C - w;
+            if (mirror && vm < a.V) om[vm] = make_float2(x.x, -x.y);
+        }
     }
 }
 

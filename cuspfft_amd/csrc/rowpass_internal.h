@@ -7,7 +7,6 @@
 namespace spfft {
 
 constexpr int kThreads = 256;
-constexpr int kPadX = 65;  // LDS row stride of the pass-1 transpose tile
 
 // Lengths M1 with instantiated k_fft1 / k_ifft1, in three groups (one per
 // translation unit, so they compile in parallel).

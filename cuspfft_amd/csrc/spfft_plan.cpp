@@ -184,8 +184,9 @@ struct spfft_plan_s {
     int R = 0, C = 0, V = 0, P = 0, M = 0, M1 = 0, M2 = 0;
     spfft_opts opts{};
     bool packed = false;
+    bool colpass_full = false;
     RowPassArgs base{};  // kernel argument template (pointers to the tables filled in)
-    DevBuf h, wr_hi, wr_lo, tw2, twM, bhat, X0, T;
+    DevBuf h, wr_hi, wr_lo, wr_full, tw2, twM, bhat, X0, T;
     // CSC workspace
     int nnz_cap = -1;
     int last_nnz = 0;
@@ -378,13 +379,22 @@ spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t
     if (pitch < p.V) return fail(SPFFT_ERR_INVALID_ARG, "output pitch smaller than cols/2+1");
     RowPassArgs a = p.base;
     a.P = (int)pitch;
+    if (a.colpass_rows == 0) {
+        // more rows per lane amortise the per-nonzero work of long columns; short
+        // columns (sparse power-law graphs) balance better with one row per lane
+        const double avg = (double)p.last_nnz / std::max(p.C, 1);
+        a.colpass_rows = avg >= 16.0 ? 4 : (avg >= 8.0 ? 2 : 1);
+    }
     a.colstart = p.colstart.as<int>();
     a.row_idx = p.row_idx.p;
     a.out = static_cast<float2*>(d_out);
+    // Rows u in [0, R/2] are computed; each also writes its mirror row R - u
+    // (X[R-u][v] = conj(X[u][(C - v) mod C]) for a real input).
     const int tile = p.opts.u_tile;
-    for (int u0 = 0; u0 < p.R; u0 += tile) {
+    const int rows = p.R / 2 + 1;
+    for (int u0 = 0; u0 < rows; u0 += tile) {
         a.u_base = u0;
-        a.tile_rows = std::min(tile, p.R - u0);
+        a.tile_rows = std::min(tile, rows - u0);
         hipEvent_t* ev = nullptr;
         if (p.profiling) {
             const size_t at = p.ev_used;
@@ -490,6 +500,15 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     a.n_hi = (rows + (1 << hs) - 1) >> hs;
     a.r_magic = ~0ULL / (uint64_t)rows + 1;
     a.inv_r = 1.0f / (float)rows;
+    if (o.colpass_rows < 0 || o.colpass_rows == 3 || o.colpass_rows > 4)
+        return fail(SPFFT_ERR_INVALID_ARG, "colpass_rows must be 0, 1, 2 or 4");
+    a.colpass_rows = o.colpass_rows ? o.colpass_rows : 0;  // 0: set from the average degree at CSC build
+    if (o.colpass_table < 0 || o.colpass_table > 2)
+        return fail(SPFFT_ERR_INVALID_ARG, "colpass_table must be 0, 1 or 2");
+    const bool full_fits = (size_t)rows * 8 <= budget;
+    if (o.colpass_table == 2 && !full_fits)
+        return fail(SPFFT_ERR_UNSUPPORTED, "W_R table does not fit in LDS");
+    p->colpass_full = o.colpass_table == 2 || (o.colpass_table == 0 && rows <= 4096);
     return choose_tiling(*p, budget);
 }
 
@@ -507,6 +526,8 @@ void fill_info(const spfft_plan_s* p, spfft_plan_info* info) {
     info->conv_fast = p->base.conv_kind;
     info->conv_stages = p->base.f2.nstages;
     for (int i = 0; i < 8; i++) info->conv_radix[i] = i < p->base.f2.nstages ? p->base.f2.radix[i] : 0;
+    info->colpass_rows = p->base.colpass_rows;
+    info->colpass_full_table = p->colpass_full ? 1 : 0;
     info->workspace_bytes = p->table_bytes;
 }
 
@@ -570,10 +591,15 @@ spfft_status spfft_plan_create(int rows, int cols, const spfft_opts* opts_in, sp
     SPFFT_HIP(upload(p->h, h), "upload chirp");
     SPFFT_HIP(upload(p->wr_lo, lo), "upload twiddles");
     SPFFT_HIP(upload(p->wr_hi, hi), "upload twiddles");
+    if (p->colpass_full) {
+        std::vector<float2> full(rows);
+        for (int k = 0; k < rows; k++) full[k] = to_f2(unit_root(k, rows, -1));
+        SPFFT_HIP(upload(p->wr_full, full), "upload twiddles");
+    }
     SPFFT_HIP(upload(p->tw2, t2), "upload twiddles");
     SPFFT_HIP(upload(p->twM, tm), "upload twiddles");
     SPFFT_HIP(upload(p->bhat, bh), "upload B^");
-    const int tile_rows = std::min(o.u_tile, rows);
+    const int tile_rows = std::min(o.u_tile, rows / 2 + 1);
     SPFFT_HIP(p->T.alloc((size_t)tile_rows * M * sizeof(float2)), "hipMalloc(tile buffer)");
     SPFFT_HIP(p->X0.alloc((size_t)tile_rows * a.ldx * sizeof(float2)), "hipMalloc(pass-1 buffer)");
     SPFFT_HIP(p->counts.alloc((size_t)(cols + 1) * 4), "hipMalloc(counts)");
@@ -585,12 +611,13 @@ spfft_status spfft_plan_create(int rows, int cols, const spfft_opts* opts_in, sp
     a.h = p->h.as<float2>();
     a.wr_hi = p->wr_hi.as<float2>();
     a.wr_lo = p->wr_lo.as<float2>();
+    a.wr_full = p->colpass_full ? p->wr_full.as<float2>() : nullptr;
     a.tw2 = p->tw2.as<float2>();
     a.twM = p->twM.as<float2>();
     a.bhat = p->bhat.as<float2>();
     a.T = p->T.as<float2>();
     a.X0 = p->X0.as<float2>();
-    p->table_bytes = p->h.bytes + p->wr_lo.bytes + p->wr_hi.bytes + p->tw2.bytes + p->twM.bytes +
+    p->table_bytes = p->h.bytes + p->wr_lo.bytes + p->wr_hi.bytes + p->wr_full.bytes + p->tw2.bytes + p->twM.bytes +
                      p->bhat.bytes + p->T.bytes + p->X0.bytes;
     *out = guard.release();
     return SPFFT_OK;

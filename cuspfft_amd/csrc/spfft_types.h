@@ -33,6 +33,7 @@ struct RowPassArgs {
     int ldx;                   // row stride of X0 (elements)
     int G2;                    // k_conv: (u, k1) rows per workgroup
     int conv_kind;             // 1: k_conv_fast<f2.radix[0], f2.radix[1]>, 0: k_conv_generic
+    int colpass_rows;          // k_colpass: rows per lane (1, 2 or 4; 64 * rows per workgroup)
     int hi_shift;              // two-level W_R table: W_R^p = hi[p >> s] * lo[p & (2^s - 1)]
     int n_hi;                  // entries in hi table
     float inv_r;               // 1/R (phase reduction estimate)
@@ -43,6 +44,7 @@ struct RowPassArgs {
     const float2* h;           // chirp h[k] = exp(-i pi k^2 / C), k < C
     const float2* wr_hi;
     const float2* wr_lo;
+    const float2* wr_full;     // W_R^p, p < R, when the whole table fits in LDS (else NULL)
     const float2* tw2;         // W_M2^i, i < M2
     const float2* twM;         // [k1*M2 + n2] = W_M^{k1 n2}
     const float2* bhat;        // [k1*M2 + k2] = B^[k1 + M1 k2] / M

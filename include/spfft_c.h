@@ -45,6 +45,8 @@ typedef struct spfft_opts {
     int fft_len;     /* 0: from length_rule; else an explicit 7-smooth M >= 2*cols-1 */
     int m1;          /* 0: automatic; else the first factor of M = m1 * m2 (a 7-smooth m1 <= 64) */
     int conv_rows;   /* 0: automatic; else rows per workgroup of the convolution kernel (<= 64) */
+    int colpass_rows;  /* 0: automatic; else 1, 2 or 4 output rows per lane in the column pass */
+    int colpass_table; /* 0: automatic; 1: two-level W_R table; 2: whole W_R table in LDS      */
 } spfft_opts;
 
 typedef struct spfft_plan_info {
@@ -59,6 +61,8 @@ typedef struct spfft_plan_info {
     int conv_fast;          /* 1: register-codelet kernel, 0: generic LDS fallback     */
     int conv_stages;        /* radix stages of the length-m2 FFT                       */
     int conv_radix[8];      /* their radices                                           */
+    int colpass_rows;       /* output rows per lane in the column pass                 */
+    int colpass_full_table; /* 1: whole W_R table in LDS, 0: two-level table           */
     size_t workspace_bytes; /* device bytes held by the plan (tables + tile buffer) */
 } spfft_plan_info;
 

@@ -293,3 +293,20 @@ def test_m1_splits(gpu, oracle_mod, cols, fft_len, m1):
     err = oracle_mod.rel_l2(X, ref)
     print(f"  plan {plan.info} rel_l2={err:.3e}")
     assert err <= REL_L2_PLAIN
+
+
+@pytest.mark.parametrize("rows_per_lane", [1, 2, 4])
+@pytest.mark.parametrize("table", [1, 2])
+@pytest.mark.parametrize("shape", [(2310, 2310, 16000, 128), (300, 4000, 9000, 100), (4095, 33, 2000, 4095)])
+def test_colpass_variants(gpu, oracle_mod, rows_per_lane, table, shape):
+    sp = gpu
+    rows, cols, nnz, tile = shape
+    rng = np.random.default_rng(rows + cols)
+    r, c = _random_coo(rng, rows, cols, nnz)
+    print(f"[colpass] {shape} rows/lane={rows_per_lane} table={table}", flush=True)
+    plan, X = _run(sp, rows, cols, r, c, u_tile=tile, colpass_rows=rows_per_lane,
+                   colpass_table=table)
+    assert plan.info["colpass_rows"] == rows_per_lane
+    assert plan.info["colpass_full_table"] == (1 if table == 2 else 0)
+    ref = oracle_mod.sparse_rfft2_f64(rows, cols, r, c)
+    assert oracle_mod.rel_l2(X, ref) <= REL_L2_PLAIN
