@@ -35,6 +35,7 @@ def parse():
     ap.add_argument("--rule", type=int, default=3)
     ap.add_argument("--tile", type=int, default=0, help="u-tile rows (0 = tuned default)")
     ap.add_argument("--m1", type=int, default=0)
+    ap.add_argument("--precision", type=int, default=0, help="0 plain FP32, 1 BF16x3 MFMA")
     ap.add_argument("--colpass-rows", type=int, default=0)
     ap.add_argument("--colpass-table", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -94,6 +95,7 @@ def main():
     m = sp.config_matrix(args.config)
     tile = args.tile if args.tile > 0 else m.rows
     plan = sp.Plan(m.rows, m.cols, length_rule=args.rule, u_tile=tile, m1=args.m1,
+                   precision=args.precision,
                    colpass_rows=args.colpass_rows, colpass_table=args.colpass_table)
     d_row = torch.as_tensor(m.row_idx).to(dev)
     d_col = torch.as_tensor(m.col_idx).to(dev)
@@ -158,7 +160,7 @@ def main():
         "config": {
             "workload": f"{args.config}: {sp.CONFIG_TEXT[args.config]}",
             "rows": m.rows, "cols": m.cols, "nnz": m.nnz,
-            "form": "plain-fp32",
+            "form": "bf16x3-mfma" if args.precision == 1 else "plain-fp32",
             "length_rule": args.rule, "fft_len": info["fft_len"], "m1": info["m1"],
             "m2": info["m2"], "u_tile": info["u_tile"], "conv_rows": info["conv_rows"],
             "conv_radix": info["conv_radix"], "colpass_rows": info["colpass_rows"],

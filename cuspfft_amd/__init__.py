@@ -22,6 +22,7 @@ CLI_PATH = os.path.join(_HERE, "_lib", "spfft_cli")
 SMOOTH_3 = 3
 SMOOTH_7 = 7
 PREC_FP32 = 0
+PREC_BF16X3 = 1
 
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "HIP", 3: "ALLOC", 4: "UNSUPPORTED", 5: "BAD_INPUT"}
 
@@ -46,7 +47,7 @@ class PlanInfo(ctypes.Structure):
                 ("conv_rows", ctypes.c_int), ("conv_fast", ctypes.c_int),
                 ("conv_stages", ctypes.c_int), ("conv_radix", ctypes.c_int * 8),
                 ("colpass_rows", ctypes.c_int), ("colpass_full_table", ctypes.c_int),
-                ("workspace_bytes", ctypes.c_size_t)]
+                ("precision", ctypes.c_int), ("workspace_bytes", ctypes.c_size_t)]
 
 
 class Result(ctypes.Structure):

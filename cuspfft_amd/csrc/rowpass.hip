@@ -145,6 +145,7 @@ static hipError_t launch_conv(const RowPassArgs& a, hipStream_t s) {
 }
 
 static hipError_t launch_fft1(const RowPassArgs& a, dim3 grid, hipStream_t s, bool inverse) {
+    if (a.precision == 1) return launch_fft1_mfma(a, s, inverse);
     hipError_t e = launch_fft1_group_a(a, grid, s, inverse);
     if (e == hipErrorInvalidValue) e = launch_fft1_group_b(a, grid, s, inverse);
     if (e == hipErrorInvalidValue) e = launch_fft1_group_c(a, grid, s, inverse);

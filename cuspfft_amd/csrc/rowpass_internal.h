@@ -25,6 +25,8 @@ hipError_t launch_fft1_group_b(const RowPassArgs& a, dim3 grid, hipStream_t s, b
 hipError_t launch_fft1_group_c(const RowPassArgs& a, dim3 grid, hipStream_t s, bool inverse);
 // k_conv launchers, one per translation unit: each returns true (and sets
 // *err) when the plan's stage radices belong to it.
+bool mfma_supports_m1(int m1);
+hipError_t launch_fft1_mfma(const RowPassArgs& a, hipStream_t s, bool inverse);
 bool launch_conv_a(const RowPassArgs& a, hipStream_t s, hipError_t* err);
 bool launch_conv_b(const RowPassArgs& a, hipStream_t s, hipError_t* err);
 bool launch_conv_c(const RowPassArgs& a, hipStream_t s, hipError_t* err);

@@ -34,6 +34,7 @@ hipError_t launch_csc_build(const CscBuildArgs& a, hipStream_t s);
 
 // ---- row pass (rowpass.hip) ----------------------------------------------
 bool rowpass_supports_m1(int m1);        // k_fft1 / k_ifft1 instantiated for this M1
+bool mfma_supports_m1(int m1);           // precision-form (BF16x3 MFMA) sub-FFT length
 bool conv_fast_supported(int r1, int r2); // k_conv_fast<r1, r2> instantiated (r2 = 1: one stage)
 int conv_max_butterflies(int r);         // butterflies per thread per k_conv_fast stage
 size_t colpass_lds_bytes(const RowPassArgs& a);

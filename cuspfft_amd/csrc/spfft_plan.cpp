@@ -60,6 +60,17 @@ cd unit_root(long long k, long long n, int sign) {
 
 float2 to_f2(cd z) { return make_float2((float)z.real(), (float)z.imag()); }
 
+// Round a float to the nearest BF16 value (ties to even), returned as a float.
+float bf16_round(float x) {
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    b &= 0xFFFF0000u;
+    float r;
+    std::memcpy(&r, &b, 4);
+    return r;
+}
+
 int next_smooth(int n, const int* primes, int np) {
     if (n < 1) n = 1;
     for (int m = n;; m++) {
@@ -186,7 +197,7 @@ struct spfft_plan_s {
     bool packed = false;
     bool colpass_full = false;
     RowPassArgs base{};  // kernel argument template (pointers to the tables filled in)
-    DevBuf h, wr_hi, wr_lo, wr_full, tw2, twM, bhat, X0, T;
+    DevBuf h, wr_hi, wr_lo, wr_full, tw2, twM, bhat, mfma_dft, X0, T;
     // CSC workspace
     int nnz_cap = -1;
     int last_nnz = 0;
@@ -241,7 +252,7 @@ bool fast_conv_plan(int L, FftStages& f) {
     return false;
 }
 
-spfft_status choose_split(spfft_plan_s& p, int m1_req, size_t budget) {
+spfft_status choose_split(spfft_plan_s& p, int m1_req, size_t budget, bool mfma) {
     const int M = p.M;
     RowPassArgs& a = p.base;
     if (M == 1) {  // cols == 1: nothing to convolve beyond the 1-point product
@@ -256,7 +267,7 @@ spfft_status choose_split(spfft_plan_s& p, int m1_req, size_t budget) {
     int best = -1;
     double best_score = 1e18;
     for (int d = 1; d <= 64; d++) {
-        if (M % d || !spfft::rowpass_supports_m1(d)) continue;
+        if (M % d || !(mfma ? spfft::mfma_supports_m1(d) : spfft::rowpass_supports_m1(d))) continue;
         if (m1_req > 0 && d != m1_req) continue;
         const int m2 = M / d;
         FftStages f;
@@ -265,7 +276,7 @@ spfft_status choose_split(spfft_plan_s& p, int m1_req, size_t budget) {
         if (fast_conv_plan(m2, f) && (size_t)m2 * 16 + 4 <= budget) score = f.nstages * 10.0;
         else if (plan_stages(m2, f) && (size_t)m2 * 24 + 4 <= budget) score = 1000.0 + f.nstages;
         else continue;
-        score += std::fabs(std::log((double)d / 40.0));
+        score += std::fabs(std::log((double)d / (mfma ? 32.0 : 40.0)));
         if (score < best_score) {
             best_score = score;
             best = d;
@@ -459,7 +470,11 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     if (opts_in) o = *opts_in; else spfft_opts_default(&o);
     if (rows <= 0 || cols <= 0) return fail(SPFFT_ERR_INVALID_ARG, "rows and cols must be positive");
     if (o.u_tile <= 0) return fail(SPFFT_ERR_INVALID_ARG, "CSC tile size must be positive");
-    if (o.precision != SPFFT_PREC_FP32) return fail(SPFFT_ERR_UNSUPPORTED, "unknown precision form");
+    if (o.precision != SPFFT_PREC_FP32 && o.precision != SPFFT_PREC_BF16X3)
+        return fail(SPFFT_ERR_UNSUPPORTED, "unknown precision form");
+    const bool mfma = o.precision == SPFFT_PREC_BF16X3;
+    if (mfma && o.m1 && !spfft::mfma_supports_m1(o.m1))
+        return fail(SPFFT_ERR_UNSUPPORTED, "the BF16x3 form needs m1 = 16 or 32");
     if (rows > (1 << 22)) return fail(SPFFT_ERR_UNSUPPORTED, "rows > 2^22 not supported");
     if (cols > (1 << 20)) return fail(SPFFT_ERR_UNSUPPORTED, "cols > 2^20 not supported");
     int M;
@@ -467,10 +482,32 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
         if (o.fft_len < 2 * cols - 1 || !is_7_smooth(o.fft_len))
             return fail(SPFFT_ERR_INVALID_ARG, "fft_len must be 7-smooth and >= 2*cols-1");
         M = o.fft_len;
-    } else if (o.length_rule == SPFFT_SMOOTH_3) {
-        M = spfft_next_3_smooth(2 * cols - 1);
-    } else if (o.length_rule == SPFFT_SMOOTH_7) {
-        M = spfft_next_7_smooth(2 * cols - 1);
+    } else if (o.length_rule == SPFFT_SMOOTH_3 || o.length_rule == SPFFT_SMOOTH_7) {
+        auto next = [&](int n) {
+            return o.length_rule == SPFFT_SMOOTH_3 ? spfft_next_3_smooth(n) : spfft_next_7_smooth(n);
+        };
+        M = next(2 * cols - 1);
+        if (mfma) {
+            // smallest smooth M that splits as 32 * m2 or 16 * m2 with a fast m2 kernel
+            // (a generic convolution kernel if no such M lies within 25% of 2*cols-1)
+            auto ok = [&](int m, bool fast) {
+                FftStages f;
+                for (int m1 : {32, 16}) {
+                    if (m % m1 || (o.m1 && o.m1 != m1)) continue;
+                    const int m2 = m / m1;
+                    if (fast ? (fast_conv_plan(m2, f) && (size_t)m2 * 16 + 4 <= budget)
+                             : (plan_stages(m2, f) && (size_t)m2 * 24 + 4 <= budget))
+                        return true;
+                }
+                return false;
+            };
+            const int m0 = M, limit = (int)std::min<long>(1L << 24, (long)(1.25 * (2.0 * cols - 1)) + 64);
+            while (M <= limit && !ok(M, true)) M = next(M + 1);
+            if (M > limit) {
+                M = m0;
+                while (M < (1 << 24) && !ok(M, false)) M = next(M + 1);
+            }
+        }
     } else {
         return fail(SPFFT_ERR_INVALID_ARG, "length_rule must be 3 or 7");
     }
@@ -483,7 +520,7 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     // one tile is at most 65535 workgroup rows of the length-M1 kernels' grid
     p->opts.u_tile = std::min(o.u_tile, 65535);
     p->packed = rows <= 65536 && cols <= 65536;
-    spfft_status st = choose_split(*p, o.m1, budget);
+    spfft_status st = choose_split(*p, o.m1, budget, mfma);
     if (st != SPFFT_OK) return st;
     RowPassArgs& a = p->base;
     a.R = rows;
@@ -500,6 +537,7 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     a.n_hi = (rows + (1 << hs) - 1) >> hs;
     a.r_magic = ~0ULL / (uint64_t)rows + 1;
     a.inv_r = 1.0f / (float)rows;
+    a.precision = mfma ? 1 : 0;
     if (o.colpass_rows < 0 || o.colpass_rows == 3 || o.colpass_rows > 4)
         return fail(SPFFT_ERR_INVALID_ARG, "colpass_rows must be 0, 1, 2 or 4");
     a.colpass_rows = o.colpass_rows ? o.colpass_rows : 0;  // 0: set from the average degree at CSC build
@@ -526,6 +564,7 @@ void fill_info(const spfft_plan_s* p, spfft_plan_info* info) {
     info->conv_fast = p->base.conv_kind;
     info->conv_stages = p->base.f2.nstages;
     for (int i = 0; i < 8; i++) info->conv_radix[i] = i < p->base.f2.nstages ? p->base.f2.radix[i] : 0;
+    info->precision = p->base.precision;
     info->colpass_rows = p->base.colpass_rows;
     info->colpass_full_table = p->colpass_full ? 1 : 0;
     info->workspace_bytes = p->table_bytes;
@@ -599,6 +638,22 @@ spfft_status spfft_plan_create(int rows, int cols, const spfft_opts* opts_in, sp
     SPFFT_HIP(upload(p->tw2, t2), "upload twiddles");
     SPFFT_HIP(upload(p->twM, tm), "upload twiddles");
     SPFFT_HIP(upload(p->bhat, bh), "upload B^");
+    if (a.precision == 1) {
+        // cos and sin of 2 pi i / M1 split into BF16 hi + lo parts from FP64
+        const int L = p->M1;
+        std::vector<float> t((size_t)4 * L);
+        for (int i = 0; i < L; i++) {
+            const cd w = unit_root(i, L, +1);  // cos + i sin
+            const double parts[2] = {w.real(), w.imag()};
+            for (int k = 0; k < 2; k++) {
+                const float hi = bf16_round((float)parts[k]);
+                const float lo = bf16_round((float)(parts[k] - (double)hi));
+                t[(size_t)(2 * k) * L + i] = hi;
+                t[(size_t)(2 * k + 1) * L + i] = lo;
+            }
+        }
+        SPFFT_HIP(upload(p->mfma_dft, t), "upload DFT matrix");
+    }
     const int tile_rows = std::min(o.u_tile, rows / 2 + 1);
     SPFFT_HIP(p->T.alloc((size_t)tile_rows * M * sizeof(float2)), "hipMalloc(tile buffer)");
     SPFFT_HIP(p->X0.alloc((size_t)tile_rows * a.ldx * sizeof(float2)), "hipMalloc(pass-1 buffer)");
@@ -615,6 +670,7 @@ spfft_status spfft_plan_create(int rows, int cols, const spfft_opts* opts_in, sp
     a.tw2 = p->tw2.as<float2>();
     a.twM = p->twM.as<float2>();
     a.bhat = p->bhat.as<float2>();
+    a.mfma_dft = a.precision == 1 ? p->mfma_dft.as<float>() : nullptr;
     a.T = p->T.as<float2>();
     a.X0 = p->X0.as<float2>();
     p->table_bytes = p->h.bytes + p->wr_lo.bytes + p->wr_hi.bytes + p->wr_full.bytes + p->tw2.bytes + p->twM.bytes +

@@ -48,6 +48,8 @@ struct RowPassArgs {
     const float2* tw2;         // W_M2^i, i < M2
     const float2* twM;         // [k1*M2 + n2] = W_M^{k1 n2}
     const float2* bhat;        // [k1*M2 + k2] = B^[k1 + M1 k2] / M
+    const float* mfma_dft;     // precision form: cos/sin(2 pi i / M1) hi and lo BF16 parts, [4][M1]
+    int precision;             // 0: FP32 codelets, 1: BF16x3 MFMA length-M1 sub-FFTs
     float2* X0;                // tile pass-1 output [u_local][ldx]
     float2* T;                 // tile workspace [u_local][k1][n2]
     float2* out;               // output base (row 0), pitch P

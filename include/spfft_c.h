@@ -35,8 +35,14 @@ typedef enum spfft_status {
  * SMOOTH_7: M = next 2^a 3^b 5^c 7^d >= 2*cols-1 (--csc-cufft-smooth, :2562). */
 enum { SPFFT_SMOOTH_3 = 3, SPFFT_SMOOTH_7 = 7 };
 
-/* Arithmetic form of the row pass. */
-enum { SPFFT_PREC_FP32 = 0 };
+/* Arithmetic form of the length-m1 sub-FFTs of the row pass.
+ * FP32: register codelets in FP32 (the plain form).
+ * BF16X3: the radix-16/32 butterflies as complex DFT-matrix products on the
+ * matrix cores (v_mfma_f32_32x32x16_bf16), BF16 hi/lo split of both operands,
+ * three products accumulated in FP32; m1 is 16 or 32, and with fft_len = 0 the
+ * Bluestein length is the smallest smooth M >= 2*cols-1 that splits as m1 * m2
+ * with a register-codelet convolution kernel for m2. */
+enum { SPFFT_PREC_FP32 = 0, SPFFT_PREC_BF16X3 = 1 };
 
 typedef struct spfft_opts {
     int length_rule; /* SPFFT_SMOOTH_3 or SPFFT_SMOOTH_7                      */
@@ -63,6 +69,7 @@ typedef struct spfft_plan_info {
     int conv_radix[8];      /* their radices                                           */
     int colpass_rows;       /* output rows per lane in the column pass                 */
     int colpass_full_table; /* 1: whole W_R table in LDS, 0: two-level table           */
+    int precision;          /* SPFFT_PREC_* of the length-m1 sub-FFTs                   */
     size_t workspace_bytes; /* device bytes held by the plan (tables + tile buffer) */
 } spfft_plan_info;
 

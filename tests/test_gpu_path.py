@@ -310,3 +310,54 @@ def test_colpass_variants(gpu, oracle_mod, rows_per_lane, table, shape):
     assert plan.info["colpass_full_table"] == (1 if table == 2 else 0)
     ref = oracle_mod.sparse_rfft2_f64(rows, cols, r, c)
     assert oracle_mod.rel_l2(X, ref) <= REL_L2_PLAIN
+
+
+# ---- precision form: BF16x3 MFMA length-16/32 sub-FFTs -------------------------
+REL_L2_BF16X3 = 1e-5
+
+
+@pytest.mark.parametrize("shape", [(2310, 2310, 16000), (97, 300, 3000), (50, 37, 400), (1, 5, 3),
+                                   (333, 1000, 9000)])
+@pytest.mark.parametrize("rule", [3, 7])
+def test_bf16x3_small(gpu, oracle_mod, shape, rule):
+    sp = gpu
+    rows, cols, nnz = shape
+    rng = np.random.default_rng(rows * 7 + cols)
+    r, c = _random_coo(rng, rows, cols, nnz)
+    print(f"[bf16x3] {shape} rule={rule}", flush=True)
+    plan, X = _run(sp, rows, cols, r, c, length_rule=rule, precision=sp.PREC_BF16X3, u_tile=64)
+    assert plan.info["precision"] == 1 and plan.info["m1"] in (16, 32)
+    ref = oracle_mod.sparse_rfft2_f64(rows, cols, r, c)
+    err = oracle_mod.rel_l2(X, ref)
+    print(f"  M={plan.info['fft_len']} m1={plan.info['m1']} rel_l2={err:.3e}")
+    assert err <= REL_L2_BF16X3
+
+
+@pytest.mark.parametrize("m1", [16, 32])
+def test_bf16x3_forced_m1(gpu, oracle_mod, m1):
+    sp = gpu
+    m = sp.config_matrix("c1")
+    plan, X = _run(sp, m.rows, m.cols, m.row_idx, m.col_idx, precision=sp.PREC_BF16X3, m1=m1)
+    assert plan.info["m1"] == m1
+    ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, nthreads=8)
+    assert oracle_mod.rel_l2(X, ref) <= REL_L2_BF16X3
+
+
+@pytest.mark.parametrize("cfg", ["c2", "c3"])
+def test_bf16x3_full_config(gpu, oracle_mod, cfg):
+    import torch
+    sp = gpu
+    m = sp.config_matrix(cfg)
+    plan = sp.Plan(m.rows, m.cols, precision=sp.PREC_BF16X3, u_tile=m.rows)
+    out = plan.alloc_output()
+    plan.execute(_dev(m.row_idx), _dev(m.col_idx), out)
+    torch.cuda.synchronize()
+    plan.check_input()
+    rng = np.random.default_rng(5)
+    rows = np.unique(np.concatenate([[0, 1, m.rows // 2, m.rows - 1], rng.choice(m.rows, 28, replace=False)]))
+    ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, u_list=rows, nthreads=8)
+    got = out[torch.as_tensor(rows).cuda(), : m.cols // 2 + 1].cpu().numpy()
+    err = oracle_mod.rel_l2(got, ref)
+    print(f"{cfg} bf16x3 M={plan.info['fft_len']} spot rel_l2={err:.3e}")
+    assert err <= REL_L2_BF16X3
+    _invariants(out, m)
