@@ -108,6 +108,16 @@ def load():
                                     ctypes.POINTER(Result)]),
         "spfft_free_output": (_i, [_vp]),
         "spfft_profile_enable": (_i, [_vp, _i]),
+        "spfft_graph_capture": (_i, [_vp, _vp, _vp, _i, _vp, ctypes.c_int64, _vp, ctypes.POINTER(_vp)]),
+        "spfft_graph_launch": (_i, [_vp, _vp]),
+        "spfft_graph_destroy": (_i, [_vp]),
+        "spfft_csc_bluestein_mixed_radix_graph": (_i, [_i, _i, _i, _vp, _vp, _i, ctypes.POINTER(Result)]),
+        "spfft_batch_create": (_i, [_i, _vp, _vp, _vp, _vp, _i, ctypes.POINTER(_vp)]),
+        "spfft_batch_destroy": (_i, [_vp]),
+        "spfft_batch_layout": (_i, [_vp, _vp, _vp, _vp, _vp]),
+        "spfft_batch_execute": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+        "spfft_batch_capture": (_i, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+        "spfft_batch_check_input": (_i, [_vp, _vp]),
         "spfft_profile_read": (_i, [_vp, _vp, ctypes.POINTER(Profile)]),
         "spfft_next_3_smooth": (_i, [_i]),
         "spfft_next_7_smooth": (_i, [_i]),
@@ -341,6 +351,107 @@ class Plan:
                 or d_out.shape[0] != self.rows or d_out.shape[1] < self.info["out_cols"]
                 or not d_out.is_contiguous()):
             raise ValueError("output must be a contiguous complex64 CUDA tensor [rows, >= cols/2+1]")
+
+
+class Graph:
+    """A captured hipGraph of a whole transform (or a whole batch); launch() replays it."""
+
+    def __init__(self, handle, keep=()):
+        self._h = handle
+        self._keep = keep  # device buffers the graph reads or writes
+
+    def launch(self, stream=None):
+        _check(load().spfft_graph_launch(self._h, _stream_handle(stream)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.spfft_graph_destroy(h)
+            self._h = None
+
+
+def capture_plan(plan: "Plan", d_row, d_col, d_out) -> Graph:
+    """Capture CSC build + row pass of `plan` on these device buffers into one graph."""
+    plan._check_io(d_row, d_col, d_out)
+    h = _vp()
+    _check(load().spfft_graph_capture(plan._h, _vp(d_row.data_ptr()), _vp(d_col.data_ptr()),
+                                      int(d_row.numel()), _vp(d_out.data_ptr()), int(d_out.shape[1]),
+                                      None, ctypes.byref(h)))
+    return Graph(h, (plan, d_row, d_col, d_out))
+
+
+class Batch:
+    """Many matrices of uneven shape run as one launch sequence on `streams`
+    worker streams (the batched form); each matrix has its own plan."""
+
+    def __init__(self, shapes, opts=None, nnz_hint=None, streams: int = 4):
+        L = load()
+        n = len(shapes)
+        self.count = n
+        self.rows = np.ascontiguousarray([s[0] for s in shapes], np.int32)
+        self.cols = np.ascontiguousarray([s[1] for s in shapes], np.int32)
+        hint = None if nnz_hint is None else np.ascontiguousarray(nnz_hint, np.int32)
+        oarr = None
+        if opts is not None:
+            oarr = (Opts * n)()
+            for i, kw in enumerate(opts):
+                oarr[i] = _opts(**kw)
+        self._h = _vp()
+        _check(L.spfft_batch_create(n, self.rows.ctypes.data, self.cols.ctypes.data,
+                                    None if hint is None else hint.ctypes.data,
+                                    None if oarr is None else ctypes.cast(oarr, _vp),
+                                    int(streams), ctypes.byref(self._h)))
+        total = ctypes.c_int64()
+        self.offsets = np.zeros(n, np.int64)
+        self.pitches = np.zeros(n, np.int32)
+        infos = (PlanInfo * n)()
+        _check(L.spfft_batch_layout(self._h, ctypes.byref(total), self.offsets.ctypes.data,
+                                    self.pitches.ctypes.data, ctypes.cast(infos, _vp)))
+        self.total = int(total.value)
+        self.infos = [_info_dict(infos[i]) for i in range(n)]
+
+    def alloc_output(self):
+        import torch
+        return torch.empty(self.total, dtype=torch.complex64, device="cuda")
+
+    def output_of(self, arena, i: int):
+        """View of matrix i's [rows, pitch] output inside the arena."""
+        r, p = int(self.rows[i]), int(self.pitches[i])
+        o = int(self.offsets[i])
+        return arena[o: o + r * p].view(r, p)
+
+    @staticmethod
+    def pack(mats):
+        """Concatenate COO pairs: (d_row, d_col, coo_off) ready for execute()."""
+        import torch
+        off = np.zeros(len(mats) + 1, np.int64)
+        for i, m in enumerate(mats):
+            off[i + 1] = off[i] + m.nnz
+        r = np.concatenate([m.row_idx for m in mats]).astype(np.int32) if off[-1] else np.zeros(1, np.int32)
+        c = np.concatenate([m.col_idx for m in mats]).astype(np.int32) if off[-1] else np.zeros(1, np.int32)
+        return torch.as_tensor(r).cuda(), torch.as_tensor(c).cuda(), off
+
+    def execute(self, d_row, d_col, coo_off, arena, stream=None):
+        off = np.ascontiguousarray(coo_off, np.int64)
+        _check(load().spfft_batch_execute(self._h, _vp(d_row.data_ptr()), _vp(d_col.data_ptr()),
+                                          off.ctypes.data, _vp(arena.data_ptr()),
+                                          _stream_handle(stream)))
+
+    def capture(self, d_row, d_col, coo_off, arena) -> Graph:
+        off = np.ascontiguousarray(coo_off, np.int64)
+        h = _vp()
+        _check(load().spfft_batch_capture(self._h, _vp(d_row.data_ptr()), _vp(d_col.data_ptr()),
+                                          off.ctypes.data, _vp(arena.data_ptr()), ctypes.byref(h)))
+        return Graph(h, (self, d_row, d_col, arena, off))
+
+    def check_input(self, stream=None):
+        _check(load().spfft_batch_check_input(self._h, _stream_handle(stream)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.spfft_batch_destroy(h)
+            self._h = None
 
 
 _hip = None

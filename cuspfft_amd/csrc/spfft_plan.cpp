@@ -23,11 +23,12 @@
 #include "../../include/spfft_c.h"
 #include "spfft_kernels.h"
 #include "spfft_types.h"
+#include "spfft_plan_internal.h"
 
 using spfft::FftStages;
 using spfft::RowPassArgs;
 
-namespace {
+namespace spfft_internal {
 
 thread_local std::string g_last_error;
 
@@ -41,11 +42,11 @@ spfft_status hip_fail(hipError_t e, const char* what) {
                 std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
 }
 
-#define SPFFT_HIP(call, what)                                   \
-    do {                                                        \
-        hipError_t e__ = (call);                                \
-        if (e__ != hipSuccess) return hip_fail(e__, what);      \
-    } while (0)
+}  // namespace spfft_internal
+
+using namespace spfft_internal;
+
+namespace {
 
 const double kPi = 3.14159265358979323846264338327950288;
 typedef std::complex<double> cd;
@@ -142,37 +143,6 @@ void fft64(std::vector<cd>& x, int sign) {
     }
 }
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { reset(); }
-    void reset() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    hipError_t alloc(size_t n) {
-        reset();
-        if (n == 0) n = 16;
-        hipError_t e = hipMalloc(&p, n);
-        if (e != hipSuccess) { p = nullptr; return e; }
-        bytes = n;
-        return hipSuccess;
-    }
-    template <class T> T* as() const { return static_cast<T*>(p); }
-};
-
-template <class T>
-hipError_t upload(DevBuf& b, const std::vector<T>& v) {
-    hipError_t e = b.alloc(v.size() * sizeof(T));
-    if (e != hipSuccess) return e;
-    if (v.empty()) return hipSuccess;
-    return hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
-}
-
 size_t g_lds_limit = 0;
 std::mutex g_lds_mutex;
 
@@ -190,42 +160,6 @@ spfft_status ensure_lds_limit(size_t& budget) {
 }
 
 }  // namespace
-
-struct spfft_plan_s {
-    int R = 0, C = 0, V = 0, P = 0, M = 0, M1 = 0, M2 = 0;
-    spfft_opts opts{};
-    bool packed = false;
-    bool colpass_full = false;
-    RowPassArgs base{};  // kernel argument template (pointers to the tables filled in)
-    DevBuf h, wr_hi, wr_lo, wr_full, tw2, twM, bhat, mfma_dft, X0, T;
-    // CSC workspace
-    int nnz_cap = -1;
-    int last_nnz = 0;
-    bool have_csc = false;
-    DevBuf counts, colstart, sparse_cols, col_ptr, totals, err, scan_tmp, hist, hist_off;
-    DevBuf keys0, keys1, vals0, vals1, row_idx;
-    size_t table_bytes = 0;
-    // optional per-kernel event timing (spfft_profile_*)
-    bool profiling = false;
-    std::vector<hipEvent_t> ev_pool;
-    size_t ev_used = 0;
-    // event groups recorded since the last read: kind 0 = CSC build (2 events),
-    // kind 1 = row-pass tile (4 events)
-    std::vector<std::pair<int, size_t>> ev_groups;
-    ~spfft_plan_s() {
-        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
-    }
-    hipEvent_t* take_events(int n) {
-        while (ev_pool.size() < ev_used + n) {
-            hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess) return nullptr;
-            ev_pool.push_back(e);
-        }
-        hipEvent_t* r = &ev_pool[ev_used];
-        ev_used += n;
-        return r;
-    }
-};
 
 namespace {
 
@@ -323,6 +257,16 @@ spfft_status choose_tiling(spfft_plan_s& p, size_t budget) {
     return SPFFT_OK;
 }
 
+bool check_host_coo(int rows, int cols, int nnz, const int* r, const int* c) {
+    for (int i = 0; i < nnz; i++)
+        if (r[i] < 0 || r[i] >= rows || c[i] < 0 || c[i] >= cols) return false;
+    return true;
+}
+
+}  // namespace
+
+namespace spfft_internal {
+
 spfft_status ensure_csc_capacity(spfft_plan_s& p, int nnz) {
     if (nnz <= p.nnz_cap) return SPFFT_OK;
     const int cap = std::max(nnz, 1);
@@ -384,7 +328,8 @@ spfft_status do_csc_build(spfft_plan_s& p, const int* d_row, const int* d_col, i
     return SPFFT_OK;
 }
 
-spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t s) {
+spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t s, float2* T,
+                        float2* X0) {
     if (!p.have_csc) return fail(SPFFT_ERR_INVALID_ARG, "no CSC built yet");
     if (!d_out) return fail(SPFFT_ERR_INVALID_ARG, "NULL output pointer");
     if (pitch < p.V) return fail(SPFFT_ERR_INVALID_ARG, "output pitch smaller than cols/2+1");
@@ -399,6 +344,9 @@ spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t
     a.colstart = p.colstart.as<int>();
     a.row_idx = p.row_idx.p;
     a.out = static_cast<float2*>(d_out);
+    if (T) a.T = T;
+    if (X0) a.X0 = X0;
+    if (!a.T || !a.X0) return fail(SPFFT_ERR_INVALID_ARG, "plan has no row-pass workspace");
     // Rows u in [0, R/2] are computed; each also writes its mirror row R - u
     // (X[R-u][v] = conj(X[u][(C - v) mod C]) for a real input).
     const int tile = p.opts.u_tile;
@@ -418,13 +366,7 @@ spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t
     return SPFFT_OK;
 }
 
-bool check_host_coo(int rows, int cols, int nnz, const int* r, const int* c) {
-    for (int i = 0; i < nnz; i++)
-        if (r[i] < 0 || r[i] >= rows || c[i] < 0 || c[i] >= cols) return false;
-    return true;
-}
-
-}  // namespace
+}  // namespace spfft_internal
 
 extern "C" {
 
@@ -586,6 +528,22 @@ spfft_status spfft_plan_describe(int rows, int cols, const spfft_opts* opts, spf
 spfft_status spfft_plan_create(int rows, int cols, const spfft_opts* opts_in, spfft_plan* out) {
     if (!out) return fail(SPFFT_ERR_INVALID_ARG, "NULL plan pointer");
     *out = nullptr;
+    return plan_create_ex(rows, cols, opts_in, true, out);
+}
+
+}  // extern "C"
+
+namespace spfft_internal {
+
+void workspace_bytes(const spfft_plan_s& p, size_t* t_bytes, size_t* x0_bytes) {
+    const size_t tile_rows = (size_t)std::min(p.opts.u_tile, p.R / 2 + 1);
+    *t_bytes = tile_rows * (size_t)p.M * sizeof(float2);
+    *x0_bytes = tile_rows * (size_t)p.base.ldx * sizeof(float2);
+}
+
+spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool own_workspace,
+                            spfft_plan_s** out) {
+    *out = nullptr;
     spfft_plan_s* p = new (std::nothrow) spfft_plan_s();
     if (!p) return fail(SPFFT_ERR_ALLOC, "host allocation failed");
     std::unique_ptr<spfft_plan_s> guard(p);
@@ -655,8 +613,10 @@ spfft_status spfft_plan_create(int rows, int cols, const spfft_opts* opts_in, sp
         SPFFT_HIP(upload(p->mfma_dft, t), "upload DFT matrix");
     }
     const int tile_rows = std::min(o.u_tile, rows / 2 + 1);
-    SPFFT_HIP(p->T.alloc((size_t)tile_rows * M * sizeof(float2)), "hipMalloc(tile buffer)");
-    SPFFT_HIP(p->X0.alloc((size_t)tile_rows * a.ldx * sizeof(float2)), "hipMalloc(pass-1 buffer)");
+    if (own_workspace) {
+        SPFFT_HIP(p->T.alloc((size_t)tile_rows * M * sizeof(float2)), "hipMalloc(tile buffer)");
+        SPFFT_HIP(p->X0.alloc((size_t)tile_rows * a.ldx * sizeof(float2)), "hipMalloc(pass-1 buffer)");
+    }
     SPFFT_HIP(p->counts.alloc((size_t)(cols + 1) * 4), "hipMalloc(counts)");
     SPFFT_HIP(p->colstart.alloc((size_t)(cols + 1) * 4), "hipMalloc(colstart)");
     SPFFT_HIP(p->sparse_cols.alloc((size_t)cols * (p->packed ? 2 : 4)), "hipMalloc(sparse_cols)");
@@ -678,6 +638,10 @@ spfft_status spfft_plan_create(int rows, int cols, const spfft_opts* opts_in, sp
     *out = guard.release();
     return SPFFT_OK;
 }
+
+}  // namespace spfft_internal
+
+extern "C" {
 
 spfft_status spfft_plan_destroy(spfft_plan p) {
     delete p;
@@ -723,7 +687,7 @@ spfft_status spfft_csc_get(spfft_plan p, void* stream, spfft_csc_view* v) {
 
 spfft_status spfft_rowpass(spfft_plan p, void* d_out, int64_t pitch, void* stream) {
     if (!p) return fail(SPFFT_ERR_INVALID_ARG, "NULL plan");
-    return do_rowpass(*p, d_out, pitch, (hipStream_t)stream);
+    return do_rowpass(*p, d_out, pitch, (hipStream_t)stream, nullptr, nullptr);
 }
 
 spfft_status spfft_execute(spfft_plan p, const int* d_row, const int* d_col, int nnz, void* d_out,
@@ -731,7 +695,7 @@ spfft_status spfft_execute(spfft_plan p, const int* d_row, const int* d_col, int
     if (!p) return fail(SPFFT_ERR_INVALID_ARG, "NULL plan");
     spfft_status st = do_csc_build(*p, d_row, d_col, nnz, (hipStream_t)stream);
     if (st != SPFFT_OK) return st;
-    return do_rowpass(*p, d_out, pitch, (hipStream_t)stream);
+    return do_rowpass(*p, d_out, pitch, (hipStream_t)stream, nullptr, nullptr);
 }
 
 spfft_status spfft_profile_enable(spfft_plan p, int on) {
@@ -811,7 +775,7 @@ spfft_status spfft_run_host_coo(int rows, int cols, int nnz, const int* h_row, c
     st = do_csc_build(*p, drow.as<int>(), dcol.as<int>(), nnz, 0);
     if (st != SPFFT_OK) return st;
     SPFFT_HIP(hipEventRecord(e1, 0), "hipEventRecord");
-    st = do_rowpass(*p, dout.p, p->P, 0);
+    st = do_rowpass(*p, dout.p, p->P, 0, nullptr, nullptr);
     if (st != SPFFT_OK) return st;
     SPFFT_HIP(hipEventRecord(e2, 0), "hipEventRecord");
     SPFFT_HIP(hipEventSynchronize(e2), "hipEventSynchronize");

@@ -1,0 +1,110 @@
+// spfft_plan_internal.h -- the plan object and helpers shared by the C ABI
+// translation units (spfft_plan.cpp, spfft_batch.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/spfft_c.h"
+#include "spfft_types.h"
+
+namespace spfft_internal {
+
+spfft_status fail(spfft_status st, const std::string& msg);
+spfft_status hip_fail(hipError_t e, const char* what);
+
+#define SPFFT_HIP(call, what)                                                      \
+    do {                                                                           \
+        hipError_t e__ = (call);                                                   \
+        if (e__ != hipSuccess) return spfft_internal::hip_fail(e__, what);         \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        reset();
+        if (n == 0) n = 16;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) { p = nullptr; return e; }
+        bytes = n;
+        return hipSuccess;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+template <class T>
+hipError_t upload(DevBuf& b, const std::vector<T>& v) {
+    hipError_t e = b.alloc(v.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (v.empty()) return hipSuccess;
+    return hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+
+}  // namespace spfft_internal
+
+using spfft_internal::DevBuf;
+
+struct spfft_plan_s {
+    int R = 0, C = 0, V = 0, P = 0, M = 0, M1 = 0, M2 = 0;
+    spfft_opts opts{};
+    bool packed = false;
+    bool colpass_full = false;
+    spfft::RowPassArgs base{};  // kernel argument template (pointers to the tables filled in)
+    DevBuf h, wr_hi, wr_lo, wr_full, tw2, twM, bhat, mfma_dft, X0, T;
+    // CSC workspace
+    int nnz_cap = -1;
+    int last_nnz = 0;
+    bool have_csc = false;
+    DevBuf counts, colstart, sparse_cols, col_ptr, totals, err, scan_tmp, hist, hist_off;
+    DevBuf keys0, keys1, vals0, vals1, row_idx;
+    size_t table_bytes = 0;
+    // optional per-kernel event timing (spfft_profile_*)
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    // event groups recorded since the last read: kind 0 = CSC build (2 events),
+    // kind 1 = row-pass tile (4 events)
+    std::vector<std::pair<int, size_t>> ev_groups;
+    ~spfft_plan_s() {
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    }
+    hipEvent_t* take_events(int n) {
+        while (ev_pool.size() < ev_used + n) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            ev_pool.push_back(e);
+        }
+        hipEvent_t* r = &ev_pool[ev_used];
+        ev_used += n;
+        return r;
+    }
+};
+
+
+namespace spfft_internal {
+
+spfft_status ensure_csc_capacity(spfft_plan_s& p, int nnz);
+spfft_status do_csc_build(spfft_plan_s& p, const int* d_row, const int* d_col, int nnz, hipStream_t s);
+// T / X0: workspace to use instead of the plan's own (NULL: the plan's own).
+spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t s, float2* T,
+                        float2* X0);
+// Plan without its own row-pass workspace (T, X0) when own_workspace is false.
+spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts, bool own_workspace,
+                            spfft_plan_s** out);
+// Bytes of the row-pass workspace: T (tile x M) and X0 (tile x ldx), complex64.
+void workspace_bytes(const spfft_plan_s& p, size_t* t_bytes, size_t* x0_bytes);
+
+}  // namespace spfft_internal

@@ -157,6 +157,43 @@ spfft_status spfft_run_host_coo(int rows, int cols, int nnz, const int* h_row, c
                                 const spfft_opts* opts, spfft_result* result);
 spfft_status spfft_free_output(void* d_output);
 
+/* ---- hipGraph form ------------------------------------------------------
+ * The CSC build and the whole tile loop of a plan captured once into a hipGraph
+ * and replayed with one launch (the reference's CUDA-graph variant,
+ * src/sparse_fft_bluestein.cu:2402-2549).  Device pointers are baked into the
+ * graph: replays read the same COO buffers and write the same output. */
+typedef struct spfft_graph_s* spfft_graph;
+spfft_status spfft_graph_capture(spfft_plan plan, const int* d_row, const int* d_col, int nnz, void* d_out,
+                                 int64_t out_pitch, void* stream, spfft_graph* graph);
+spfft_status spfft_graph_launch(spfft_graph graph, void* stream);
+spfft_status spfft_graph_destroy(spfft_graph graph);
+/* Replaces sparse_fft_csc_bluestein_mixed_radix_graph (include/sparse_fft.h:63): ms is
+ * the GPU time of the single graph launch. */
+spfft_status spfft_csc_bluestein_mixed_radix_graph(int rows, int cols, int nnz, const int* h_row,
+                                                   const int* h_col, int u_tile, spfft_result* result);
+
+/* ---- batched form --------------------------------------------------------
+ * `count` matrices of uneven shape, each with its own plan (opts: `count`
+ * entries, or NULL for defaults with one tile per matrix), run on `streams`
+ * worker streams (longest first) that share one row-pass workspace per stream.
+ * Inputs: all COO pairs concatenated in device memory, matrix i at
+ * [coo_off[i], coo_off[i+1]) (host array of count+1).  Output: one arena; matrix
+ * i at element offset offsets[i] with pitch pitches[i] (spfft_batch_layout).
+ * nnz_hint (may be NULL) only steers the assignment to streams. */
+typedef struct spfft_batch_s* spfft_batch;
+spfft_status spfft_batch_create(int count, const int* rows, const int* cols, const int* nnz_hint,
+                                const spfft_opts* opts, int streams, spfft_batch* batch);
+spfft_status spfft_batch_destroy(spfft_batch batch);
+spfft_status spfft_batch_layout(spfft_batch batch, int64_t* total_elems, int64_t* offsets, int* pitches,
+                                spfft_plan_info* infos);
+spfft_status spfft_batch_execute(spfft_batch batch, const int* d_row, const int* d_col,
+                                 const int64_t* coo_off, void* d_out, void* stream);
+/* Captures the fork / per-matrix work / join sequence of spfft_batch_execute into one graph. */
+spfft_status spfft_batch_capture(spfft_batch batch, const int* d_row, const int* d_col,
+                                 const int64_t* coo_off, void* d_out, spfft_graph* graph);
+/* Synchronises `stream`; SPFFT_ERR_BAD_INPUT names the first matrix with an out-of-range index. */
+spfft_status spfft_batch_check_input(spfft_batch batch, void* stream);
+
 /* Smooth lengths and the radix order of the reference planner
  * (src/sparse_fft_bluestein.cu:1975-2011). */
 int spfft_next_3_smooth(int n);
