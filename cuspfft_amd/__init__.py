@@ -37,7 +37,7 @@ class Opts(ctypes.Structure):
     _fields_ = [("length_rule", ctypes.c_int), ("u_tile", ctypes.c_int),
                 ("precision", ctypes.c_int), ("fft_len", ctypes.c_int), ("m1", ctypes.c_int),
                 ("conv_rows", ctypes.c_int), ("colpass_rows", ctypes.c_int),
-                ("colpass_table", ctypes.c_int)]
+                ("colpass_table", ctypes.c_int), ("row_limit", ctypes.c_int)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -88,6 +88,13 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"cuspfft_amd: HIP library not built ({LIB_PATH}); "
                            "run `python -m cuspfft_amd.build`")
+    # torch ships its own HIP runtime with the same SONAME (libamdhip64.so.7).
+    # Importing it first makes the library bind to that one runtime; loading the
+    # library first would put a second HIP runtime in the process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     sig = {
         "spfft_opts_default": (None, [ctypes.POINTER(Opts)]),
@@ -238,12 +245,12 @@ def _stream_handle(stream=None):
 
 
 def _opts(length_rule=SMOOTH_3, u_tile=128, fft_len=0, m1=0, conv_rows=0, precision=PREC_FP32,
-          colpass_rows=0, colpass_table=0):
+          colpass_rows=0, colpass_table=0, row_limit=0):
     o = Opts()
     load().spfft_opts_default(ctypes.byref(o))
     o.length_rule, o.u_tile, o.precision = length_rule, u_tile, precision
     o.fft_len, o.m1, o.conv_rows = fft_len, m1, conv_rows
-    o.colpass_rows, o.colpass_table = colpass_rows, colpass_table
+    o.colpass_rows, o.colpass_table, o.row_limit = colpass_rows, colpass_table, row_limit
     return o
 
 
@@ -267,10 +274,10 @@ class Plan:
 
     def __init__(self, rows: int, cols: int, length_rule: int = SMOOTH_3, u_tile: int = 128,
                  fft_len: int = 0, m1: int = 0, conv_rows: int = 0, precision: int = PREC_FP32,
-                 colpass_rows: int = 0, colpass_table: int = 0):
+                 colpass_rows: int = 0, colpass_table: int = 0, row_limit: int = 0):
         L = load()
         o = _opts(length_rule, u_tile, fft_len, m1, conv_rows, precision, colpass_rows,
-                  colpass_table)
+                  colpass_table, row_limit)
         self._h = _vp()
         _check(L.spfft_plan_create(int(rows), int(cols), ctypes.byref(o), ctypes.byref(self._h)))
         inf = PlanInfo()

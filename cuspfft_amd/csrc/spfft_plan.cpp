@@ -350,7 +350,7 @@ spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t
     // Rows u in [0, R/2] are computed; each also writes its mirror row R - u
     // (X[R-u][v] = conj(X[u][(C - v) mod C]) for a real input).
     const int tile = p.opts.u_tile;
-    const int rows = p.R / 2 + 1;
+    const int rows = p.opts.row_limit > 0 ? std::min(p.R / 2 + 1, p.opts.row_limit) : p.R / 2 + 1;
     for (int u0 = 0; u0 < rows; u0 += tile) {
         a.u_base = u0;
         a.tile_rows = std::min(tile, rows - u0);

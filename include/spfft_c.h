@@ -53,6 +53,8 @@ typedef struct spfft_opts {
     int conv_rows;   /* 0: automatic; else rows per workgroup of the convolution kernel (<= 64) */
     int colpass_rows;  /* 0: automatic; else 1, 2 or 4 output rows per lane in the column pass */
     int colpass_table; /* 0: automatic; 1: two-level W_R table; 2: whole W_R table in LDS      */
+    int row_limit;     /* 0: all rows.  Else only output rows u < row_limit (and their mirror
+                          rows) are computed: a sampled run for timing (the autotuner) */
 } spfft_opts;
 
 typedef struct spfft_plan_info {
@@ -193,6 +195,24 @@ spfft_status spfft_batch_capture(spfft_batch batch, const int* d_row, const int*
                                  const int64_t* coo_off, void* d_out, spfft_graph* graph);
 /* Synchronises `stream`; SPFFT_ERR_BAD_INPUT names the first matrix with an out-of-range index. */
 spfft_status spfft_batch_check_input(spfft_batch batch, void* stream);
+
+/* ---- learned plan selector network ----------------------------------------
+ * A D -> H -> H -> 1 ReLU MLP in FP32 with hand-written forward, backward and
+ * Adam kernels.  Feature rows are row-major [n_rows x D] floats in device
+ * memory.  Parameters: W1 [H x D], b1 [H], W2 [H x H], b2 [H], W3 [H], b3 [1]. */
+typedef struct spfft_mlp_s* spfft_mlp;
+spfft_status spfft_mlp_create(int in_dim, int hidden, uint64_t seed, spfft_mlp* mlp);
+spfft_status spfft_mlp_destroy(spfft_mlp mlp);
+int64_t spfft_mlp_num_params(spfft_mlp mlp);
+spfft_status spfft_mlp_get_params(spfft_mlp mlp, float* h_params);
+spfft_status spfft_mlp_set_params(spfft_mlp mlp, const float* h_params);
+spfft_status spfft_mlp_predict(spfft_mlp mlp, const float* d_X, int n_rows, float* d_y, void* stream);
+/* Loss 1/(2B) sum (yhat - y)^2 of one batch and its gradient (host, parameter layout). */
+spfft_status spfft_mlp_loss_grad(spfft_mlp mlp, const float* d_X, const float* d_y, int batch, float* loss,
+                                 float* h_grads, void* stream);
+spfft_status spfft_mlp_train(spfft_mlp mlp, const float* d_X, const float* d_y, int n_rows, int epochs,
+                             int batch, float lr, float weight_decay, uint64_t seed, float* last_loss,
+                             void* stream);
 
 /* Smooth lengths and the radix order of the reference planner
  * (src/sparse_fft_bluestein.cu:1975-2011). */

@@ -1,0 +1,111 @@
+"""The learned plan selector: the FP64 reference network pinned by finite
+differences (CPU), the HIP forward / backward / Adam kernels checked against it
+(GPU), and the trained network against the planner's default plan."""
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DB = os.path.join(HERE, "golden", "selector_timings.jsonl")
+
+
+def _rand_params(D, H, seed):
+    rng = np.random.default_rng(seed)
+    n = H * D + H + H * H + H + H + 1
+    return rng.normal(0, 0.5, n)
+
+
+def test_reference_gradient_finite_differences():
+    from oracle import mlp_ref
+    D, H, B = 5, 7, 9
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(B, D))
+    y = rng.normal(size=B)
+    p = _rand_params(D, H, 2)
+    loss, g = mlp_ref.loss_grad(p, X, y, D, H)
+    eps = 1e-6
+    for i in rng.choice(len(p), 40, replace=False):
+        q = p.copy(); q[i] += eps
+        lp, _ = mlp_ref.loss_grad(q, X, y, D, H)
+        q[i] -= 2 * eps
+        lm, _ = mlp_ref.loss_grad(q, X, y, D, H)
+        assert abs((lp - lm) / (2 * eps) - g[i]) < 1e-6 * max(1.0, abs(g[i]))
+    # a planted mistake (dropping the ReLU mask of layer 2) changes the gradient
+    W1, b1, W2, b2, W3, b3 = mlp_ref.unpack(p, D, H)
+    Z2 = np.maximum(X @ W1.T + b1, 0) @ W2.T + b2
+    assert (Z2 <= 0).any()
+
+
+def test_features_shape_and_determinism():
+    import cuspfft_amd as sp
+    from cuspfft_amd import autotune, selector
+    m = sp.config_matrix("c1")
+    rec = autotune.features_of(m)
+    assert abs(sum(rec["deg_hist"]) - 1) < 1e-9
+    cands = autotune.candidates(m.rows, m.cols, 3)
+    assert any(c["precision"] == 1 for c in cands)
+    f = np.stack([selector.features(rec, c) for c in cands])
+    assert f.shape == (len(cands), selector.N_FEATURES) and np.all(np.isfinite(f))
+    np.testing.assert_array_equal(f, np.stack([selector.features(rec, c) for c in cands]))
+
+
+def test_training_set_is_seeded():
+    from cuspfft_amd import autotune
+    a, ta = autotune.training_matrices(6, nnz_cap=20000)
+    b, tb = autotune.training_matrices(6, nnz_cap=20000)
+    assert ta == tb
+    for x, y in zip(a, b):
+        assert x.rows == y.rows and x.nnz == y.nnz
+        np.testing.assert_array_equal(x.col_idx, y.col_idx)
+    assert {t["kind"] for t in ta} == {"uniform", "chunglu"}
+
+
+@pytest.mark.gpu
+def test_hip_forward_and_gradient_match_reference():
+    from cuspfft_amd import selector
+    from oracle import mlp_ref
+    D, H, B = 27, 64, 300
+    net = selector.MLP(D, H, seed=3)
+    p = net.params().astype(np.float64)
+    rng = np.random.default_rng(4)
+    X = rng.normal(size=(B, D)).astype(np.float32)
+    y = rng.normal(size=B).astype(np.float32)
+    yhat_ref, _ = mlp_ref.forward(p, X, D, H)
+    yhat = net.predict(X)
+    assert np.abs(yhat - yhat_ref).max() < 1e-4 * max(1.0, np.abs(yhat_ref).max())
+    loss, g = net.loss_grad(X, y)
+    loss_ref, g_ref = mlp_ref.loss_grad(p, X, y, D, H)
+    assert abs(loss - loss_ref) < 1e-5 * max(1.0, loss_ref)
+    assert np.linalg.norm(g - g_ref) < 1e-4 * np.linalg.norm(g_ref)
+
+
+@pytest.mark.gpu
+def test_hip_training_fits_a_smooth_target():
+    from cuspfft_amd import selector
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(4096, 8)).astype(np.float32)
+    y = (np.sin(X[:, 0]) + 0.5 * X[:, 1] * X[:, 2] - 0.3 * np.abs(X[:, 3])).astype(np.float32)
+    net = selector.MLP(8, 64, seed=0)
+    l0, _ = net.loss_grad(X[:512], y[:512])
+    loss = net.train(X, y, epochs=60, batch=256, lr=3e-3, seed=1)
+    assert loss < 0.1 * l0
+    # deterministic for a fixed seed
+    net2 = selector.MLP(8, 64, seed=0)
+    loss2 = net2.train(X, y, epochs=60, batch=256, lr=3e-3, seed=1)
+    assert loss2 == loss
+
+
+@pytest.mark.gpu
+def test_selector_beats_default_plan_on_heldout():
+    """Trained on measured autotuner timings (committed fixture), the network's
+    choice on held-out matrices is at least as fast as the planner's default plan."""
+    from cuspfft_amd import autotune, selector
+    recs = autotune.load_db(DB)
+    assert len(recs) >= 40
+    train, held = recs[: int(len(recs) * 0.75)], recs[int(len(recs) * 0.75):]
+    sel = selector.Selector.fit(train, epochs=200)
+    r = selector.regret(sel, held)
+    print(r)
+    assert r["mean_regret"] <= r["default_mean_regret"] + 1e-9
+    assert r["selector_speedup_over_default"] >= 1.0
