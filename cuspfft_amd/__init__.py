@@ -529,3 +529,28 @@ def sparse_fft_csc_bluestein_mixed_radix(coo: COOMatrix, u_tile: int = 128) -> S
 def sparse_fft_csc_bluestein_cufft_smooth(coo: COOMatrix, u_tile: int = 128) -> SparseFFTResult:
     """Bluestein at next_7_smooth(2*cols-1); reference src/sparse_fft_bluestein.cu:2557-2719."""
     return _run_reference_shaped(coo, u_tile, SMOOTH_7)
+
+
+def sparse_fft_csc_bluestein_mixed_radix_graph(coo: COOMatrix, u_tile: int = 128) -> SparseFFTResult:
+    """The mixed-radix transform with the CSC build and the tile loop captured
+    into one hipGraph and launched once; ms is the GPU time of that launch.
+    Reference src/sparse_fft_bluestein.cu:2402-2549."""
+    import torch
+    if u_tile <= 0:
+        raise RuntimeError("CSC tile size must be positive")
+    free0, _ = torch.cuda.mem_get_info()
+    plan = Plan(coo.rows, coo.cols, length_rule=SMOOTH_3, u_tile=u_tile)
+    d_row = torch.as_tensor(np.ascontiguousarray(coo.row_idx, np.int32)).cuda()
+    d_col = torch.as_tensor(np.ascontiguousarray(coo.col_idx, np.int32)).cuda()
+    out = plan.alloc_output()
+    torch.cuda.synchronize()
+    graph = capture_plan(plan, d_row, d_col, out)
+    e0, e1 = (torch.cuda.Event(enable_timing=True) for _ in range(2))
+    e0.record()
+    graph.launch()
+    e1.record()
+    e1.synchronize()
+    plan.check_input()
+    free1, _ = torch.cuda.mem_get_info()
+    ms = e0.elapsed_time(e1)
+    return SparseFFTResult(out, plan.pitch, ms, max(free0 - free1, 0), ms)

@@ -1,10 +1,11 @@
 // cli_main.cpp -- benchmark driver for the two reported variants, keeping the
 // reference CLI's flags and table format (reference src/main.cu:217-311 for
 // the flags, :45-83 for the warm-up + median rule and the table lines,
-// :486-524 for the two variants).
+// :486-524 and :649-659 for the variants).
 //
 //   spfft_cli <matrix.mtx | gen:c1 | gen:c2 | gen:c3 | gen:c5> [options]
 //     --csc-mixed-radix   Bluestein at next_3_smooth(2*cols-1)
+//     --csc-mixed-radix-graph  the same, captured into one hipGraph and launched once
 //     --csc-cufft-smooth  Bluestein at next_7_smooth(2*cols-1) (hand-written FFTs, no library)
 //     --csc-tile N        output rows per tile (default 1024, as the reference)
 //     --repeat N          1 warm-up + N timed runs; median, min, max
@@ -78,7 +79,8 @@ void print_bench(const char* label, float ms, size_t mem, int n, float mn, float
 
 void usage(const char* p) {
     fprintf(stderr,
-            "Usage: %s <matrix.mtx | gen:c1|c2|c3|c5> [--csc-mixed-radix] [--csc-cufft-smooth]\n"
+            "Usage: %s <matrix.mtx | gen:c1|c2|c3|c5> [--csc-mixed-radix] [--csc-mixed-radix-graph]\n"
+            "          [--csc-cufft-smooth]\n"
             "          [--csc-tile N] [--repeat N] [--sparse-only] [--cpu-reference]\n",
             p);
 }
@@ -88,13 +90,14 @@ void usage(const char* p) {
 int main(int argc, char** argv) {
     if (argc < 2) { usage(argv[0]); return 1; }
     if (!std::strcmp(argv[1], "--help") || !std::strcmp(argv[1], "-h")) { usage(argv[0]); return 0; }
-    bool mixed = false, smooth = false, cpu_ref = false;
+    bool mixed = false, mixed_graph = false, smooth = false, cpu_ref = false;
     int tile = 1024, repeat = 1;
     try {
         for (int i = 2; i < argc; i++) {
             const std::string f(argv[i]);
             if (f == "--sparse-only") {}
             else if (f == "--csc-mixed-radix") mixed = true;
+            else if (f == "--csc-mixed-radix-graph") mixed_graph = true;
             else if (f == "--csc-cufft-smooth") smooth = true;
             else if (f == "--cpu-reference") cpu_ref = true;
             else if (f == "--csc-tile") {
@@ -118,8 +121,10 @@ int main(int argc, char** argv) {
         if (cpu_ref)
             printf("%-26s  %10s  %12s  [%s]\n", "CPU reference (FFTW3)", "FAIL", "FAIL",
                    "FFTW3 not available in this build");
-        struct V { bool on; const char* label; bool s7; };
-        const V vs[] = {{mixed, "Sparse CSC (mixed-radix)", false}, {smooth, "Sparse CSC (cufft smooth)", true}};
+        struct V { bool on; const char* label; int kind; };
+        const V vs[] = {{mixed, "Sparse CSC (mixed-radix)", 0},
+                        {mixed_graph, "Sparse CSC (mr-graph)", 1},
+                        {smooth, "Sparse CSC (cufft smooth)", 2}};
         for (const V& v : vs) {
             if (!v.on) continue;
             try {
@@ -127,8 +132,9 @@ int main(int argc, char** argv) {
                 SparseFFTResult r = run_repeated(
                     repeat,
                     [&] {
-                        return v.s7 ? sparse_fft_csc_bluestein_cufft_smooth(coo, tile)
-                                    : sparse_fft_csc_bluestein_mixed_radix(coo, tile);
+                        if (v.kind == 2) return sparse_fft_csc_bluestein_cufft_smooth(coo, tile);
+                        if (v.kind == 1) return sparse_fft_csc_bluestein_mixed_radix_graph(coo, tile);
+                        return sparse_fft_csc_bluestein_mixed_radix(coo, tile);
                     },
                     &mn, &mx);
                 print_bench(v.label, r.ms, r.mem_bytes, repeat, mn, mx);

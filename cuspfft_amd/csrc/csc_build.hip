@@ -183,10 +183,12 @@ __global__ void k_radix_hist(const int* __restrict__ keys, int nnz, int shift, i
 // Stable scatter.  Wave w of block b owns elements
 // [b*kSortBlock + w*64*kSortIters, +64*kSortIters) in order; inside a wave the
 // rank of a lane among equal digits is the popcount of the matching lanes
-// below it, found with 8 ballots.
+// below it, found with 8 ballots.  The final pass writes rows outside [0, R)
+// as 0, so that the column pass never indexes its tables with an invalid row
+// (such input is reported through the err flag; its output is undefined).
 template <bool FINAL, bool PACKED>
 __global__ void k_radix_scatter(const int* __restrict__ keys_in, const int* __restrict__ vals_in,
-                                int nnz, int shift, const int* __restrict__ offs, int nblk,
+                                int nnz, int R, int shift, const int* __restrict__ offs, int nblk,
                                 int* __restrict__ keys_out, int* __restrict__ vals_out,
                                 void* __restrict__ final_out) {
     __shared__ int s_cnt[kSortWaves][256];
@@ -230,8 +232,9 @@ __global__ void k_radix_scatter(const int* __restrict__ keys_in, const int* __re
         if (valid) {
             if (lane == leader) cnt[d] = base + __popcll(peer);
             const int pos = base + rank;
-            const int val = vals_in[i];
+            int val = vals_in[i];
             if (FINAL) {
+                if ((uint32_t)val >= (uint32_t)R) val = 0;
                 if (PACKED) ((uint16_t*)final_out)[pos] = (uint16_t)val;
                 else ((int*)final_out)[pos] = val;
             } else {
@@ -301,13 +304,13 @@ hipError_t launch_csc_build(const CscBuildArgs& a, hipStream_t s) {
         if (fin) {
             if (a.packed)
                 hipLaunchKernelGGL((k_radix_scatter<true, true>), dim3(nblk), dim3(kSortThreads), 0, s, kin, vin,
-                                   nnz, shift, a.hist_off, nblk, nullptr, nullptr, a.row_idx);
+                                   nnz, R, shift, a.hist_off, nblk, nullptr, nullptr, a.row_idx);
             else
                 hipLaunchKernelGGL((k_radix_scatter<true, false>), dim3(nblk), dim3(kSortThreads), 0, s, kin, vin,
-                                   nnz, shift, a.hist_off, nblk, nullptr, nullptr, a.row_idx);
+                                   nnz, R, shift, a.hist_off, nblk, nullptr, nullptr, a.row_idx);
         } else {
             hipLaunchKernelGGL((k_radix_scatter<false, false>), dim3(nblk), dim3(kSortThreads), 0, s, kin, vin,
-                               nnz, shift, a.hist_off, nblk, kout, vout, nullptr);
+                               nnz, R, shift, a.hist_off, nblk, kout, vout, nullptr);
             kin = kout;
             vin = vout;
             kout = (kout == a.keys[0]) ? a.keys[1] : a.keys[0];

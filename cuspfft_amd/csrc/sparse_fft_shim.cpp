@@ -10,15 +10,25 @@
 
 namespace {
 
-SparseFFTResult run(const COOMatrix& coo, int u_tile, bool smooth7) {
+enum class Variant { MixedRadix, MixedRadixGraph, Smooth7 };
+
+SparseFFTResult run(const COOMatrix& coo, int u_tile, Variant v) {
     if (coo.nnz < 0 || (size_t)coo.nnz > coo.row_idx.size() || (size_t)coo.nnz > coo.col_idx.size())
         throw std::runtime_error("COOMatrix nnz does not match its index arrays");
     spfft_result r;
-    const spfft_status st =
-        smooth7 ? spfft_csc_bluestein_cufft_smooth(coo.rows, coo.cols, coo.nnz, coo.row_idx.data(),
-                                                   coo.col_idx.data(), u_tile, &r)
-                : spfft_csc_bluestein_mixed_radix(coo.rows, coo.cols, coo.nnz, coo.row_idx.data(),
-                                                  coo.col_idx.data(), u_tile, &r);
+    const int* hr = coo.row_idx.data();
+    const int* hc = coo.col_idx.data();
+    spfft_status st;
+    switch (v) {
+        case Variant::Smooth7:
+            st = spfft_csc_bluestein_cufft_smooth(coo.rows, coo.cols, coo.nnz, hr, hc, u_tile, &r);
+            break;
+        case Variant::MixedRadixGraph:
+            st = spfft_csc_bluestein_mixed_radix_graph(coo.rows, coo.cols, coo.nnz, hr, hc, u_tile, &r);
+            break;
+        default:
+            st = spfft_csc_bluestein_mixed_radix(coo.rows, coo.cols, coo.nnz, hr, hc, u_tile, &r);
+    }
     if (st != SPFFT_OK) throw std::runtime_error(spfft_last_error());
     SparseFFTResult out;
     out.d_output = static_cast<hipFloatComplex*>(r.d_output);
@@ -31,11 +41,15 @@ SparseFFTResult run(const COOMatrix& coo, int u_tile, bool smooth7) {
 }  // namespace
 
 SparseFFTResult sparse_fft_csc_bluestein_mixed_radix(const COOMatrix& coo, int u_tile) {
-    return run(coo, u_tile, false);
+    return run(coo, u_tile, Variant::MixedRadix);
+}
+
+SparseFFTResult sparse_fft_csc_bluestein_mixed_radix_graph(const COOMatrix& coo, int u_tile) {
+    return run(coo, u_tile, Variant::MixedRadixGraph);
 }
 
 SparseFFTResult sparse_fft_csc_bluestein_cufft_smooth(const COOMatrix& coo, int u_tile) {
-    return run(coo, u_tile, true);
+    return run(coo, u_tile, Variant::Smooth7);
 }
 
 COOMatrix read_mtx(const std::string& path) {

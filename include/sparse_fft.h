@@ -19,6 +19,12 @@ struct SparseFFTResult {
 // sparse_fft_csc_bluestein_mixed_radix (reference src/sparse_fft_bluestein.cu:2256-2392).
 SparseFFTResult sparse_fft_csc_bluestein_mixed_radix(const COOMatrix& coo, int u_tile = 128);
 
+// Same transform as sparse_fft_csc_bluestein_mixed_radix, with the CSC build and the
+// whole tile loop captured into one hipGraph and launched once; ms is the GPU time of
+// that launch.  Replaces sparse_fft_csc_bluestein_mixed_radix_graph (reference
+// include/sparse_fft.h:63, src/sparse_fft_bluestein.cu:2402-2549).
+SparseFFTResult sparse_fft_csc_bluestein_mixed_radix_graph(const COOMatrix& coo, int u_tile = 128);
+
 // Bluestein length next_7_smooth(2*cols - 1), hand-written radix-2/3/4/5/7/8
 // sub-FFTs in place of cuFFT.  Replaces sparse_fft_csc_bluestein_cufft_smooth
 // (reference src/sparse_fft_bluestein.cu:2557-2719).

@@ -29,6 +29,7 @@ def test_library_exports_every_declared_symbol():
     dem = subprocess.run(["nm", "-DC", "--defined-only", sp.LIB_PATH], capture_output=True,
                          text=True, check=True).stdout
     for sym in ("sparse_fft_csc_bluestein_mixed_radix(COOMatrix const&, int)",
+                "sparse_fft_csc_bluestein_mixed_radix_graph(COOMatrix const&, int)",
                 "sparse_fft_csc_bluestein_cufft_smooth(COOMatrix const&, int)",
                 "read_mtx("):
         assert sym in dem, sym

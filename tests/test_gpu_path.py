@@ -217,14 +217,23 @@ def test_errors(gpu):
     with pytest.raises(sp.SpfftError) as ei:
         sp.Plan(10, 10, u_tile=-3)
     assert ei.value.status == 1
-    bad_r = np.array([0, 5, 10], np.int32)
-    bad_c = np.array([0, 1, 2], np.int32)
-    plan = sp.Plan(10, 10)
-    out = plan.alloc_output()
-    plan.execute(_dev(bad_r), _dev(bad_c), out)
-    with pytest.raises(sp.SpfftError) as ei:
-        plan.check_input()
-    assert ei.value.status == 5
+    # out-of-range rows and columns (too large and negative): the kernels skip
+    # them in the column counts and store invalid rows as 0, so every access
+    # stays in bounds; check_input reports the bad input afterwards
+    for bad_r, bad_c in (([0, 5, 10], [0, 1, 2]), ([-7, 3, 2, 1], [4, 1, 70000, -1])):
+        plan = sp.Plan(10, 10)
+        out = plan.alloc_output()
+        plan.execute(_dev(np.array(bad_r, np.int32)), _dev(np.array(bad_c, np.int32)), out)
+        with pytest.raises(sp.SpfftError) as ei:
+            plan.check_input()
+        assert ei.value.status == 5
+    # a valid transform on the same plan afterwards is unaffected
+    plan.execute(_dev(np.array([1, 2], np.int32)), _dev(np.array([3, 4], np.int32)), out)
+    plan.check_input()
+    got = out[:, :6].cpu().numpy()
+    u, v = np.arange(10)[:, None], np.arange(6)[None, :]
+    want = np.exp(-2j * np.pi * (u * 1 + v * 3) / 10) + np.exp(-2j * np.pi * (u * 2 + v * 4) / 10)
+    np.testing.assert_allclose(got, want, atol=2e-6)
 
 
 def test_reference_shaped_entry_points(gpu, oracle_mod):
@@ -401,6 +410,36 @@ def test_reference_graph_entry_point(gpu, oracle_mod):
     assert L.spfft_free_output(res.d_output) == 0
     ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, nthreads=8)
     assert oracle_mod.rel_l2(X, ref) <= REL_L2_PLAIN and res.ms > 0
+
+
+def test_python_mirror_graph(gpu, oracle_mod):
+    sp = gpu
+    m = sp.config_matrix("c1")
+    r = sp.sparse_fft_csc_bluestein_mixed_radix_graph(m, 128)
+    X = r.d_output[:, : m.cols // 2 + 1].cpu().numpy()
+    ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, nthreads=8)
+    assert r.ms > 0 and oracle_mod.rel_l2(X, ref) <= REL_L2_PLAIN
+    with pytest.raises(RuntimeError, match="CSC tile size must be positive"):
+        sp.sparse_fft_csc_bluestein_mixed_radix_graph(m, 0)
+
+
+def test_cli_reference_flags(gpu):
+    """The benchmark driver with the reference's flags runs all three variants
+    through the C++ drop-in entry points and prints the reference's table lines."""
+    import os
+    import subprocess
+    sp = gpu
+    cli = os.path.join(os.path.dirname(sp.LIB_PATH), "spfft_cli")
+    res = subprocess.run([cli, "gen:c1", "--csc-mixed-radix", "--csc-mixed-radix-graph",
+                          "--csc-cufft-smooth", "--csc-tile", "256", "--repeat", "3"],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    for label in ("Sparse CSC (mixed-radix)", "Sparse CSC (mr-graph)", "Sparse CSC (cufft smooth)"):
+        assert label in res.stdout
+    assert "OOM" not in res.stdout
+    bad = subprocess.run([cli, "gen:c1", "--csc-mixed-radix", "--csc-tile", "0"],
+                         capture_output=True, text=True, timeout=60)
+    assert bad.returncode == 1 and "--csc-tile must be positive" in bad.stderr
 
 
 def _batch_mats(n, seed):
