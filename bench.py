@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--precision", type=int, default=0, help="0 plain FP32, 1 BF16x3 MFMA")
     ap.add_argument("--colpass-rows", type=int, default=0)
     ap.add_argument("--colpass-table", type=int, default=0)
+    ap.add_argument("--row-kernel", type=int, default=0,
+                    help="0 auto (fused row kernel when M has a shape), 1 four-step kernels, 2 fused")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows sampled by the CPU baseline")
     ap.add_argument("--no-check", action="store_true")
@@ -214,7 +216,7 @@ def main():
     m = sp.config_matrix(args.config)
     tile = args.tile if args.tile > 0 else m.rows
     plan = sp.Plan(m.rows, m.cols, length_rule=args.rule, u_tile=tile, m1=args.m1,
-                   precision=args.precision,
+                   precision=args.precision, row_kernel=args.row_kernel,
                    colpass_rows=args.colpass_rows, colpass_table=args.colpass_table)
     d_row = torch.as_tensor(m.row_idx).to(dev)
     d_col = torch.as_tensor(m.col_idx).to(dev)
@@ -235,6 +237,10 @@ def main():
     prof = plan.profile_read(stream)
     plan.profile(False)
     plan.check_input()
+    if plan.info["row_kernel"] == sp.ROW_FUSED:
+        # one kernel (k_rowfused) does fft1 + conv + ifft1; its time is reported under conv
+        prof = {("row_fused" if k == "conv" else k): v for k, v in prof.items()
+                if k not in ("fft1", "ifft1_finalize")}
 
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     value = 1000.0 / ms_per_step
@@ -275,8 +281,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        n = args.cpu_rows if args.cpu_rows > 0 else {"c1": 2310, "c2": 400, "c3": 300,
-                                                     "c5": 80}.get(args.config, 200)
+        # about 10 s of single-thread work on the host
+        n = args.cpu_rows if args.cpu_rows > 0 else {"c1": 2310, "c2": 4000, "c3": 3000,
+                                                     "c5": 800}.get(args.config, 2000)
         cpu = cpu_baseline(m, args.rule, n)
 
     info = plan.info
@@ -301,6 +308,7 @@ def main():
             "m2": info["m2"], "u_tile": info["u_tile"], "conv_rows": info["conv_rows"],
             "conv_radix": info["conv_radix"], "colpass_rows": info["colpass_rows"],
             "colpass_full_table": info["colpass_full_table"],
+            "row_kernel": {1: "four-step", 2: "fused"}[info["row_kernel"]],
         },
         "rel_l2_vs_fp64": rel,
         "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 5),

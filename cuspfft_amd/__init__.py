@@ -23,6 +23,8 @@ SMOOTH_3 = 3
 SMOOTH_7 = 7
 PREC_FP32 = 0
 PREC_BF16X3 = 1
+# row-pass kernels (spfft_opts.row_kernel)
+ROW_AUTO, ROW_FOUR_STEP, ROW_FUSED = 0, 1, 2
 
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "HIP", 3: "ALLOC", 4: "UNSUPPORTED", 5: "BAD_INPUT"}
 
@@ -37,7 +39,8 @@ class Opts(ctypes.Structure):
     _fields_ = [("length_rule", ctypes.c_int), ("u_tile", ctypes.c_int),
                 ("precision", ctypes.c_int), ("fft_len", ctypes.c_int), ("m1", ctypes.c_int),
                 ("conv_rows", ctypes.c_int), ("colpass_rows", ctypes.c_int),
-                ("colpass_table", ctypes.c_int), ("row_limit", ctypes.c_int)]
+                ("colpass_table", ctypes.c_int), ("row_limit", ctypes.c_int),
+                ("row_kernel", ctypes.c_int)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -47,7 +50,8 @@ class PlanInfo(ctypes.Structure):
                 ("conv_rows", ctypes.c_int), ("conv_fast", ctypes.c_int),
                 ("conv_stages", ctypes.c_int), ("conv_radix", ctypes.c_int * 8),
                 ("colpass_rows", ctypes.c_int), ("colpass_full_table", ctypes.c_int),
-                ("precision", ctypes.c_int), ("workspace_bytes", ctypes.c_size_t)]
+                ("precision", ctypes.c_int), ("workspace_bytes", ctypes.c_size_t),
+                ("row_kernel", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -129,6 +133,7 @@ def load():
         "spfft_next_3_smooth": (_i, [_i]),
         "spfft_next_7_smooth": (_i, [_i]),
         "spfft_factor_3_smooth": (_i, [_i, _vp, _i]),
+        "spfft_fused_lengths": (_i, [_vp, _i]),
         "spfft_coo_generate": (_i, [ctypes.c_char_p, _i, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
                                     ctypes.POINTER(HostCoo)]),
@@ -163,6 +168,13 @@ def factor_3_smooth(n: int) -> list[int]:
     if k < 0:
         raise RuntimeError("factor_3_smooth: not 3-smooth")
     return list(buf[:k])
+
+
+def fused_lengths() -> list[int]:
+    """Bluestein lengths M with a fused row-kernel shape (row_kernel=ROW_FUSED)."""
+    buf = (ctypes.c_int * 256)()
+    k = load().spfft_fused_lengths(ctypes.cast(buf, _vp), 256)
+    return list(buf[:min(k, 256)])
 
 
 # ---------------------------------------------------------------------------
@@ -245,12 +257,13 @@ def _stream_handle(stream=None):
 
 
 def _opts(length_rule=SMOOTH_3, u_tile=128, fft_len=0, m1=0, conv_rows=0, precision=PREC_FP32,
-          colpass_rows=0, colpass_table=0, row_limit=0):
+          colpass_rows=0, colpass_table=0, row_limit=0, row_kernel=ROW_AUTO):
     o = Opts()
     load().spfft_opts_default(ctypes.byref(o))
     o.length_rule, o.u_tile, o.precision = length_rule, u_tile, precision
     o.fft_len, o.m1, o.conv_rows = fft_len, m1, conv_rows
     o.colpass_rows, o.colpass_table, o.row_limit = colpass_rows, colpass_table, row_limit
+    o.row_kernel = row_kernel
     return o
 
 
@@ -274,10 +287,11 @@ class Plan:
 
     def __init__(self, rows: int, cols: int, length_rule: int = SMOOTH_3, u_tile: int = 128,
                  fft_len: int = 0, m1: int = 0, conv_rows: int = 0, precision: int = PREC_FP32,
-                 colpass_rows: int = 0, colpass_table: int = 0, row_limit: int = 0):
+                 colpass_rows: int = 0, colpass_table: int = 0, row_limit: int = 0,
+                 row_kernel: int = ROW_AUTO):
         L = load()
         o = _opts(length_rule, u_tile, fft_len, m1, conv_rows, precision, colpass_rows,
-                  colpass_table, row_limit)
+                  colpass_table, row_limit, row_kernel)
         self._h = _vp()
         _check(L.spfft_plan_create(int(rows), int(cols), ctypes.byref(o), ctypes.byref(self._h)))
         inf = PlanInfo()

@@ -1,5 +1,5 @@
 """Autotuner for the per-matrix plan (Bluestein length M, its split M = M1 * M2,
-arithmetic form, column-pass rows per lane).
+row-pass kernel (four-step or fused), arithmetic form, column-pass rows per lane).
 
 For each matrix it times every candidate plan on the GPU with the library's own
 HIP-event stage timing (csc_build, colpass, fft1, conv, ifft1_finalize), keeps
@@ -18,7 +18,7 @@ import time
 
 import numpy as np
 
-from . import Plan, SpfftError, describe_plan, PREC_BF16X3
+from . import Plan, SpfftError, describe_plan, fused_lengths, PREC_BF16X3, ROW_FOUR_STEP, ROW_FUSED
 
 # M1 lengths tried as alternatives to the planner's choice (register codelets
 # with a fast convolution kernel for M / M1).
@@ -27,18 +27,21 @@ COLPASS_ROWS = (1, 2, 4)
 
 
 def plan_key(opts: dict) -> str:
-    return "r{length_rule}_p{precision}_m{m1}_c{colpass_rows}".format(
-        length_rule=opts.get("length_rule", 3), precision=opts.get("precision", 0),
-        m1=opts.get("m1", 0), colpass_rows=opts.get("colpass_rows", 0))
+    return "r{length_rule}_M{fft_len}_p{precision}_m{m1}_k{row_kernel}_c{colpass_rows}".format(
+        length_rule=opts.get("length_rule", 3), fft_len=opts.get("fft_len", 0),
+        precision=opts.get("precision", 0), m1=opts.get("m1", 0),
+        row_kernel=opts.get("row_kernel", 0), colpass_rows=opts.get("colpass_rows", 0))
 
 
 def candidates(rows: int, cols: int, max_m1: int = 3) -> list[dict]:
-    """Row-pass structure candidates: for both length rules, the planner's M1
-    and up to max_m1-1 alternatives with a fast convolution kernel, plus the
-    BF16x3 MFMA form."""
+    """Row-pass structure candidates: for both length rules, the four-step
+    kernels with the planner's M1 and up to max_m1-1 alternatives with a fast
+    convolution kernel, the BF16x3 MFMA form, and the fused row kernel at the
+    rule's M when it has a shape; plus the fused row kernel at the smallest
+    fused length M >= 2*cols-1."""
     out = []
     for rule in (3, 7):
-        base = describe_plan(rows, cols, length_rule=rule)
+        base = describe_plan(rows, cols, length_rule=rule, row_kernel=ROW_FOUR_STEP)
         M = base["fft_len"]
         m1s = [base["m1"]]
         for d in ALT_M1:
@@ -47,19 +50,24 @@ def candidates(rows: int, cols: int, max_m1: int = 3) -> list[dict]:
             if d in m1s or M % d:
                 continue
             try:
-                info = describe_plan(rows, cols, length_rule=rule, m1=d)
+                info = describe_plan(rows, cols, length_rule=rule, m1=d, row_kernel=ROW_FOUR_STEP)
             except SpfftError:
                 continue
             if info["conv_fast"]:
                 m1s.append(d)
         for m1 in m1s:
-            out.append(dict(length_rule=rule, m1=m1, precision=0))
-        out.append(dict(length_rule=rule, m1=0, precision=PREC_BF16X3))
+            out.append(dict(length_rule=rule, m1=m1, precision=0, row_kernel=ROW_FOUR_STEP))
+        out.append(dict(length_rule=rule, m1=0, precision=PREC_BF16X3, row_kernel=ROW_FOUR_STEP))
+        if describe_plan(rows, cols, length_rule=rule)["row_kernel"] == ROW_FUSED:
+            out.append(dict(length_rule=rule, m1=0, precision=0, row_kernel=ROW_FUSED))
+    fl = [M for M in fused_lengths() if M >= 2 * cols - 1]
+    if fl:
+        out.append(dict(length_rule=3, fft_len=fl[0], m1=0, precision=0, row_kernel=ROW_FUSED))
     # identical plans (e.g. both rules give the same M) are tried once
     seen, uniq = set(), []
     for o in out:
         info = describe_plan(rows, cols, **o)
-        sig = (info["fft_len"], info["m1"], info["precision"])
+        sig = (info["fft_len"], info["m1"], info["precision"], info["row_kernel"])
         if sig not in seen:
             seen.add(sig)
             o = dict(o, m1=info["m1"])
@@ -100,6 +108,7 @@ def tune_matrix(m, reps: int = 3, max_m1: int = 3, record: dict | None = None, r
 
     def trial(o, default=False):
         ms, st, info = time_plan(m.rows, m.cols, d_row, d_col, o, reps, row_limit)
+        o = dict(o, row_kernel=info["row_kernel"])  # the kernel the plan resolved to
         t = dict(opts=o, ms=ms, stages=st, fft_len=info["fft_len"], m1=info["m1"], m2=info["m2"])
         if default:
             t["default"] = True

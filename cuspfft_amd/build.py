@@ -31,6 +31,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
 LIB_SOURCES = ["csc_build.hip", "rowpass.hip", "rowpass_conv_a.hip", "rowpass_conv_b.hip",
                "rowpass_conv_c.hip", "rowpass_conv_d.hip", "rowpass_conv_e.hip",
                "rowpass_fft1_a.hip", "rowpass_fft1_b.hip", "rowpass_fft1_c.hip", "rowpass_mfma.hip",
+               "rowpass_fused_a.hip", "rowpass_fused_b.hip",
                "spfft_plan.cpp", "spfft_batch.cpp", "mlp.hip",
                "spfft_gen.cpp", "sparse_fft_shim.cpp"]
 CLI_SOURCES = ["cli_main.cpp"]

@@ -7,13 +7,12 @@
 // on a four-step split M = M1*M2 of the smooth Bluestein length:
 //
 //   k_colpass  pass 1.  A wave owns one column at a time and its 64 lanes own
-//              64 consecutive output rows u, so the column's row indices are
-//              wave-uniform loads and every lane sums W_R^{(u r) mod R} with the
-//              exact integer phase (a scalar fastmod for the wave's first row,
-//              then lane*r added and reduced with a float-reciprocal estimate
-//              corrected in integers).  W_R comes from a two-level LDS table.
-//              The chirp h[c] is applied, and a 64x64 LDS transpose makes the
-//              X0 row writes coalesced (the reference writes with stride M).
+//              64 consecutive output rows u, so the column is uniform over the
+//              wave; its row indices are read 64 at a time and broadcast with
+//              readlane, and every lane sums W_R^{(u r) mod R} with the exact
+//              integer phase.  W_R comes from an LDS table.  The chirp h[c] is
+//              applied and an LDS tile turns the column sums into whole X0 row
+//              segments (the reference writes with stride M).
 //   k_fft1<M1> one thread per (u, n2): the length-M1 FFT over n1 of
 //              x[M2 n1 + n2] entirely in registers (compile-time codelet; the
 //              zero padding c >= C is never loaded), times W_M^{k1 n2}.
@@ -24,6 +23,9 @@
 //   k_ifft1<M1> one thread per (u, n2): times W_M^{-k1 n2}, inverse length-M1
 //              codelet, chirp post-multiply, and the half-spectrum write of
 //              X[u][v] for v = M2 n1 + n2 <= C/2 at pitch P.
+//
+// When the plan has a k_rowfused shape for M (fused_impl.h), one kernel does
+// k_fft1 + k_conv + k_ifft1 per row on chip and T is not used.
 //
 // Only rows u <= R/2 go through the four kernels: the input is real, so each
 // column DFT is Hermitian and X[R-u][v] = conj(X[u][(C - v) mod C]).  The
@@ -37,70 +39,121 @@ namespace spfft {
 namespace {
 
 // ---------------------------------------------------------------------------
-// k_colpass: 64 columns x 64*U rows per workgroup.  Wave w takes columns
-// c0 + w + 4i; lane l takes rows u0 + l + 64t (t < U).  For a nonzero row r:
-//   ph_0 = (u0 r + l r) mod R     (scalar fastmod of u0 r, then l r added and
-//                                  reduced with a float estimate fixed in integers)
-//   ph_t = ph_{t-1} + (64 r mod R), one conditional subtraction
-// W_R^ph comes from one LDS table of R entries (FULL) or from the two-level
-// table hi[ph >> s] * lo[ph & (2^s - 1)].
+// k_colpass: a tile of 64 columns x 64*U rows per workgroup (8 waves).
+// Wave w takes columns c0 + w + 8i, one at a time, so the column is uniform over
+// the wave; lane l takes rows u0 + l + 64t (t < U).  A column's row indices are
+// read 64 at a time (one per lane) together with their per-nonzero constants,
+// computed once per lane:
+//   base = (u0 r) mod R, d64 = (64 r) mod R, w64 = W_R^{d64}
+// and then broadcast one nonzero at a time with readlane, so the inner loop has
+// no memory wait and no divergence.  Per nonzero and lane:
+//   ph = (base + l r) mod R     (x = base + l r < 64 R; q = floor(x * inv_r) with inv_r
+//                                 just below 1/R is floor(x / R) or one less, so
+//                                 x - q R < 2 R needs one conditional subtraction)
+//   w  = W_R^ph                 (LDS table: whole, or two-level hi[ph >> s] * lo[ph & mask])
+//   acc[0] += w, acc[t] += w * w64^t
+// The chirp h[c] is applied and the sums go through an LDS tile (re and im
+// planes, row stride 65: conflict-free both ways), so X0 is written in whole
+// 64-column row segments.
 // ---------------------------------------------------------------------------
+constexpr int kTileStride = 65;
+constexpr int kColThreads = 512;
+
 template <bool PACKED, bool FULL, int U>
-__global__ __launch_bounds__(kThreads) void k_colpass(RowPassArgs a) {
+__global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
     extern __shared__ float2 smem[];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    constexpr int kWaves = kColThreads / 64;
+    const int lane = threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c0 = blockIdx.x * 64;
     const int u0l = blockIdx.y * 64 * U;
     const uint32_t R = (uint32_t)a.R;
     const int nlo = 1 << a.hi_shift;
-    float2* s_lo = smem;                       // FULL: the whole W_R table
+    // the tables first, so that their offsets fit the LDS instructions' immediate field
+    float2* s_lo = smem;  // FULL: the whole W_R table
     float2* s_hi = s_lo + nlo;
+    float* t_re = reinterpret_cast<float*>(FULL ? s_lo + R : s_hi + a.n_hi);  // [64U][65]
+    float* t_im = t_re + 64 * U * kTileStride;                                 // [64U][65]
+    int* s_cs = reinterpret_cast<int*>(t_im + 64 * U * kTileStride);         // colstart[c0 .. c0+64]
     if (FULL) {
-        for (int i = threadIdx.x; i < (int)R; i += kThreads) s_lo[i] = a.wr_full[i];
+        for (int i = threadIdx.x; i < (int)R; i += kColThreads) s_lo[i] = a.wr_full[i];
     } else {
-        for (int i = threadIdx.x; i < nlo; i += kThreads) s_lo[i] = a.wr_lo[i];
-        for (int i = threadIdx.x; i < a.n_hi; i += kThreads) s_hi[i] = a.wr_hi[i];
+        for (int i = threadIdx.x; i < nlo; i += kColThreads) s_lo[i] = a.wr_lo[i];
+        for (int i = threadIdx.x; i < a.n_hi; i += kColThreads) s_hi[i] = a.wr_hi[i];
     }
+    for (int i = threadIdx.x; i <= 64; i += kColThreads) s_cs[i] = a.colstart[min(c0 + i, a.C)];
     __syncthreads();
 
     const uint32_t mask = (uint32_t)nlo - 1u;
     const int hs = a.hi_shift;
+    auto wr = [&](uint32_t ph) -> float2 {
+        return FULL ? s_lo[ph] : cmul(s_hi[ph >> hs], s_lo[ph & mask]);
+    };
     const uint32_t u0 = (uint32_t)(a.u_base + u0l);
-    for (int i = 0; i < 16; i++) {
-        const int c = c0 + wid + 4 * i;
-        if (c >= a.C) break;
+    for (int i = 0; i < 64 / kWaves; i++) {
+        const int cc = wid + kWaves * i;  // column within the tile (uniform)
+        const int c = c0 + cc;
         float2 acc[U];
 #pragma unroll
         for (int t = 0; t < U; t++) acc[t] = make_float2(0.f, 0.f);
-        const int beg = a.colstart[c], end = a.colstart[c + 1];
-        for (int p = beg; p < end; p++) {
-            uint32_t r = PACKED ? (uint32_t)((const uint16_t*)a.row_idx)[p]
-                                : (uint32_t)((const int*)a.row_idx)[p];
-            if (r >= R) r = fastmod_u32(r, a.r_magic, R);
-            const uint32_t base = PACKED ? fastmod_u32(u0 * r, a.r_magic, R)
-                                         : (uint32_t)(((uint64_t)u0 * r) % (uint64_t)R);
-            const uint32_t d64 = fastmod_u32(r << 6, a.r_magic, R);
-            // (u0 + lane) * r mod R = (base + lane * r) mod R with base + lane * r < 64 R
-            const uint32_t x = base + __umul24((uint32_t)lane, r);
-            const uint32_t q = (uint32_t)((float)x * a.inv_r);
-            int ph = (int)(x - __umul24(q, R));
-            if (ph < 0) ph += (int)R;
-            else if (ph >= (int)R) ph -= (int)R;
+        if (c < a.C) {
+            const int beg = s_cs[cc], end = s_cs[cc + 1];
+            for (int p0 = beg; p0 < end; p0 += 64) {
+                const int n = min(64, end - p0);
+                // per-nonzero constants, one nonzero per lane
+                uint32_t r = 0;
+                if (lane < n)
+                    r = PACKED ? (uint32_t)((const uint16_t*)a.row_idx)[p0 + lane]
+                               : (uint32_t)((const int*)a.row_idx)[p0 + lane];
+                const uint32_t base = PACKED ? fastmod_u32(u0 * r, a.r_magic, R)
+                                             : (uint32_t)(((uint64_t)u0 * r) % (uint64_t)R);
+                const uint32_t d64 = fastmod_u32(r << 6, a.r_magic, R);
+                float2 w64 = make_float2(1.f, 0.f);
+                if (U > 1) w64 = wr(d64);
+#pragma unroll 2
+                for (int j = 0; j < n; j++) {
+                    const uint32_t rj = (uint32_t)__builtin_amdgcn_readlane((int)r, j);
+                    const uint32_t bj = (uint32_t)__builtin_amdgcn_readlane((int)base, j);
+                    // (u0 + lane) r mod R = (bj + lane rj) mod R, bj + lane rj < 64 R
+                    const uint32_t x = bj + __umul24((uint32_t)lane, rj);
+                    const uint32_t q = (uint32_t)((float)x * a.inv_r);
+                    const uint32_t y = x - __umul24(q, R);  // in [0, 2R)
+                    const uint32_t ph = min(y, y - R);      // y - R wraps when y < R
+                    float2 w = wr(ph);
+                    acc[0].x += w.x;
+                    acc[0].y += w.y;
+                    if (U > 1) {
+                        const float2 s = make_float2(__builtin_bit_cast(float, __builtin_amdgcn_readlane(
+                                                         __builtin_bit_cast(int, w64.x), j)),
+                                                     __builtin_bit_cast(float, __builtin_amdgcn_readlane(
+                                                         __builtin_bit_cast(int, w64.y), j)));
 #pragma unroll
-            for (int t = 0; t < U; t++) {
-                const float2 w = FULL ? s_lo[ph] : cmul(s_hi[(uint32_t)ph >> hs], s_lo[(uint32_t)ph & mask]);
-                acc[t].x += w.x;
-                acc[t].y += w.y;
-                ph += (int)d64;
-                if (ph >= (int)R) ph -= (int)R;
+                        for (int t = 1; t < U; t++) {
+                            w = cmul(w, s);
+                            acc[t].x += w.x;
+                            acc[t].y += w.y;
+                        }
+                    }
+                }
             }
+            const float2 hc = a.h[c];
+#pragma unroll
+            for (int t = 0; t < U; t++) acc[t] = cmul(acc[t], hc);
         }
-        const float2 hc = a.h[c];
 #pragma unroll
         for (int t = 0; t < U; t++) {
-            const int ul = u0l + lane + 64 * t;
-            if (ul < a.tile_rows) a.X0[(size_t)ul * a.ldx + c] = cmul(acc[t], hc);
+            t_re[(t * 64 + lane) * kTileStride + cc] = acc[t].x;
+            t_im[(t * 64 + lane) * kTileStride + cc] = acc[t].y;
         }
+    }
+    __syncthreads();
+    // whole 64-column row segments of X0 (ldx is a multiple of 64)
+    for (int e = threadIdx.x; e < 64 * 64 * U; e += kColThreads) {
+        const int row = e >> 6, col = e & 63;
+        const int ul = u0l + row;
+        if (ul < a.tile_rows)
+            a.X0[(size_t)ul * a.ldx + c0 + col] =
+                make_float2(t_re[row * kTileStride + col], t_im[row * kTileStride + col]);
     }
 }
 
@@ -116,8 +169,34 @@ bool rowpass_supports_m1(int m1) {
 }
 
 size_t colpass_lds_bytes(const RowPassArgs& a) {
-    if (a.wr_full) return (size_t)a.R * sizeof(float2);
-    return ((size_t)(1 << a.hi_shift) + a.n_hi) * sizeof(float2);
+    const int U = a.colpass_rows > 0 ? a.colpass_rows : 1;
+    const size_t tile = (size_t)2 * 64 * U * kTileStride * sizeof(float) + 66 * sizeof(int);
+    if (a.wr_full) return tile + (size_t)a.R * sizeof(float2);
+    return tile + ((size_t)(1 << a.hi_shift) + a.n_hi) * sizeof(float2);
+}
+
+hipError_t colpass_prepare(size_t lds_limit) {
+    // raise the dynamic LDS limit of every k_colpass instantiation (the tile is
+    // 33 KB per 64 rows; with U = 2 or 4 it passes 64 KB)
+    const void* ks[] = {
+        reinterpret_cast<const void*>(&k_colpass<true, true, 1>),
+        reinterpret_cast<const void*>(&k_colpass<true, true, 2>),
+        reinterpret_cast<const void*>(&k_colpass<true, true, 4>),
+        reinterpret_cast<const void*>(&k_colpass<true, false, 1>),
+        reinterpret_cast<const void*>(&k_colpass<true, false, 2>),
+        reinterpret_cast<const void*>(&k_colpass<true, false, 4>),
+        reinterpret_cast<const void*>(&k_colpass<false, true, 1>),
+        reinterpret_cast<const void*>(&k_colpass<false, true, 2>),
+        reinterpret_cast<const void*>(&k_colpass<false, true, 4>),
+        reinterpret_cast<const void*>(&k_colpass<false, false, 1>),
+        reinterpret_cast<const void*>(&k_colpass<false, false, 2>),
+        reinterpret_cast<const void*>(&k_colpass<false, false, 4>),
+    };
+    for (const void* k : ks) {
+        const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 template <bool PACKED, bool FULL>
@@ -126,9 +205,9 @@ static void launch_colpass_u(const RowPassArgs& a, hipStream_t s) {
     const dim3 gp((a.C + 63) / 64, (a.tile_rows + 64 * U - 1) / (64 * U));
     const size_t lds = colpass_lds_bytes(a);
     switch (U) {
-        case 1: hipLaunchKernelGGL((k_colpass<PACKED, FULL, 1>), gp, dim3(kThreads), lds, s, a); break;
-        case 2: hipLaunchKernelGGL((k_colpass<PACKED, FULL, 2>), gp, dim3(kThreads), lds, s, a); break;
-        default: hipLaunchKernelGGL((k_colpass<PACKED, FULL, 4>), gp, dim3(kThreads), lds, s, a); break;
+        case 1: hipLaunchKernelGGL((k_colpass<PACKED, FULL, 1>), gp, dim3(kColThreads), lds, s, a); break;
+        case 2: hipLaunchKernelGGL((k_colpass<PACKED, FULL, 2>), gp, dim3(kColThreads), lds, s, a); break;
+        default: hipLaunchKernelGGL((k_colpass<PACKED, FULL, 4>), gp, dim3(kColThreads), lds, s, a); break;
     }
 }
 
@@ -165,6 +244,14 @@ hipError_t launch_rowpass_tile(const RowPassArgs& a, bool packed, hipStream_t s,
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev[1], s)) != hipSuccess) return e;
+    if (a.fused) {
+        // one kernel for the whole row pass; the fft1 and ifft1 stages time as empty
+        if (ev && (e = hipEventRecord(ev[2], s)) != hipSuccess) return e;
+        if ((e = launch_fused(a, s, false)) != hipSuccess) return e;
+        if (ev && (e = hipEventRecord(ev[3], s)) != hipSuccess) return e;
+        if (ev && (e = hipEventRecord(ev[4], s)) != hipSuccess) return e;
+        return hipSuccess;
+    }
     const dim3 g1((a.M2 + 63) / 64, (a.tile_rows + 3) / 4);
     if ((e = launch_fft1(a, g1, s, false)) != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev[2], s)) != hipSuccess) return e;

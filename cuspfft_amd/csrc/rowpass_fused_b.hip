@@ -1,0 +1,29 @@
+// rowpass_fused_b.hip -- k_rowfused instantiations of group B (see fused_impl.h).
+#include "fused_impl.h"
+
+namespace spfft {
+
+hipError_t launch_fused_group_b(const RowPassArgs& a, hipStream_t s, bool prepare_only) {
+    SPFFT_FUSED_GROUP_B(SPFFT_FUSED_LAUNCH_CASE)
+    return hipErrorInvalidValue;
+}
+
+bool fused_shape(int i, FusedShape* out) {
+    static const FusedShape kShapes[] = {
+#define SPFFT_SHAPE(m1, nb, rpw, nt, r1, r2, r3) \
+    {m1, nb, rpw, nt, {r1, r2, r3}, fused_lds_bytes_of(m1, nb, rpw, r1 * r2 * r3)},
+        SPFFT_FUSED_LIST(SPFFT_SHAPE)
+#undef SPFFT_SHAPE
+    };
+    if (i < 0 || i >= (int)(sizeof(kShapes) / sizeof(kShapes[0]))) return false;
+    *out = kShapes[i];
+    return true;
+}
+
+hipError_t launch_fused(const RowPassArgs& a, hipStream_t s, bool prepare_only) {
+    hipError_t e = launch_fused_group_a(a, s, prepare_only);
+    if (e == hipErrorInvalidValue) e = launch_fused_group_b(a, s, prepare_only);
+    return e;
+}
+
+}  // namespace spfft

@@ -37,9 +37,20 @@ bool rowpass_supports_m1(int m1);        // k_fft1 / k_ifft1 instantiated for th
 bool mfma_supports_m1(int m1);           // precision-form (BF16x3 MFMA) sub-FFT length
 bool conv_fast_supported(int r1, int r2); // k_conv_fast<r1, r2> instantiated (r2 = 1: one stage)
 int conv_max_butterflies(int r);         // butterflies per thread per k_conv_fast stage
-size_t colpass_lds_bytes(const RowPassArgs& a);
+size_t colpass_lds_bytes(const RowPassArgs& a);   // needs a.colpass_rows and a.wr_full set
+hipError_t colpass_prepare(size_t lds_limit);      // once per process, before launches
 size_t conv_lds_bytes(const RowPassArgs& a);
-// ev: NULL, or 5 events recorded before k_colpass and after each of the 4 kernels.
+// Fused row kernel (k_rowfused) shapes: M1, LDS batches, rows per workgroup,
+// threads, stage radices r[3] (1 = no stage).  Returns false past the last one.
+struct FusedShape {
+    int m1, nb, rpw, nt, r[3];
+    size_t lds_bytes;
+};
+bool fused_shape(int i, FusedShape* out);
+// prepare_only: raise the kernel's dynamic LDS limit (once per process, before launches)
+hipError_t launch_fused(const RowPassArgs& a, hipStream_t s, bool prepare_only);
+// ev: NULL, or 5 events recorded before k_colpass and after each of the 4 kernels
+// (with the fused row kernel: after k_colpass, twice, after k_rowfused, twice).
 hipError_t launch_rowpass_tile(const RowPassArgs& a, bool packed, hipStream_t s, hipEvent_t* ev);
 
 }  // namespace spfft

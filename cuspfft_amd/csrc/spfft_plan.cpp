@@ -144,6 +144,7 @@ void fft64(std::vector<cd>& x, int sign) {
 }
 
 size_t g_lds_limit = 0;
+constexpr size_t kGfx950Lds = 163840;
 std::mutex g_lds_mutex;
 
 spfft_status ensure_lds_limit(size_t& budget) {
@@ -336,17 +337,18 @@ spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t
     RowPassArgs a = p.base;
     a.P = (int)pitch;
     if (a.colpass_rows == 0) {
-        // more rows per lane amortise the per-nonzero work of long columns; short
-        // columns (sparse power-law graphs) balance better with one row per lane
-        const double avg = (double)p.last_nnz / std::max(p.C, 1);
-        a.colpass_rows = avg >= 16.0 ? 4 : (avg >= 8.0 ? 2 : 1);
+        // two rows per lane: the second row costs one complex product per nonzero
+        // instead of a phase reduction and a table lookup
+        a.colpass_rows = 2;
     }
+    // the whole W_R table only while it fits in LDS next to the row tile
+    if (a.wr_full && spfft::colpass_lds_bytes(a) > g_lds_limit) a.wr_full = nullptr;
     a.colstart = p.colstart.as<int>();
     a.row_idx = p.row_idx.p;
     a.out = static_cast<float2*>(d_out);
     if (T) a.T = T;
     if (X0) a.X0 = X0;
-    if (!a.T || !a.X0) return fail(SPFFT_ERR_INVALID_ARG, "plan has no row-pass workspace");
+    if ((!a.T && !a.fused) || !a.X0) return fail(SPFFT_ERR_INVALID_ARG, "plan has no row-pass workspace");
     // Rows u in [0, R/2] are computed; each also writes its mirror row R - u
     // (X[R-u][v] = conj(X[u][(C - v) mod C]) for a real input).
     const int tile = p.opts.u_tile;
@@ -391,6 +393,16 @@ int spfft_next_7_smooth(int n) {
     return next_smooth(n, p, 4);
 }
 
+int spfft_fused_lengths(int* out, int cap) {
+    std::vector<int> ms;
+    spfft::FusedShape f;
+    for (int i = 0; spfft::fused_shape(i, &f); i++) ms.push_back(f.m1 * f.r[0] * f.r[1] * f.r[2]);
+    std::sort(ms.begin(), ms.end());
+    ms.erase(std::unique(ms.begin(), ms.end()), ms.end());
+    for (int i = 0; out && i < (int)ms.size() && i < cap; i++) out[i] = ms[i];
+    return (int)ms.size();
+}
+
 int spfft_factor_3_smooth(int n, int* out, int cap) {
     int c3 = 0, k = 0;
     if (n < 1) return -1;
@@ -407,7 +419,20 @@ namespace {
 
 // Shape planning only (no device allocation): M, the M1 x M2 split, the
 // convolution kernel and its rows per workgroup.
-spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t budget, spfft_plan_s* p) {
+// Fused row-kernel shape for M, or -1.  lds_limit: the device's LDS per workgroup.
+int find_fused(int M, int m1_req, size_t lds_limit, spfft::FusedShape* out) {
+    spfft::FusedShape f;
+    for (int i = 0; spfft::fused_shape(i, &f); i++) {
+        if (f.m1 * f.r[0] * f.r[1] * f.r[2] != M || (m1_req && f.m1 != m1_req) || f.lds_bytes > lds_limit)
+            continue;
+        *out = f;
+        return i;
+    }
+    return -1;
+}
+
+spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t budget, size_t lds_limit,
+                        spfft_plan_s* p) {
     spfft_opts o;
     if (opts_in) o = *opts_in; else spfft_opts_default(&o);
     if (rows <= 0 || cols <= 0) return fail(SPFFT_ERR_INVALID_ARG, "rows and cols must be positive");
@@ -462,9 +487,48 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     // one tile is at most 65535 workgroup rows of the length-M1 kernels' grid
     p->opts.u_tile = std::min(o.u_tile, 65535);
     p->packed = rows <= 65536 && cols <= 65536;
-    spfft_status st = choose_split(*p, o.m1, budget, mfma);
-    if (st != SPFFT_OK) return st;
+    if (o.row_kernel < SPFFT_ROW_AUTO || o.row_kernel > SPFFT_ROW_FUSED)
+        return fail(SPFFT_ERR_INVALID_ARG, "row_kernel must be 0, 1 or 2");
+    spfft::FusedShape fs{};
+    const bool fused_ok = !mfma && o.conv_rows == 0 && o.row_kernel != SPFFT_ROW_FOUR_STEP &&
+                          find_fused(M, o.m1, lds_limit, &fs) >= 0;
+    if (o.row_kernel == SPFFT_ROW_FUSED && !fused_ok)
+        return fail(SPFFT_ERR_UNSUPPORTED, "no fused row-kernel shape for M = " + std::to_string(M) +
+                                               (mfma ? " in the BF16x3 form" : ""));
     RowPassArgs& a = p->base;
+    spfft_status st = SPFFT_OK;
+    if (fused_ok) {
+        p->M1 = fs.m1;
+        p->M2 = fs.r[0] * fs.r[1] * fs.r[2];
+        a.fused = 1;
+        a.fused_nb = fs.nb;
+        a.fused_rpw = fs.rpw;
+        a.fused_nt = fs.nt;
+        a.conv_kind = 1;
+        a.f2.L = p->M2;
+        a.f2.nstages = 0;
+        for (int i = 0, ns = 1; i < 3; i++) {
+            if (fs.r[i] == 1) continue;
+            a.f2.radix[a.f2.nstages] = fs.r[i];
+            a.f2.ns_before[a.f2.nstages++] = ns;
+            ns *= fs.r[i];
+        }
+    } else {
+        a.fused = 0;
+        st = choose_split(*p, o.m1, budget, mfma);
+    }
+    if (st != SPFFT_OK && o.fft_len == 0 && o.m1 == 0) {
+        // a rule-derived M without a supported split (e.g. 5^7): the next smooth length
+        auto next = [&](int n) {
+            return o.length_rule == SPFFT_SMOOTH_3 ? spfft_next_3_smooth(n) : spfft_next_7_smooth(n);
+        };
+        for (int tries = 0; st != SPFFT_OK && tries < 64 && M < (1 << 24); tries++) {
+            M = next(M + 1);
+            p->M = M;
+            st = choose_split(*p, 0, budget, mfma);
+        }
+    }
+    if (st != SPFFT_OK) return st;
     a.R = rows;
     a.C = cols;
     a.V = p->V;
@@ -478,7 +542,8 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     a.hi_shift = hs;
     a.n_hi = (rows + (1 << hs) - 1) >> hs;
     a.r_magic = ~0ULL / (uint64_t)rows + 1;
-    a.inv_r = 1.0f / (float)rows;
+    // just below 1/R: floor(x * inv_r) is floor(x / R) or one less for x < 2^28
+    a.inv_r = (float)((1.0 / (double)rows) * (1.0 - 0x1p-20));
     a.precision = mfma ? 1 : 0;
     if (o.colpass_rows < 0 || o.colpass_rows == 3 || o.colpass_rows > 4)
         return fail(SPFFT_ERR_INVALID_ARG, "colpass_rows must be 0, 1, 2 or 4");
@@ -488,7 +553,11 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     const bool full_fits = (size_t)rows * 8 <= budget;
     if (o.colpass_table == 2 && !full_fits)
         return fail(SPFFT_ERR_UNSUPPORTED, "W_R table does not fit in LDS");
-    p->colpass_full = o.colpass_table == 2 || (o.colpass_table == 0 && rows <= 4096);
+    p->colpass_full = o.colpass_table == 2 || (o.colpass_table == 0 && full_fits);
+    if (a.fused) {
+        a.G2 = 0;
+        return SPFFT_OK;
+    }
     return choose_tiling(*p, budget);
 }
 
@@ -510,6 +579,7 @@ void fill_info(const spfft_plan_s* p, spfft_plan_info* info) {
     info->colpass_rows = p->base.colpass_rows;
     info->colpass_full_table = p->colpass_full ? 1 : 0;
     info->workspace_bytes = p->table_bytes;
+    info->row_kernel = p->base.fused ? SPFFT_ROW_FUSED : SPFFT_ROW_FOUR_STEP;
 }
 
 }  // namespace
@@ -519,7 +589,8 @@ extern "C" {
 spfft_status spfft_plan_describe(int rows, int cols, const spfft_opts* opts, spfft_plan_info* info) {
     if (!info) return fail(SPFFT_ERR_INVALID_ARG, "NULL info");
     spfft_plan_s p;
-    spfft_status st = plan_shape(rows, cols, opts, 65536, &p);
+    // the planner's view of a gfx950 device: 160 KiB of LDS per workgroup
+    spfft_status st = plan_shape(rows, cols, opts, 65536, kGfx950Lds, &p);
     if (st != SPFFT_OK) return st;
     fill_info(&p, info);
     return SPFFT_OK;
@@ -537,7 +608,7 @@ namespace spfft_internal {
 
 void workspace_bytes(const spfft_plan_s& p, size_t* t_bytes, size_t* x0_bytes) {
     const size_t tile_rows = (size_t)std::min(p.opts.u_tile, p.R / 2 + 1);
-    *t_bytes = tile_rows * (size_t)p.M * sizeof(float2);
+    *t_bytes = p.base.fused ? 0 : tile_rows * (size_t)p.M * sizeof(float2);
     *x0_bytes = tile_rows * (size_t)p.base.ldx * sizeof(float2);
 }
 
@@ -550,7 +621,13 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
     size_t budget = 0;
     spfft_status st = ensure_lds_limit(budget);
     if (st != SPFFT_OK) return st;
-    st = plan_shape(rows, cols, opts_in, budget, p);
+    {
+        static std::once_flag once;
+        static hipError_t prep = hipSuccess;
+        std::call_once(once, [] { prep = spfft::colpass_prepare(g_lds_limit); });
+        SPFFT_HIP(prep, "column pass LDS limit");
+    }
+    st = plan_shape(rows, cols, opts_in, budget, g_lds_limit, p);
     if (st != SPFFT_OK) return st;
     RowPassArgs& a = p->base;
     const int M = p->M;
@@ -614,7 +691,8 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
     }
     const int tile_rows = std::min(o.u_tile, rows / 2 + 1);
     if (own_workspace) {
-        SPFFT_HIP(p->T.alloc((size_t)tile_rows * M * sizeof(float2)), "hipMalloc(tile buffer)");
+        if (!a.fused)
+            SPFFT_HIP(p->T.alloc((size_t)tile_rows * M * sizeof(float2)), "hipMalloc(tile buffer)");
         SPFFT_HIP(p->X0.alloc((size_t)tile_rows * a.ldx * sizeof(float2)), "hipMalloc(pass-1 buffer)");
     }
     SPFFT_HIP(p->counts.alloc((size_t)(cols + 1) * 4), "hipMalloc(counts)");
@@ -633,6 +711,7 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
     a.mfma_dft = a.precision == 1 ? p->mfma_dft.as<float>() : nullptr;
     a.T = p->T.as<float2>();
     a.X0 = p->X0.as<float2>();
+    if (a.fused) SPFFT_HIP(spfft::launch_fused(a, nullptr, true), "fused row kernel LDS limit");
     p->table_bytes = p->h.bytes + p->wr_lo.bytes + p->wr_hi.bytes + p->wr_full.bytes + p->tw2.bytes + p->twM.bytes +
                      p->bhat.bytes + p->T.bytes + p->X0.bytes;
     *out = guard.release();

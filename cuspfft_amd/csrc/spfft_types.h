@@ -36,7 +36,7 @@ struct RowPassArgs {
     int colpass_rows;          // k_colpass: rows per lane (1, 2 or 4; 64 * rows per workgroup)
     int hi_shift;              // two-level W_R table: W_R^p = hi[p >> s] * lo[p & (2^s - 1)]
     int n_hi;                  // entries in hi table
-    float inv_r;               // 1/R (phase reduction estimate)
+    float inv_r;               // (1 - 2^-20) / R: a phase quotient estimate that never exceeds
     uint64_t r_magic;          // fastmod magic for R
     FftStages f2;              // radix stages of the length-M2 FFT in k_conv
     const int* colstart;       // C+1 dense column starts into row_idx
@@ -50,6 +50,10 @@ struct RowPassArgs {
     const float2* bhat;        // [k1*M2 + k2] = B^[k1 + M1 k2] / M
     const float* mfma_dft;     // precision form: cos/sin(2 pi i / M1) hi and lo BF16 parts, [4][M1]
     int precision;             // 0: FP32 codelets, 1: BF16x3 MFMA length-M1 sub-FFTs
+    int fused;                 // 1: k_rowfused<M1, fused_nb, fused_rpw, f2.radix...> replaces
+                               //    k_fft1 + k_conv + k_ifft1 (no tile buffer T)
+    int fused_nb, fused_rpw;   // k_rowfused: LDS batches per row, rows per workgroup
+    int fused_nt;              // k_rowfused: threads per workgroup
     float2* X0;                // tile pass-1 output [u_local][ldx]
     float2* T;                 // tile workspace [u_local][k1][n2]
     float2* out;               // output base (row 0), pitch P

@@ -13,7 +13,8 @@ Features
           fraction of active columns and the 8-bin log2 column-degree histogram
   plan:   log2 M, M / (2 cols - 1), the exponents of 2, 3, 5, 7 in M, log2 M1,
           log2 M2, fast convolution kernel, convolution stages, arithmetic form,
-          column-pass rows per lane (one-hot 1 / 2 / 4, auto = all zero)
+          column-pass rows per lane (one-hot 1 / 2 / 4, auto = all zero), fused
+          row kernel
 """
 from __future__ import annotations
 
@@ -25,7 +26,7 @@ import numpy as np
 
 from . import _check, _vp, describe_plan, load
 
-N_FEATURES = 27
+N_FEATURES = 28
 
 
 def _exponents(M: int):
@@ -42,8 +43,11 @@ def _exponents(M: int):
 def features(rec: dict, opts: dict, info: dict | None = None) -> np.ndarray:
     """Feature row of matrix `rec` (autotune.features_of) with plan `opts`."""
     if info is None:
-        info = describe_plan(rec["rows"], rec["cols"], **{k: v for k, v in opts.items()
-                                                           if k != "u_tile"})
+        # timings recorded before the fused row kernel existed carry no row_kernel:
+        # they ran the four-step kernels
+        o = {k: v for k, v in opts.items() if k != "u_tile"}
+        o.setdefault("row_kernel", 1)
+        info = describe_plan(rec["rows"], rec["cols"], **o)
     rows, cols, nnz = rec["rows"], rec["cols"], rec["nnz"]
     act = max(rec.get("active_cols", cols), 1)
     M, m1, m2 = info["fft_len"], info["m1"], info["m2"]
@@ -55,6 +59,7 @@ def features(rec: dict, opts: dict, info: dict | None = None) -> np.ndarray:
     f += [math.log2(m1), math.log2(max(m2, 1)), float(info["conv_fast"]), float(info["conv_stages"]),
           float(info["precision"])]
     f += [1.0 if cpr == 1 else 0.0, 1.0 if cpr == 2 else 0.0, 1.0 if cpr == 4 else 0.0]
+    f += [1.0 if info.get("row_kernel") == 2 else 0.0]
     assert len(f) == N_FEATURES
     return np.asarray(f, np.float32)
 

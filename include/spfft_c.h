@@ -55,7 +55,14 @@ typedef struct spfft_opts {
     int colpass_table; /* 0: automatic; 1: two-level W_R table; 2: whole W_R table in LDS      */
     int row_limit;     /* 0: all rows.  Else only output rows u < row_limit (and their mirror
                           rows) are computed: a sampled run for timing (the autotuner) */
+    int row_kernel;    /* SPFFT_ROW_*: how the Bluestein row pass runs                       */
 } spfft_opts;
+
+/* Row-pass kernels.  FOUR_STEP: k_fft1 + k_conv + k_ifft1 with the tile buffer T
+ * in HBM (any supported M).  FUSED: one kernel per row with the length-M row
+ * held in registers and LDS (no T; the M of a fixed list of shapes, FP32 form).
+ * AUTO: FUSED when M has a fused shape and the options allow it, else FOUR_STEP. */
+enum { SPFFT_ROW_AUTO = 0, SPFFT_ROW_FOUR_STEP = 1, SPFFT_ROW_FUSED = 2 };
 
 typedef struct spfft_plan_info {
     int rows, cols;
@@ -73,6 +80,7 @@ typedef struct spfft_plan_info {
     int colpass_full_table; /* 1: whole W_R table in LDS, 0: two-level table           */
     int precision;          /* SPFFT_PREC_* of the length-m1 sub-FFTs                   */
     size_t workspace_bytes; /* device bytes held by the plan (tables + tile buffer) */
+    int row_kernel;         /* SPFFT_ROW_FOUR_STEP or SPFFT_ROW_FUSED                   */
 } spfft_plan_info;
 
 typedef struct spfft_plan_s* spfft_plan;
@@ -221,6 +229,10 @@ int spfft_next_7_smooth(int n);
 /* Writes the factor_3_smooth radix order into out (cap entries); returns the
  * count, or -1 if n is not 3-smooth. */
 int spfft_factor_3_smooth(int n, int* out, int cap);
+/* Bluestein lengths M that have a fused row-kernel shape (SPFFT_ROW_FUSED), in
+ * increasing order: writes up to cap of them to out and returns how many exist.
+ * No counterpart in the reference (its row pass has no fused kernel). */
+int spfft_fused_lengths(int* out, int cap);
 
 /* Seeded synthetic binary matrices (host memory, canonical COO: deduplicated,
  * sorted by (row, col), zero diagonal, symmetric).  Release with spfft_coo_free. */

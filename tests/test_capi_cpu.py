@@ -46,7 +46,10 @@ def test_smooth_lengths_match_oracle(oracle_mod):
 
 def test_planner_covers_many_shapes():
     import cuspfft_amd as sp
-    for cols in list(range(1, 200)) + list(range(200, 70000, 997)):
+    # (p^k + 1) / 2 makes 2*cols - 1 a pure prime power (e.g. 5^7 = 78125), whose
+    # smooth length may have no supported split: the planner moves to the next one
+    powers = [(p ** k + 1) // 2 for p, k in ((3, 9), (3, 10), (5, 5), (5, 6), (5, 7), (7, 5), (7, 6))]
+    for cols in list(range(1, 200)) + list(range(200, 70000, 997)) + powers:
         for rule in (3, 7):
             info = sp.describe_plan(17, cols, length_rule=rule)
             M = info["fft_len"]
@@ -64,6 +67,34 @@ def test_planner_config_choices():
         b = sp.describe_plan(cols, cols, length_rule=7)
         assert a["fft_len"] == m3 and b["fft_len"] == m7
         assert a["conv_fast"] == 1 and b["conv_fast"] == 1
+
+
+def test_planner_fused_row_kernel():
+    """The planner picks the fused row kernel for the lengths it has a shape for
+    (the configurations' 3- and 7-smooth lengths), the four-step kernels
+    otherwise or when asked, and rejects impossible requests."""
+    import cuspfft_amd as sp
+    for cols, M in [(2310, 5184), (2310, 4704), (8219, 17496), (8219, 16464), (16706, 34992),
+                    (16706, 33600)]:
+        rule = 3 if sp.next_3_smooth(2 * cols - 1) == M else 7
+        info = sp.describe_plan(cols, cols, length_rule=rule)
+        assert info["fft_len"] == M and info["row_kernel"] == sp.ROW_FUSED
+        assert info["m1"] * info["m2"] == M and int(np.prod(info["conv_radix"])) == info["m2"]
+        four = sp.describe_plan(cols, cols, length_rule=rule, row_kernel=sp.ROW_FOUR_STEP)
+        assert four["row_kernel"] == sp.ROW_FOUR_STEP and four["fft_len"] == M
+    # no shape for this length, the BF16x3 form, explicit convolution rows -> four-step
+    assert sp.describe_plan(100, 1000, fft_len=2048)["row_kernel"] == sp.ROW_FOUR_STEP
+    assert sp.describe_plan(2310, 2310, precision=sp.PREC_BF16X3)["row_kernel"] == sp.ROW_FOUR_STEP
+    assert sp.describe_plan(2310, 2310, conv_rows=2)["row_kernel"] == sp.ROW_FOUR_STEP
+    with pytest.raises(sp.SpfftError) as ei:
+        sp.describe_plan(100, 1000, fft_len=2048, row_kernel=sp.ROW_FUSED)
+    assert ei.value.status == 4
+    with pytest.raises(sp.SpfftError) as ei:
+        sp.describe_plan(100, 2310, row_kernel=sp.ROW_FUSED, precision=sp.PREC_BF16X3)
+    assert ei.value.status == 4
+    with pytest.raises(sp.SpfftError) as ei:
+        sp.describe_plan(100, 100, row_kernel=5)
+    assert ei.value.status == 1
 
 
 def test_planner_errors():

@@ -108,12 +108,17 @@ def test_csc_bit_exact(gpu, oracle_mod, case):
         _csc_case(sp, oracle_mod, rows, cols, r, c)
 
 
+ROW_KERNELS = pytest.mark.parametrize("row_kernel", [0, 1], ids=["auto", "four_step"])
+
+
+@ROW_KERNELS
 @pytest.mark.parametrize("rule", [3, 7])
-def test_c1_full_vs_fp64(gpu, oracle_mod, rule):
+def test_c1_full_vs_fp64(gpu, oracle_mod, rule, row_kernel):
     sp = gpu
     m = sp.config_matrix("c1")
-    print(f"[fft] c1 rule={rule}", flush=True)
-    plan, X = _run(sp, m.rows, m.cols, m.row_idx, m.col_idx, length_rule=rule)
+    print(f"[fft] c1 rule={rule} row_kernel={row_kernel}", flush=True)
+    plan, X = _run(sp, m.rows, m.cols, m.row_idx, m.col_idx, length_rule=rule, row_kernel=row_kernel)
+    assert plan.info["row_kernel"] == (row_kernel or 2)
     ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, nthreads=8)
     err = oracle_mod.rel_l2(X, ref)
     print(f"c1 rule={rule} M={plan.info['fft_len']} rel_l2={err:.3e}")
@@ -176,13 +181,15 @@ def _invariants(X_dev, m):
     assert sym < 1e-5, sym
 
 
+@ROW_KERNELS
 @pytest.mark.parametrize("cfg,rule", [("c2", 3), ("c2", 7), ("c3", 3), ("c3", 7)])
-def test_full_config_spot_rows_and_invariants(gpu, oracle_mod, cfg, rule):
+def test_full_config_spot_rows_and_invariants(gpu, oracle_mod, cfg, rule, row_kernel):
     import torch
     sp = gpu
     m = sp.config_matrix(cfg)
-    print(f"[fft] {cfg} rule={rule}", flush=True)
-    plan = sp.Plan(m.rows, m.cols, length_rule=rule, u_tile=256)
+    print(f"[fft] {cfg} rule={rule} row_kernel={row_kernel}", flush=True)
+    plan = sp.Plan(m.rows, m.cols, length_rule=rule, u_tile=256, row_kernel=row_kernel)
+    assert plan.info["row_kernel"] == (row_kernel or 2)
     out = plan.alloc_output()
     plan.execute(_dev(m.row_idx), _dev(m.col_idx), out)
     torch.cuda.synchronize()
@@ -197,16 +204,21 @@ def test_full_config_spot_rows_and_invariants(gpu, oracle_mod, cfg, rule):
     _invariants(out, m)
 
 
-def test_tile_invariance(gpu):
-    import torch
+def test_tile_invariance(gpu, oracle_mod):
+    """Any tiling computes the same transform: identical for repeated runs of one
+    plan, and within FP32 rounding across tile sizes (the column pass derives the
+    twiddle of a lane's second row by one complex product from its first, and the
+    rows that are first or second in a tile depend on the tiling)."""
     sp = gpu
     m = sp.config_matrix("c1")
     outs = []
     for tile in (1, 100, 2310, 5000):
-        _, X = _run(sp, m.rows, m.cols, m.row_idx, m.col_idx, u_tile=tile)
+        plan, X = _run(sp, m.rows, m.cols, m.row_idx, m.col_idx, u_tile=tile)
         outs.append(X)
+        _, X2 = _run(sp, m.rows, m.cols, m.row_idx, m.col_idx, u_tile=tile)
+        np.testing.assert_array_equal(X, X2)
     for X in outs[1:]:
-        np.testing.assert_array_equal(X, outs[0])
+        assert oracle_mod.rel_l2(X, outs[0]) <= 1e-6
 
 
 def test_errors(gpu):
@@ -294,7 +306,7 @@ def test_m1_splits(gpu, oracle_mod, cols, fft_len, m1):
     rng = np.random.default_rng(cols + fft_len + m1)
     r, c = _random_coo(rng, rows, cols, 3000)
     print(f"[split] cols={cols} M={fft_len} m1={m1}", flush=True)
-    plan, X = _run(sp, rows, cols, r, c, fft_len=fft_len, m1=m1, u_tile=50)
+    plan, X = _run(sp, rows, cols, r, c, fft_len=fft_len, m1=m1, u_tile=50, row_kernel=1)
     assert plan.info["fft_len"] == fft_len
     if m1:
         assert plan.info["m1"] == m1
@@ -302,6 +314,33 @@ def test_m1_splits(gpu, oracle_mod, cols, fft_len, m1):
     err = oracle_mod.rel_l2(X, ref)
     print(f"  plan {plan.info} rel_l2={err:.3e}")
     assert err <= REL_L2_PLAIN
+
+
+# Bluestein lengths with a fused row-kernel shape (one per instantiated shape)
+FUSED_LENGTHS = [34992, 33600, 17496, 16464, 5184, 4704]
+
+
+@pytest.mark.parametrize("M", FUSED_LENGTHS)
+@pytest.mark.parametrize("rows,tile", [(13, 128), (2, 128), (40, 5)])
+def test_fused_row_kernel(gpu, oracle_mod, M, rows, tile):
+    """k_rowfused for every instantiated shape: against the FP64 oracle and against
+    the four-step kernels at the same M, for the longest row the length allows
+    (cols = (M + 1) / 2) and a shorter one, odd and even row counts, row counts
+    that are not a multiple of the rows per workgroup, and several tiles."""
+    sp = gpu
+    for cols in ((M + 1) // 2, M // 3 + 5):
+        rng = np.random.default_rng(M + cols + rows)
+        r, c = _random_coo(rng, rows, cols, 4 * cols)
+        print(f"[fused] M={M} rows={rows} cols={cols} tile={tile}", flush=True)
+        plan, X = _run(sp, rows, cols, r, c, fft_len=M, u_tile=tile, row_kernel=2)
+        assert plan.info["row_kernel"] == 2 and plan.info["fft_len"] == M
+        _, X4 = _run(sp, rows, cols, r, c, fft_len=M, u_tile=tile, row_kernel=1)
+        ref = oracle_mod.sparse_rfft2_f64(rows, cols, r, c)
+        err, err4 = oracle_mod.rel_l2(X, ref), oracle_mod.rel_l2(X4, ref)
+        print(f"  plan m1={plan.info['m1']} radix={plan.info['conv_radix']} rel_l2 fused={err:.3e} "
+              f"four-step={err4:.3e}")
+        assert err <= REL_L2_PLAIN and err4 <= REL_L2_PLAIN
+        assert oracle_mod.rel_l2(X, X4) <= 2 * REL_L2_PLAIN
 
 
 @pytest.mark.parametrize("rows_per_lane", [1, 2, 4])
