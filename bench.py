@@ -35,6 +35,7 @@ def parse():
     ap.add_argument("--rule", type=int, default=3)
     ap.add_argument("--tile", type=int, default=0, help="u-tile rows (0 = tuned default)")
     ap.add_argument("--m1", type=int, default=0)
+    ap.add_argument("--fft-len", type=int, default=0, help="explicit Bluestein length M (0: the rule's)")
     ap.add_argument("--precision", type=int, default=0, help="0 plain FP32, 1 BF16x3 MFMA")
     ap.add_argument("--colpass-rows", type=int, default=0)
     ap.add_argument("--colpass-table", type=int, default=0)
@@ -215,7 +216,7 @@ def main():
     dev = torch.device("cuda:0")
     m = sp.config_matrix(args.config)
     tile = args.tile if args.tile > 0 else m.rows
-    plan = sp.Plan(m.rows, m.cols, length_rule=args.rule, u_tile=tile, m1=args.m1,
+    plan = sp.Plan(m.rows, m.cols, length_rule=args.rule, u_tile=tile, m1=args.m1, fft_len=args.fft_len,
                    precision=args.precision, row_kernel=args.row_kernel,
                    colpass_rows=args.colpass_rows, colpass_table=args.colpass_table)
     d_row = torch.as_tensor(m.row_idx).to(dev)

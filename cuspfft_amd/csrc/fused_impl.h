@@ -10,21 +10,23 @@
 //            KB = M1 / NB rows: each thread swaps its columns of the batch with
 //            the rows processed in the previous batch, then all threads run the
 //            Stockham stages R1, R2, R3 forward (the first one times W_M^{k1 n2}
-//            on its reads, the last one times B^[k1 + M1 k2] / M) and inverse
-//            (the last one times W_M^{-k1 n2} on its writes), in place in LDS.
+//            on its reads, the last one times B^[k1 + M1 k2] / M) and R3, R2, R1
+//            inverse (the last one times W_M^{-k1 n2} on its writes), in place
+//            in LDS; the last forward and first inverse stage share one pass.
 //   phase C  each column runs the inverse length-M1 codelet, applies the chirp
 //            and writes row u and its mirror row R - u
 //            (X[R-u][v] = conj(X[u][(C - v) mod C]) for a real input).
 // HBM traffic per row: the pass-1 row in, the two half-spectrum rows out.
 //
-// Global memory is read and written with buffer instructions.  Table reads
-// (always in range) put the per-thread part of the offset in a VGPR and the
-// strided part (kbase * L + r * nb) in a scalar offset.  Row reads and writes
-// that may fall outside the row are predicated without branches: their offset
-// is replaced by kOob, past the end of the buffer, so the load returns 0 (the
-// zero padding of x for n >= C) and the store is dropped (v > C/2).  Branches
-// around each of the M1 accesses would keep the whole column live in registers.
-// RPW > 1 puts RPW short rows in one workgroup.
+// Global memory is read and written with buffer instructions: the per-thread
+// part of an offset in a VGPR, the strided part (M2 n1, kbase * L + r * nb) in
+// a scalar offset.  On gfx950 the buffer bounds check takes the sum of both, so
+// a row made a buffer of its valid entries gets the zero padding of x (reads
+// past C return 0) and drops the writes past v = C/2 with no per-element code;
+// other accesses that may leave the row take the offset kOob (past the end of
+// every buffer) instead of a branch, since branches around each of the M1
+// accesses would keep the whole column live in registers.  RPW > 1 puts RPW
+// short rows in one workgroup (one descriptor for all, explicit bounds).
 #pragma once
 #include "rowpass_internal.h"
 
@@ -63,16 +65,33 @@ __device__ __forceinline__ void store2(float2 x, rsrc_t r, int voff, int soff) {
 
 // One in-place Stockham stage of radix R over the G rows (length L) in LDS,
 // NT threads.  Row g belongs to k1 = kbase + g % KB.  Reads of the whole stage
-// finish at a barrier before any write; the stage ends with a barrier.  TWIN
-// multiplies the values read by W_M^{k1 n2} (first forward stage), TWOUT the
-// values written by W_M^{-k1 n2} (last inverse stage), MUL by B^ (last forward).
-template <int R, int L, int NS, bool INV, bool MUL, bool TWIN, bool TWOUT, int G, int KB, int NT>
-__device__ __forceinline__ void fused_stage(fused::rsrc_t r_tw, fused::rsrc_t r_bh, float2* buf,
-                                            const float2* s_tw, int kbase) {
+// finish at a barrier before any write; the stage ends with a barrier.
+//   MODE kFwd / kInv: a forward / inverse stage after stages with product NS.
+//   MODE kPivot: the last forward stage (NS = L / R), the product with B^, and
+//     the first inverse stage, which runs the inverse radices in reverse order
+//     so that it takes the same R values as the butterfly that produced them
+//     (j + r L/R): one LDS round trip for both.
+// TWIN multiplies the values read by W_M^{k1 n2} (first forward stage), TWOUT
+// the values written by W_M^{-k1 n2} (last inverse stage), both from an LDS
+// table (a global table read here would stall the stage on its latency).
+enum { kFwd = 0, kInv = 1, kPivot = 2 };
+// the W_M^e table: 256 low entries and (M + 255) / 256 high ones, M <= 65536
+constexpr int kFusedWmEntries = 256 + 256;
+
+// W_M^e, e = k1 n2 < M, from the two-level LDS table hi[e >> 8] * lo[e & 255].
+__device__ __forceinline__ float2 wm_pow(const float2* s_wm, int e) {
+    return cmul(s_wm[256 + (e >> 8)], s_wm[e & 255]);
+}
+
+template <int R, int L, int NS, int MODE, bool TWIN, bool TWOUT, int G, int KB, int NT>
+__device__ __forceinline__ void fused_stage(fused::rsrc_t r_bh, float2* buf, const float2* s_tw,
+                                            const float2* s_wm, int kbase) {
     constexpr int nb = L / R;
     constexpr int total = G * nb;
     constexpr int MB = (total + NT - 1) / NT;
     constexpr int step = L / (NS * R);
+    constexpr bool INV = MODE == kInv;
+    static_assert(MODE != kPivot || NS == nb, "the pivot stage is the last forward stage");
     float2 v[MB][R];
 #pragma unroll
     for (int b = 0; b < MB; b++) {
@@ -95,10 +114,10 @@ __device__ __forceinline__ void fused_stage(fused::rsrc_t r_tw, fused::rsrc_t r_
             const int jq = j / NS;
             const int k = j - jq * NS;
             const int rowoff = (g % KB) * L;  // (k1 - kbase) * L
+            const int k1 = kbase + g % KB;
             if (TWIN) {
 #pragma unroll
-                for (int r = 0; r < R; r++)
-                    v[b][r] = cmul(v[b][r], fused::load2(r_tw, (rowoff + j) * 8, (kbase * L + r * nb) * 8));
+                for (int r = 0; r < R; r++) v[b][r] = cmul(v[b][r], wm_pow(s_wm, k1 * (j + r * nb)));
             }
             if (NS > 1 && k != 0) {
 #pragma unroll
@@ -108,34 +127,49 @@ __device__ __forceinline__ void fused_stage(fused::rsrc_t r_tw, fused::rsrc_t r_
                 }
             }
             RegFft<R>::run(v[b], INV);
-            const int base = jq * NS * R + k;
-            if (MUL) {
+            int base = jq * NS * R + k;
+            int stride = NS;
+            if (MODE == kPivot) {
+                // B^ at the forward outputs j + r NS, then the inverse radix-R stage
+                // with no earlier stages: outputs j R + r
 #pragma unroll
                 for (int r = 0; r < R; r++)
                     v[b][r] = cmul(v[b][r], fused::load2(r_bh, (rowoff + base) * 8, (kbase * L + r * NS) * 8));
+                RegFft<R>::run(v[b], true);
+                base = j * R;
+                stride = 1;
             }
             if (TWOUT) {
 #pragma unroll
-                for (int r = 0; r < R; r++)
-                    v[b][r] = cmulc(v[b][r], fused::load2(r_tw, (rowoff + base) * 8, (kbase * L + r * NS) * 8));
+                for (int r = 0; r < R; r++) v[b][r] = cmulc(v[b][r], wm_pow(s_wm, k1 * (base + r * stride)));
             }
             float2* dst = buf + g * L + base;
 #pragma unroll
-            for (int r = 0; r < R; r++) dst[r * NS] = v[b][r];
+            for (int r = 0; r < R; r++) dst[r * stride] = v[b][r];
         }
     }
     __syncthreads();
 }
 
-// The stage list R, Rest... of one length-L FFT (radices of 1 are no stage).
-template <int L, int NS, bool INV, int G, int KB, int NT, int R, int... Rest>
-__device__ __forceinline__ void fused_stages(fused::rsrc_t r_tw, fused::rsrc_t r_bh, float2* buf,
-                                             const float2* s_tw, int kbase) {
-    constexpr bool last = sizeof...(Rest) == 0 || ((Rest == 1) && ...);
-    if constexpr (R > 1)
-        fused_stage<R, L, NS, INV, !INV && last, !INV && NS == 1, INV && last, G, KB, NT>(r_tw, r_bh, buf, s_tw,
-                                                                                          kbase);
-    if constexpr (!last) fused_stages<L, NS * R, INV, G, KB, NT, Rest...>(r_tw, r_bh, buf, s_tw, kbase);
+// The convolution of the batch's rows: forward stages R1 .. Rn, B^, inverse
+// stages Rn .. R1 (radices of 1 are no stage), the last forward and first
+// inverse stage merged (kPivot).
+template <int L, int G, int KB, int NT, int R1, int R2, int R3>
+__device__ __forceinline__ void fused_conv(fused::rsrc_t r_bh, float2* buf, const float2* s_tw,
+                                           const float2* s_wm, int kbase) {
+    if constexpr (R3 > 1) {
+        fused_stage<R1, L, 1, kFwd, true, false, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
+        fused_stage<R2, L, R1, kFwd, false, false, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
+        fused_stage<R3, L, R1 * R2, kPivot, false, false, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
+        fused_stage<R2, L, R3, kInv, false, false, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
+        fused_stage<R1, L, R3 * R2, kInv, false, true, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
+    } else if constexpr (R2 > 1) {
+        fused_stage<R1, L, 1, kFwd, true, false, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
+        fused_stage<R2, L, R1, kPivot, false, false, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
+        fused_stage<R1, L, R2, kInv, false, true, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
+    } else {
+        fused_stage<R1, L, 1, kPivot, true, true, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
+    }
 }
 
 template <int M1, int NB, int RPW, int NT, int R1, int R2, int R3>
@@ -148,8 +182,9 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
     static_assert(RPW == 1 || CPT == 1, "several rows per workgroup: one column per thread");
     static_assert(R1 > 1 && (R2 > 1 || R3 == 1), "stage radices first");
     extern __shared__ float2 smem[];
-    float2* s_tw = smem;      // W_M2^i, i < M2
-    float2* buf = smem + M2;  // G rows of length M2: [row of workgroup][k1 of batch][n2]
+    float2* s_tw = smem;                    // W_M2^i, i < M2
+    float2* s_wm = s_tw + M2;               // W_M^e: 256 low + (M + 255) / 256 high entries
+    float2* buf = s_wm + kFusedWmEntries;   // G rows of length M2: [row of workgroup][k1 of batch][n2]
     const int t = threadIdx.x;
     const int row0 = blockIdx.x * RPW;  // first tile row of this workgroup
     int n2[CPT], g[CPT];
@@ -162,23 +197,29 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
         live[c] = (RPW == 1 ? q < M2 : g[c] < RPW) && row0 + g[c] < a.tile_rows;
     }
     for (int i = t; i < M2; i += NT) s_tw[i] = a.tw2[i];
-    const fused::rsrc_t r_tw = fused::make_rsrc(a.twM, (uint32_t)a.M * 8u);
+    for (int i = t; i < 256 + (a.M + 255) / 256; i += NT) s_wm[i] = a.wm2[i];
     const fused::rsrc_t r_bh = fused::make_rsrc(a.bhat, (uint32_t)a.M * 8u);
 
     // phase A
     float2 v[CPT][M1];
     {
         const int nrows = min(RPW, a.tile_rows - row0);
-        const fused::rsrc_t r_x =
-            fused::make_rsrc(a.X0 + (size_t)row0 * a.ldx, (uint32_t)(nrows * a.ldx) * 8u);
+        // RPW == 1: the row as a buffer of C entries, so reads past it return 0 (the
+        // zero padding) with the stride M2 n1 as the scalar offset
+        const fused::rsrc_t r_x = fused::make_rsrc(a.X0 + (size_t)row0 * a.ldx,
+                                                   (uint32_t)(RPW == 1 ? a.C : nrows * a.ldx) * 8u);
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
             if (live[c]) {
                 const int rb = g[c] * a.ldx;
 #pragma unroll
                 for (int n1 = 0; n1 < M1; n1++) {
-                    const int w = n2[c] + M2 * n1;
-                    v[c][n1] = fused::load2(r_x, w < a.C ? (rb + w) * 8 : fused::kOob, 0);
+                    if constexpr (RPW == 1) {
+                        v[c][n1] = fused::load2(r_x, n2[c] * 8, M2 * n1 * 8);
+                    } else {
+                        const int w = n2[c] + M2 * n1;
+                        v[c][n1] = fused::load2(r_x, w < a.C ? (rb + w) * 8 : fused::kOob, 0);
+                    }
                 }
                 RegFft<M1>::run(v[c], false);
             }
@@ -205,34 +246,64 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
             }
         }
         __syncthreads();
-        fused_stages<M2, 1, false, G, KB, NT, R1, R2, R3>(r_tw, r_bh, buf, s_tw, b * KB);
-        fused_stages<M2, 1, true, G, KB, NT, R1, R2, R3>(r_tw, r_bh, buf, s_tw, b * KB);
+        fused_conv<M2, G, KB, NT, R1, R2, R3>(r_bh, buf, s_tw, s_wm, b * KB);
     });
 
-    // phase C.  Buffer descriptors must be uniform: one for the workgroup's output
-    // rows u0 .. u0 + nrows - 1 and one for their mirror rows R - u (the lowest is mlo).
-    const fused::rsrc_t r_h = fused::make_rsrc(a.h, (uint32_t)a.C * 8u);
+    // phase C
+    const fused::rsrc_t r_h = fused::make_rsrc(a.h, (uint32_t)a.C * 8u);  // reads past C return 0
     const int u0 = a.u_base + row0;
-    const int nrows = min(RPW, a.tile_rows - row0);
-    const int mlo = a.R - (u0 + nrows - 1);
-    const fused::rsrc_t r_o = fused::make_rsrc(a.out + (size_t)u0 * a.P, (uint32_t)(nrows * a.P) * 8u);
-    const fused::rsrc_t r_m = fused::make_rsrc(a.out + (size_t)mlo * a.P, (uint32_t)(nrows * a.P) * 8u);
+    if constexpr (RPW == 1) {
+        // Row u as a buffer of C/2 + 1 entries: writes of v > C/2 are dropped.  Mirror
+        // row R - u: X[R-u][C - w] = conj(X[u][w]) for 0 < w < C, at offset
+        // M - 1 - w = (M2 (M1 - 1 - n1) + (M2 - 1 - n2)) from a base M - 1 - C entries
+        // before the row, a buffer ending at entry C/2 of the row: writes of
+        // C - w > C/2 are dropped, those of w >= C take kOob, w = 0 is written apart.
+        const int u = u0;
+        const bool mirror = u > 0 && a.R - u > u;
+        const int dm = a.M - 1 - a.C;
+        const fused::rsrc_t r_o = fused::make_rsrc(a.out + (size_t)u * a.P, (uint32_t)a.V * 8u);
+        const fused::rsrc_t r_m = fused::make_rsrc(a.out + (size_t)(a.R - u) * a.P - dm,
+                                                   mirror ? (uint32_t)(dm + a.V) * 8u : 0u);
 #pragma unroll
-    for (int c = 0; c < CPT; c++) {
-        if (live[c]) {
-            const float2* col = buf + g[c] * KB * M2 + n2[c];
+        for (int c = 0; c < CPT; c++) {
+            if (live[c]) {
+                const float2* col = buf + n2[c];
 #pragma unroll
-            for (int k = 0; k < KB; k++) v[c][(NB - 1) * KB + k] = col[k * M2];
-            RegFft<M1>::run(v[c], true);
-            const int u = u0 + g[c];
+                for (int k = 0; k < KB; k++) v[c][(NB - 1) * KB + k] = col[k * M2];
+                RegFft<M1>::run(v[c], true);
+                const int n1c = (a.C - n2[c] + M2 - 1) / M2;  // columns w = n2 + M2 n1 < C
+                const int vm0 = (M2 - 1 - n2[c]) * 8;
+#pragma unroll
+                for (int n1 = 0; n1 < M1; n1++) {
+                    const float2 x = cmul(v[c][n1], fused::load2(r_h, n2[c] * 8, M2 * n1 * 8));
+                    fused::store2(x, r_o, n2[c] * 8, M2 * n1 * 8);
+                    fused::store2(make_float2(x.x, -x.y), r_m, n1 < n1c ? vm0 : fused::kOob,
+                                  M2 * (M1 - 1 - n1) * 8);
+                    if (n1 == 0 && n2[c] == 0) fused::store2(make_float2(x.x, -x.y), r_m, dm * 8, 0);
+                }
+            }
+        }
+    } else {
+        // one descriptor for the workgroup's output rows u0 .. u0 + nrows - 1 and one for
+        // their mirror rows R - u (the lowest is mlo); explicit bounds
+        const int nrows = min(RPW, a.tile_rows - row0);
+        const int mlo = a.R - (u0 + nrows - 1);
+        const fused::rsrc_t r_o = fused::make_rsrc(a.out + (size_t)u0 * a.P, (uint32_t)(nrows * a.P) * 8u);
+        const fused::rsrc_t r_m = fused::make_rsrc(a.out + (size_t)mlo * a.P, (uint32_t)(nrows * a.P) * 8u);
+        if (live[0]) {
+            const float2* col = buf + g[0] * KB * M2 + n2[0];
+#pragma unroll
+            for (int k = 0; k < KB; k++) v[0][(NB - 1) * KB + k] = col[k * M2];
+            RegFft<M1>::run(v[0], true);
+            const int u = u0 + g[0];
             const bool mirror = u > 0 && a.R - u > u;
-            const int ob = g[c] * a.P;                  // row u in r_o
+            const int ob = g[0] * a.P;                          // row u in r_o
             const int mb = mirror ? (a.R - u - mlo) * a.P : 0;  // row R - u in r_m
 #pragma unroll
             for (int n1 = 0; n1 < M1; n1++) {
-                const int w = n2[c] + M2 * n1;
+                const int w = n2[0] + M2 * n1;
                 const bool in = w < a.C;
-                const float2 x = cmul(v[c][n1], fused::load2(r_h, in ? w * 8 : fused::kOob, 0));
+                const float2 x = cmul(v[0][n1], fused::load2(r_h, n2[0] * 8, M2 * n1 * 8));
                 fused::store2(x, r_o, w < a.V ? (ob + w) * 8 : fused::kOob, 0);
                 const int vm = w == 0 ? 0 : a.C - w;
                 fused::store2(make_float2(x.x, -x.y), r_m, in && mirror && vm < a.V ? (mb + vm) * 8 : fused::kOob, 0);
@@ -246,16 +317,18 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
 // without spills.  Group A: the long rows (several LDS batches); group B: one batch.
 //   34992 = 36 * 972   (3-smooth length of 2 * 16706 - 1)
 //   33600 = 35 * 960   (7-smooth length of 2 * 16706 - 1)
+//   36864 = 36 * 1024, 35840 = 35 * 1024 (power-of-two convolution lengths)
 //   17496 = 24 * 729, 16464 = 21 * 784, 5184 = 16 * 324 (3 rows), 4704 = 21 * 224 (4 rows)
-#define SPFFT_FUSED_GROUP_A(X) \
-    X(36, 2, 1, 512, 12, 9, 9) X(35, 5, 1, 512, 8, 12, 10)
+#define SPFFT_FUSED_GROUP_A(X)                                                              \
+    X(36, 2, 1, 512, 9, 12, 9) X(35, 5, 1, 512, 8, 12, 10) X(36, 3, 1, 512, 16, 8, 8)     \
+    X(35, 5, 1, 512, 16, 8, 8)
 #define SPFFT_FUSED_GROUP_B(X)                                                                  \
     X(24, 1, 1, 768, 27, 27, 1) X(21, 1, 1, 1024, 28, 28, 1) X(16, 1, 3, 1024, 18, 18, 1) \
     X(21, 1, 4, 1024, 8, 28, 1)
 #define SPFFT_FUSED_LIST(X) SPFFT_FUSED_GROUP_A(X) SPFFT_FUSED_GROUP_B(X)
 
 constexpr size_t fused_lds_bytes_of(int m1, int nb, int rpw, int m2) {
-    return (size_t)m2 * sizeof(float2) * (1 + (size_t)rpw * (m1 / nb));
+    return ((size_t)m2 * (1 + (size_t)rpw * (m1 / nb)) + kFusedWmEntries) * sizeof(float2);
 }
 
 // Each returns hipErrorInvalidValue when the shape is not in its group.

@@ -38,9 +38,62 @@ namespace spfft {
 
 namespace {
 
+constexpr int kTileStride = 65;
+constexpr int kColThreads = 512;
+constexpr int kColWaves = kColThreads / 64;
+
+// ---------------------------------------------------------------------------
+// k_colsched: for every 64-column tile of k_colpass, shares its columns out to
+// the kColWaves waves by longest-processing-time first: power-of-two classes of
+// cost (nonzeros + 1) from the largest down, each column to the least-loaded
+// wave.  One wave per tile; lane = column.  Output: the tile's columns grouped
+// by wave, sched[tile][0 .. 64), and the groups' offsets off[tile][0 .. kColWaves].
+// Columns past C are not listed.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_colsched(const int* __restrict__ colstart, int C,
+                                                 int* __restrict__ sched, int* __restrict__ off) {
+    const int lane = threadIdx.x;
+    const int c = blockIdx.x * 64 + lane;
+    const int cost = c < C ? colstart[c + 1] - colstart[c] + 1 : 0;
+    const int cls = cost > 0 ? 31 - __clz(cost) : -1;
+    int load[kColWaves] = {}, cnt[kColWaves] = {};
+    int mine_w = 0, mine_k = 0;  // this lane's column: wave and position in its group
+    for (int b = 30; b >= 0; b--) {
+        uint64_t m = __ballot(cls == b);
+        while (m) {
+            const int l = __ffsll((unsigned long long)m) - 1;
+            m &= m - 1;
+            int w = 0;
+#pragma unroll
+            for (int k = 1; k < kColWaves; k++) w = load[k] < load[w] ? k : w;
+            if (lane == l) {
+                mine_w = w;
+                mine_k = cnt[w];
+            }
+            load[w] += __builtin_amdgcn_readlane(cost, l);
+            cnt[w] += 1;
+        }
+    }
+    int start[kColWaves + 1];
+    start[0] = 0;
+#pragma unroll
+    for (int k = 0; k < kColWaves; k++) start[k + 1] = start[k] + cnt[k];
+    int sw = 0;
+#pragma unroll
+    for (int k = 0; k < kColWaves; k++) sw = mine_w == k ? start[k] : sw;
+    if (cls >= 0) sched[blockIdx.x * 64 + sw + mine_k] = lane;
+    if (lane <= kColWaves) {
+        int v = 0;
+#pragma unroll
+        for (int k = 0; k <= kColWaves; k++) v = lane == k ? start[k] : v;
+        off[blockIdx.x * (kColWaves + 1) + lane] = v;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_colpass: a tile of 64 columns x 64*U rows per workgroup (8 waves).
-// Wave w takes columns c0 + w + 8i, one at a time, so the column is uniform over
+// The tile's columns are shared out to the waves by k_colsched, once per
+// transform; a wave takes its columns one at a time, so the column is uniform over
 // the wave; lane l takes rows u0 + l + 64t (t < U).  A column's row indices are
 // read 64 at a time (one per lane) together with their per-nonzero constants,
 // computed once per lane:
@@ -56,13 +109,9 @@ namespace {
 // planes, row stride 65: conflict-free both ways), so X0 is written in whole
 // 64-column row segments.
 // ---------------------------------------------------------------------------
-constexpr int kTileStride = 65;
-constexpr int kColThreads = 512;
-
 template <bool PACKED, bool FULL, int U>
 __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
     extern __shared__ float2 smem[];
-    constexpr int kWaves = kColThreads / 64;
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c0 = blockIdx.x * 64;
@@ -90,8 +139,12 @@ __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
         return FULL ? s_lo[ph] : cmul(s_hi[ph >> hs], s_lo[ph & mask]);
     };
     const uint32_t u0 = (uint32_t)(a.u_base + u0l);
-    for (int i = 0; i < 64 / kWaves; i++) {
-        const int cc = wid + kWaves * i;  // column within the tile (uniform)
+    // this wave's columns (k_colsched): sched[blk][off[w] .. off[w+1])
+    const int* sched = a.col_sched + blockIdx.x * 64;
+    const int* off = a.col_sched_off + blockIdx.x * (kColWaves + 1);
+    const int i0 = __builtin_amdgcn_readfirstlane(off[wid]), i1 = __builtin_amdgcn_readfirstlane(off[wid + 1]);
+    for (int i = i0; i < i1; i++) {
+        const int cc = __builtin_amdgcn_readfirstlane(sched[i]);  // column of the tile
         const int c = c0 + cc;
         float2 acc[U];
 #pragma unroll
@@ -173,6 +226,18 @@ size_t colpass_lds_bytes(const RowPassArgs& a) {
     const size_t tile = (size_t)2 * 64 * U * kTileStride * sizeof(float) + 66 * sizeof(int);
     if (a.wr_full) return tile + (size_t)a.R * sizeof(float2);
     return tile + ((size_t)(1 << a.hi_shift) + a.n_hi) * sizeof(float2);
+}
+
+size_t colsched_elems(int cols, size_t* off_elems) {
+    const size_t tiles = ((size_t)cols + 63) / 64;
+    *off_elems = tiles * (kColWaves + 1);
+    return tiles * 64;
+}
+
+hipError_t launch_colsched(const int* colstart, int cols, int* sched, int* off, hipStream_t s) {
+    if (cols <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_colsched, dim3((cols + 63) / 64), dim3(64), 0, s, colstart, cols, sched, off);
+    return hipGetLastError();
 }
 
 hipError_t colpass_prepare(size_t lds_limit) {

@@ -38,6 +38,10 @@ bool mfma_supports_m1(int m1);           // precision-form (BF16x3 MFMA) sub-FFT
 bool conv_fast_supported(int r1, int r2); // k_conv_fast<r1, r2> instantiated (r2 = 1: one stage)
 int conv_max_butterflies(int r);         // butterflies per thread per k_conv_fast stage
 size_t colpass_lds_bytes(const RowPassArgs& a);   // needs a.colpass_rows and a.wr_full set
+// k_colsched: the per-tile column schedule of k_colpass (once per CSC build);
+// sizes in ints of sched (returned) and of off (*off_elems)
+size_t colsched_elems(int cols, size_t* off_elems);
+hipError_t launch_colsched(const int* colstart, int cols, int* sched, int* off, hipStream_t s);
 hipError_t colpass_prepare(size_t lds_limit);      // once per process, before launches
 size_t conv_lds_bytes(const RowPassArgs& a);
 // Fused row kernel (k_rowfused) shapes: M1, LDS batches, rows per workgroup,

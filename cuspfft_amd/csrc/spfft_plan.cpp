@@ -323,6 +323,8 @@ spfft_status do_csc_build(spfft_plan_s& p, const int* d_row, const int* d_col, i
         SPFFT_HIP(hipEventRecord(ev[0], s), "hipEventRecord");
     }
     SPFFT_HIP(spfft::launch_csc_build(a, s), "csc build launch");
+    SPFFT_HIP(spfft::launch_colsched(a.colstart, p.C, p.col_sched.as<int>(), p.col_sched_off.as<int>(), s),
+              "column schedule launch");
     if (ev) SPFFT_HIP(hipEventRecord(ev[1], s), "hipEventRecord");
     p.last_nnz = nnz;
     p.have_csc = true;
@@ -338,12 +340,16 @@ spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t
     a.P = (int)pitch;
     if (a.colpass_rows == 0) {
         // two rows per lane: the second row costs one complex product per nonzero
-        // instead of a phase reduction and a table lookup
-        a.colpass_rows = 2;
+        // instead of a phase reduction and a table lookup, which pays off on
+        // long columns; short ones keep twice as many workgroups busy with one
+        const double avg = (double)p.last_nnz / std::max(p.C, 1);
+        a.colpass_rows = avg >= 12.0 ? 2 : 1;
     }
     // the whole W_R table only while it fits in LDS next to the row tile
     if (a.wr_full && spfft::colpass_lds_bytes(a) > g_lds_limit) a.wr_full = nullptr;
     a.colstart = p.colstart.as<int>();
+    a.col_sched = p.col_sched.as<int>();
+    a.col_sched_off = p.col_sched_off.as<int>();
     a.row_idx = p.row_idx.p;
     a.out = static_cast<float2*>(d_out);
     if (T) a.T = T;
@@ -672,6 +678,13 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
     }
     SPFFT_HIP(upload(p->tw2, t2), "upload twiddles");
     SPFFT_HIP(upload(p->twM, tm), "upload twiddles");
+    if (a.fused) {
+        // two-level W_M table of the fused row kernel: low 256 entries, then W_M^{256 i}
+        std::vector<float2> w2(256 + (M + 255) / 256);
+        for (int e = 0; e < 256; e++) w2[e] = to_f2(unit_root(e, M, -1));
+        for (int i = 0; i < (M + 255) / 256; i++) w2[256 + i] = to_f2(unit_root(256LL * i, M, -1));
+        SPFFT_HIP(upload(p->wm2, w2), "upload twiddles");
+    }
     SPFFT_HIP(upload(p->bhat, bh), "upload B^");
     if (a.precision == 1) {
         // cos and sin of 2 pi i / M1 split into BF16 hi + lo parts from FP64
@@ -701,18 +714,26 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
     SPFFT_HIP(p->col_ptr.alloc((size_t)(cols + 1) * 4), "hipMalloc(col_ptr)");
     SPFFT_HIP(p->totals.alloc(2 * 4), "hipMalloc(totals)");
     SPFFT_HIP(p->err.alloc(4), "hipMalloc(err)");
+    {
+        size_t off_elems = 0;
+        const size_t sched_elems = spfft::colsched_elems(cols, &off_elems);
+        SPFFT_HIP(p->col_sched.alloc(sched_elems * 4), "hipMalloc(column schedule)");
+        SPFFT_HIP(p->col_sched_off.alloc(off_elems * 4), "hipMalloc(column schedule)");
+    }
     a.h = p->h.as<float2>();
     a.wr_hi = p->wr_hi.as<float2>();
     a.wr_lo = p->wr_lo.as<float2>();
     a.wr_full = p->colpass_full ? p->wr_full.as<float2>() : nullptr;
     a.tw2 = p->tw2.as<float2>();
     a.twM = p->twM.as<float2>();
+    a.wm2 = a.fused ? p->wm2.as<float2>() : nullptr;
     a.bhat = p->bhat.as<float2>();
     a.mfma_dft = a.precision == 1 ? p->mfma_dft.as<float>() : nullptr;
     a.T = p->T.as<float2>();
     a.X0 = p->X0.as<float2>();
     if (a.fused) SPFFT_HIP(spfft::launch_fused(a, nullptr, true), "fused row kernel LDS limit");
     p->table_bytes = p->h.bytes + p->wr_lo.bytes + p->wr_hi.bytes + p->wr_full.bytes + p->tw2.bytes + p->twM.bytes +
+                     p->wm2.bytes +
                      p->bhat.bytes + p->T.bytes + p->X0.bytes;
     *out = guard.release();
     return SPFFT_OK;

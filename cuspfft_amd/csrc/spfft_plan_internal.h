@@ -63,12 +63,13 @@ struct spfft_plan_s {
     bool packed = false;
     bool colpass_full = false;
     spfft::RowPassArgs base{};  // kernel argument template (pointers to the tables filled in)
-    DevBuf h, wr_hi, wr_lo, wr_full, tw2, twM, bhat, mfma_dft, X0, T;
+    DevBuf h, wr_hi, wr_lo, wr_full, tw2, twM, wm2, bhat, mfma_dft, X0, T;
     // CSC workspace
     int nnz_cap = -1;
     int last_nnz = 0;
     bool have_csc = false;
     DevBuf counts, colstart, sparse_cols, col_ptr, totals, err, scan_tmp, hist, hist_off;
+    DevBuf col_sched, col_sched_off;  // k_colpass column schedule
     DevBuf keys0, keys1, vals0, vals1, row_idx;
     size_t table_bytes = 0;
     // optional per-kernel event timing (spfft_profile_*)

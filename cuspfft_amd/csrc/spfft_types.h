@@ -40,6 +40,8 @@ struct RowPassArgs {
     uint64_t r_magic;          // fastmod magic for R
     FftStages f2;              // radix stages of the length-M2 FFT in k_conv
     const int* colstart;       // C+1 dense column starts into row_idx
+    const int* col_sched;      // k_colpass: per 64-column tile, its columns grouped by wave
+    const int* col_sched_off;  // k_colpass: per tile, the groups' offsets (waves + 1)
     const void* row_idx;       // uint16 (packed) or int32 rows, CSC order
     const float2* h;           // chirp h[k] = exp(-i pi k^2 / C), k < C
     const float2* wr_hi;
@@ -47,6 +49,7 @@ struct RowPassArgs {
     const float2* wr_full;     // W_R^p, p < R, when the whole table fits in LDS (else NULL)
     const float2* tw2;         // W_M2^i, i < M2
     const float2* twM;         // [k1*M2 + n2] = W_M^{k1 n2}
+    const float2* wm2;         // W_M^e = wm2[256 + (e >> 8)] * wm2[e & 255], e < M (k_rowfused)
     const float2* bhat;        // [k1*M2 + k2] = B^[k1 + M1 k2] / M
     const float* mfma_dft;     // precision form: cos/sin(2 pi i / M1) hi and lo BF16 parts, [4][M1]
     int precision;             // 0: FP32 codelets, 1: BF16x3 MFMA length-M1 sub-FFTs
