@@ -50,13 +50,14 @@ def candidates(rows: int, cols: int, max_m1: int = 3) -> list[dict]:
             if d in m1s or M % d:
                 continue
             try:
-                info = describe_plan(rows, cols, length_rule=rule, m1=d, row_kernel=ROW_FOUR_STEP)
+                info = describe_plan(rows, cols, fft_len=M, m1=d, row_kernel=ROW_FOUR_STEP)
             except SpfftError:
                 continue
             if info["conv_fast"]:
                 m1s.append(d)
         for m1 in m1s:
-            out.append(dict(length_rule=rule, m1=m1, precision=0, row_kernel=ROW_FOUR_STEP))
+            # M pinned: the rule's length may lack this split (the planner then moves on)
+            out.append(dict(length_rule=rule, fft_len=M, m1=m1, precision=0, row_kernel=ROW_FOUR_STEP))
         out.append(dict(length_rule=rule, m1=0, precision=PREC_BF16X3, row_kernel=ROW_FOUR_STEP))
         if describe_plan(rows, cols, length_rule=rule)["row_kernel"] == ROW_FUSED:
             out.append(dict(length_rule=rule, m1=0, precision=0, row_kernel=ROW_FUSED))
@@ -83,7 +84,7 @@ def time_plan(rows: int, cols: int, d_row, d_col, opts: dict, reps: int = 3, row
     import torch
     full = rows // 2 + 1
     lim = row_limit if 0 < row_limit < full else 0
-    plan = Plan(rows, cols, u_tile=rows, row_limit=lim, **opts)
+    plan = Plan(rows, cols, u_tile=lim or rows, row_limit=lim, **opts)
     out = plan.alloc_output()
     plan.execute(d_row, d_col, out)
     torch.cuda.synchronize()

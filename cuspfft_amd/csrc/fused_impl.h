@@ -52,13 +52,22 @@ __device__ f32x2 raw_load_v2f32(rsrc_t r, int voff, int soff, int aux) __asm("ll
 __device__ void raw_store_v2f32(f32x2 v, rsrc_t r, int voff, int soff, int aux) __asm(
     "llvm.amdgcn.raw.ptr.buffer.store.v2f32");
 
+// Cache policy of the accesses made once per transform (the pass-1 row read,
+// the output writes): non-temporal (bit 1 of the aux operand on gfx950), so they
+// do not evict the tables every row reuses from L2.
+constexpr int kStream = 2;
+
 __device__ __forceinline__ float2 load2(rsrc_t r, int voff, int soff) {
     const f32x2 d = raw_load_v2f32(r, voff, soff, 0);
     return make_float2(d[0], d[1]);
 }
-__device__ __forceinline__ void store2(float2 x, rsrc_t r, int voff, int soff) {
+__device__ __forceinline__ float2 load2_stream(rsrc_t r, int voff, int soff) {
+    const f32x2 d = raw_load_v2f32(r, voff, soff, kStream);
+    return make_float2(d[0], d[1]);
+}
+__device__ __forceinline__ void store2_stream(float2 x, rsrc_t r, int voff, int soff) {
     const f32x2 d = {x.x, x.y};
-    raw_store_v2f32(d, r, voff, soff, 0);
+    raw_store_v2f32(d, r, voff, soff, kStream);
 }
 
 }  // namespace fused
@@ -215,10 +224,10 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
 #pragma unroll
                 for (int n1 = 0; n1 < M1; n1++) {
                     if constexpr (RPW == 1) {
-                        v[c][n1] = fused::load2(r_x, n2[c] * 8, M2 * n1 * 8);
+                        v[c][n1] = fused::load2_stream(r_x, n2[c] * 8, M2 * n1 * 8);
                     } else {
                         const int w = n2[c] + M2 * n1;
-                        v[c][n1] = fused::load2(r_x, w < a.C ? (rb + w) * 8 : fused::kOob, 0);
+                        v[c][n1] = fused::load2_stream(r_x, w < a.C ? (rb + w) * 8 : fused::kOob, 0);
                     }
                 }
                 RegFft<M1>::run(v[c], false);
@@ -276,10 +285,10 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
 #pragma unroll
                 for (int n1 = 0; n1 < M1; n1++) {
                     const float2 x = cmul(v[c][n1], fused::load2(r_h, n2[c] * 8, M2 * n1 * 8));
-                    fused::store2(x, r_o, n2[c] * 8, M2 * n1 * 8);
-                    fused::store2(make_float2(x.x, -x.y), r_m, n1 < n1c ? vm0 : fused::kOob,
+                    fused::store2_stream(x, r_o, n2[c] * 8, M2 * n1 * 8);
+                    fused::store2_stream(make_float2(x.x, -x.y), r_m, n1 < n1c ? vm0 : fused::kOob,
                                   M2 * (M1 - 1 - n1) * 8);
-                    if (n1 == 0 && n2[c] == 0) fused::store2(make_float2(x.x, -x.y), r_m, dm * 8, 0);
+                    if (n1 == 0 && n2[c] == 0) fused::store2_stream(make_float2(x.x, -x.y), r_m, dm * 8, 0);
                 }
             }
         }
@@ -304,9 +313,9 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
                 const int w = n2[0] + M2 * n1;
                 const bool in = w < a.C;
                 const float2 x = cmul(v[0][n1], fused::load2(r_h, n2[0] * 8, M2 * n1 * 8));
-                fused::store2(x, r_o, w < a.V ? (ob + w) * 8 : fused::kOob, 0);
+                fused::store2_stream(x, r_o, w < a.V ? (ob + w) * 8 : fused::kOob, 0);
                 const int vm = w == 0 ? 0 : a.C - w;
-                fused::store2(make_float2(x.x, -x.y), r_m, in && mirror && vm < a.V ? (mb + vm) * 8 : fused::kOob, 0);
+                fused::store2_stream(make_float2(x.x, -x.y), r_m, in && mirror && vm < a.V ? (mb + vm) * 8 : fused::kOob, 0);
             }
         }
     }

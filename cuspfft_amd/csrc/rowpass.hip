@@ -38,23 +38,22 @@ namespace spfft {
 
 namespace {
 
-constexpr int kTileStride = 65;
 constexpr int kColThreads = 512;
 constexpr int kColWaves = kColThreads / 64;
 
 // ---------------------------------------------------------------------------
-// k_colsched: for every 64-column tile of k_colpass, shares its columns out to
+// k_colsched: for every TC-column tile of k_colpass, shares its columns out to
 // the kColWaves waves by longest-processing-time first: power-of-two classes of
 // cost (nonzeros + 1) from the largest down, each column to the least-loaded
 // wave.  One wave per tile; lane = column.  Output: the tile's columns grouped
-// by wave, sched[tile][0 .. 64), and the groups' offsets off[tile][0 .. kColWaves].
+// by wave, sched[tile][0 .. TC), and the groups' offsets off[tile][0 .. kColWaves].
 // Columns past C are not listed.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_colsched(const int* __restrict__ colstart, int C,
+__global__ __launch_bounds__(64) void k_colsched(const int* __restrict__ colstart, int C, int TC,
                                                  int* __restrict__ sched, int* __restrict__ off) {
     const int lane = threadIdx.x;
-    const int c = blockIdx.x * 64 + lane;
-    const int cost = c < C ? colstart[c + 1] - colstart[c] + 1 : 0;
+    const int c = blockIdx.x * TC + lane;
+    const int cost = lane < TC && c < C ? colstart[c + 1] - colstart[c] + 1 : 0;
     const int cls = cost > 0 ? 31 - __clz(cost) : -1;
     int load[kColWaves] = {}, cnt[kColWaves] = {};
     int mine_w = 0, mine_k = 0;  // this lane's column: wave and position in its group
@@ -81,7 +80,7 @@ __global__ __launch_bounds__(64) void k_colsched(const int* __restrict__ colstar
     int sw = 0;
 #pragma unroll
     for (int k = 0; k < kColWaves; k++) sw = mine_w == k ? start[k] : sw;
-    if (cls >= 0) sched[blockIdx.x * 64 + sw + mine_k] = lane;
+    if (cls >= 0) sched[blockIdx.x * TC + sw + mine_k] = lane;
     if (lane <= kColWaves) {
         int v = 0;
 #pragma unroll
@@ -91,7 +90,7 @@ __global__ __launch_bounds__(64) void k_colsched(const int* __restrict__ colstar
 }
 
 // ---------------------------------------------------------------------------
-// k_colpass: a tile of 64 columns x 64*U rows per workgroup (8 waves).
+// k_colpass: a tile of TC = 64/U columns x 64*U rows per workgroup (8 waves).
 // The tile's columns are shared out to the waves by k_colsched, once per
 // transform; a wave takes its columns one at a time, so the column is uniform over
 // the wave; lane l takes rows u0 + l + 64t (t < U).  A column's row indices are
@@ -106,15 +105,17 @@ __global__ __launch_bounds__(64) void k_colsched(const int* __restrict__ colstar
 //   w  = W_R^ph                 (LDS table: whole, or two-level hi[ph >> s] * lo[ph & mask])
 //   acc[0] += w, acc[t] += w * w64^t
 // The chirp h[c] is applied and the sums go through an LDS tile (re and im
-// planes, row stride 65: conflict-free both ways), so X0 is written in whole
-// 64-column row segments.
+// planes, odd row stride TC + 1: conflict-free both ways), so X0 is written in
+// whole TC-column row segments (512, 256 or 128 bytes).
 // ---------------------------------------------------------------------------
 template <bool PACKED, bool FULL, int U>
 __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
     extern __shared__ float2 smem[];
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int c0 = blockIdx.x * 64;
+    constexpr int TC = 64 / U;        // columns of the tile
+    constexpr int TS = TC + 1;        // tile row stride
+    const int c0 = blockIdx.x * TC;
     const int u0l = blockIdx.y * 64 * U;
     const uint32_t R = (uint32_t)a.R;
     const int nlo = 1 << a.hi_shift;
@@ -122,15 +123,15 @@ __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
     float2* s_lo = smem;  // FULL: the whole W_R table
     float2* s_hi = s_lo + nlo;
     float* t_re = reinterpret_cast<float*>(FULL ? s_lo + R : s_hi + a.n_hi);  // [64U][65]
-    float* t_im = t_re + 64 * U * kTileStride;                                 // [64U][65]
-    int* s_cs = reinterpret_cast<int*>(t_im + 64 * U * kTileStride);         // colstart[c0 .. c0+64]
+    float* t_im = t_re + 64 * U * TS;                                          // [64U][TS]
+    int* s_cs = reinterpret_cast<int*>(t_im + 64 * U * TS);                  // colstart[c0 .. c0+TC]
     if (FULL) {
         for (int i = threadIdx.x; i < (int)R; i += kColThreads) s_lo[i] = a.wr_full[i];
     } else {
         for (int i = threadIdx.x; i < nlo; i += kColThreads) s_lo[i] = a.wr_lo[i];
         for (int i = threadIdx.x; i < a.n_hi; i += kColThreads) s_hi[i] = a.wr_hi[i];
     }
-    for (int i = threadIdx.x; i <= 64; i += kColThreads) s_cs[i] = a.colstart[min(c0 + i, a.C)];
+    for (int i = threadIdx.x; i <= TC; i += kColThreads) s_cs[i] = a.colstart[min(c0 + i, a.C)];
     __syncthreads();
 
     const uint32_t mask = (uint32_t)nlo - 1u;
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
     };
     const uint32_t u0 = (uint32_t)(a.u_base + u0l);
     // this wave's columns (k_colsched): sched[blk][off[w] .. off[w+1])
-    const int* sched = a.col_sched + blockIdx.x * 64;
+    const int* sched = a.col_sched + blockIdx.x * TC;
     const int* off = a.col_sched_off + blockIdx.x * (kColWaves + 1);
     const int i0 = __builtin_amdgcn_readfirstlane(off[wid]), i1 = __builtin_amdgcn_readfirstlane(off[wid + 1]);
     for (int i = i0; i < i1; i++) {
@@ -195,18 +196,20 @@ __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
         }
 #pragma unroll
         for (int t = 0; t < U; t++) {
-            t_re[(t * 64 + lane) * kTileStride + cc] = acc[t].x;
-            t_im[(t * 64 + lane) * kTileStride + cc] = acc[t].y;
+            t_re[(t * 64 + lane) * TS + cc] = acc[t].x;
+            t_im[(t * 64 + lane) * TS + cc] = acc[t].y;
         }
     }
     __syncthreads();
-    // whole 64-column row segments of X0 (ldx is a multiple of 64)
-    for (int e = threadIdx.x; e < 64 * 64 * U; e += kColThreads) {
-        const int row = e >> 6, col = e & 63;
+    // whole TC-column row segments of X0 (ldx is a multiple of 64)
+    for (int e = threadIdx.x; e < 64 * 64; e += kColThreads) {
+        const int row = e / TC, col = e % TC;
         const int ul = u0l + row;
-        if (ul < a.tile_rows)
-            a.X0[(size_t)ul * a.ldx + c0 + col] =
-                make_float2(t_re[row * kTileStride + col], t_im[row * kTileStride + col]);
+        if (ul < a.tile_rows) {  // X0 is read once, by the row pass: non-temporal
+            typedef float f32x2 __attribute__((ext_vector_type(2)));
+            const f32x2 x = {t_re[row * TS + col], t_im[row * TS + col]};
+            __builtin_nontemporal_store(x, reinterpret_cast<f32x2*>(&a.X0[(size_t)ul * a.ldx + c0 + col]));
+        }
     }
 }
 
@@ -223,20 +226,24 @@ bool rowpass_supports_m1(int m1) {
 
 size_t colpass_lds_bytes(const RowPassArgs& a) {
     const int U = a.colpass_rows > 0 ? a.colpass_rows : 1;
-    const size_t tile = (size_t)2 * 64 * U * kTileStride * sizeof(float) + 66 * sizeof(int);
+    const int TC = 64 / U;
+    const size_t tile = (size_t)2 * 64 * U * (TC + 1) * sizeof(float) + (TC + 2) * sizeof(int);
     if (a.wr_full) return tile + (size_t)a.R * sizeof(float2);
     return tile + ((size_t)(1 << a.hi_shift) + a.n_hi) * sizeof(float2);
 }
 
 size_t colsched_elems(int cols, size_t* off_elems) {
-    const size_t tiles = ((size_t)cols + 63) / 64;
+    // sized for the narrowest tile (16 columns, U = 4)
+    const size_t tiles = ((size_t)cols + 15) / 16;
     *off_elems = tiles * (kColWaves + 1);
-    return tiles * 64;
+    return tiles * 16;
 }
 
-hipError_t launch_colsched(const int* colstart, int cols, int* sched, int* off, hipStream_t s) {
+hipError_t launch_colsched(const int* colstart, int cols, int rows_per_lane, int* sched, int* off,
+                           hipStream_t s) {
     if (cols <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_colsched, dim3((cols + 63) / 64), dim3(64), 0, s, colstart, cols, sched, off);
+    const int TC = 64 / rows_per_lane;
+    hipLaunchKernelGGL(k_colsched, dim3((cols + TC - 1) / TC), dim3(64), 0, s, colstart, cols, TC, sched, off);
     return hipGetLastError();
 }
 
@@ -267,7 +274,7 @@ hipError_t colpass_prepare(size_t lds_limit) {
 template <bool PACKED, bool FULL>
 static void launch_colpass_u(const RowPassArgs& a, hipStream_t s) {
     const int U = a.colpass_rows;
-    const dim3 gp((a.C + 63) / 64, (a.tile_rows + 64 * U - 1) / (64 * U));
+    const dim3 gp((a.C + 64 / U - 1) / (64 / U), (a.tile_rows + 64 * U - 1) / (64 * U));
     const size_t lds = colpass_lds_bytes(a);
     switch (U) {
         case 1: hipLaunchKernelGGL((k_colpass<PACKED, FULL, 1>), gp, dim3(kColThreads), lds, s, a); break;

@@ -41,7 +41,8 @@ size_t colpass_lds_bytes(const RowPassArgs& a);   // needs a.colpass_rows and a.
 // k_colsched: the per-tile column schedule of k_colpass (once per CSC build);
 // sizes in ints of sched (returned) and of off (*off_elems)
 size_t colsched_elems(int cols, size_t* off_elems);
-hipError_t launch_colsched(const int* colstart, int cols, int* sched, int* off, hipStream_t s);
+hipError_t launch_colsched(const int* colstart, int cols, int rows_per_lane, int* sched, int* off,
+                           hipStream_t s);
 hipError_t colpass_prepare(size_t lds_limit);      // once per process, before launches
 size_t conv_lds_bytes(const RowPassArgs& a);
 // Fused row kernel (k_rowfused) shapes: M1, LDS batches, rows per workgroup,

@@ -323,7 +323,15 @@ spfft_status do_csc_build(spfft_plan_s& p, const int* d_row, const int* d_col, i
         SPFFT_HIP(hipEventRecord(ev[0], s), "hipEventRecord");
     }
     SPFFT_HIP(spfft::launch_csc_build(a, s), "csc build launch");
-    SPFFT_HIP(spfft::launch_colsched(a.colstart, p.C, p.col_sched.as<int>(), p.col_sched_off.as<int>(), s),
+    // column-pass rows per lane (the option, or from the average column degree)
+    p.colpass_rows = p.base.colpass_rows;
+    if (p.colpass_rows == 0) {
+        // four rows per lane (16-column tiles): the extra rows cost one complex
+        // product per nonzero instead of a phase reduction and a table lookup
+        p.colpass_rows = 4;
+    }
+    SPFFT_HIP(spfft::launch_colsched(a.colstart, p.C, p.colpass_rows, p.col_sched.as<int>(),
+                                     p.col_sched_off.as<int>(), s),
               "column schedule launch");
     if (ev) SPFFT_HIP(hipEventRecord(ev[1], s), "hipEventRecord");
     p.last_nnz = nnz;
@@ -338,13 +346,7 @@ spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t
     if (pitch < p.V) return fail(SPFFT_ERR_INVALID_ARG, "output pitch smaller than cols/2+1");
     RowPassArgs a = p.base;
     a.P = (int)pitch;
-    if (a.colpass_rows == 0) {
-        // two rows per lane: the second row costs one complex product per nonzero
-        // instead of a phase reduction and a table lookup, which pays off on
-        // long columns; short ones keep twice as many workgroups busy with one
-        const double avg = (double)p.last_nnz / std::max(p.C, 1);
-        a.colpass_rows = avg >= 12.0 ? 2 : 1;
-    }
+    a.colpass_rows = p.colpass_rows;  // resolved with the column schedule at the CSC build
     // the whole W_R table only while it fits in LDS next to the row tile
     if (a.wr_full && spfft::colpass_lds_bytes(a) > g_lds_limit) a.wr_full = nullptr;
     a.colstart = p.colstart.as<int>();

@@ -70,6 +70,7 @@ struct spfft_plan_s {
     bool have_csc = false;
     DevBuf counts, colstart, sparse_cols, col_ptr, totals, err, scan_tmp, hist, hist_off;
     DevBuf col_sched, col_sched_off;  // k_colpass column schedule
+    int colpass_rows = 1;             // k_colpass rows per lane of the last CSC build
     DevBuf keys0, keys1, vals0, vals1, row_idx;
     size_t table_bytes = 0;
     // optional per-kernel event timing (spfft_profile_*)

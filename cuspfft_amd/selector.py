@@ -21,12 +21,15 @@ from __future__ import annotations
 import ctypes
 import json
 import math
+import os
 
 import numpy as np
 
 from . import _check, _vp, describe_plan, load
 
 N_FEATURES = 28
+# the network trained on the autotuner timings of 2000 seeded matrices (train_selector)
+DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "selector.json")
 
 
 def _exponents(M: int):
@@ -64,12 +67,20 @@ def features(rec: dict, opts: dict, info: dict | None = None) -> np.ndarray:
     return np.asarray(f, np.float32)
 
 
+def trial_opts(t: dict) -> dict:
+    """The options of a timed trial with its Bluestein length pinned to the one it ran."""
+    o = dict(t["opts"])
+    if t.get("fft_len"):
+        o["fft_len"] = t["fft_len"]
+    return o
+
+
 def dataset(records: list[dict]):
     """(X, y, groups): one row per timed trial; y = log(ms); groups[i] = record index."""
     X, y, g = [], [], []
     for i, rec in enumerate(records):
         for t in rec["trials"]:
-            X.append(features(rec, t["opts"]))
+            X.append(features(rec, trial_opts(t)))
             y.append(math.log(max(t["ms"], 1e-4)))
             g.append(i)
     return np.asarray(X, np.float32), np.asarray(y, np.float32), np.asarray(g, np.int64)
@@ -189,6 +200,22 @@ class Selector:
         return cls(mlp, np.asarray(d["mean"], np.float32), np.asarray(d["std"], np.float32))
 
 
+_default = None
+
+
+def choose_plan(m, max_m1: int = 2) -> dict:
+    """Plan options for matrix m (COOMatrix) chosen by the shipped network among
+    the autotuner's candidates (cuspfft_amd.autotune.candidates), for Plan(**opts)."""
+    global _default
+    from . import autotune
+    if _default is None:
+        _default = Selector.load(DEFAULT_MODEL)
+    rec = autotune.features_of(m)
+    cands = [dict(length_rule=3)] + autotune.candidates(m.rows, m.cols, max_m1)
+    cands += [dict(c, colpass_rows=r) for c in cands[1:] for r in autotune.COLPASS_ROWS]
+    return _default.choose(rec, cands)
+
+
 def regret(sel: Selector, records: list[dict]) -> dict:
     """Per matrix: measured time of the network's choice among the timed trials
     over the best measured trial (the exhaustive search), and the same ratio for
@@ -197,7 +224,7 @@ def regret(sel: Selector, records: list[dict]) -> dict:
     for rec in records:
         trials = rec["trials"]
         best = min(t["ms"] for t in trials)
-        X = np.stack([features(rec, t["opts"]) for t in trials])
+        X = np.stack([features(rec, trial_opts(t)) for t in trials])
         pick = trials[int(np.argmin(sel.predict(X)))]
         ratios.append(pick["ms"] / best)
         dflt = [t for t in trials if t.get("default")]

@@ -79,8 +79,14 @@ def main(argv=None):
     fit = selector.regret(sel, train)
 
     hmats, htags = autotune.training_matrices(args.heldout, seed0=900000)
-    held = tune_set(hmats, htags, os.path.join(args.dir, "heldout.jsonl"), args.row_limit, args.reps)
+    held = tune_set(hmats, htags, os.path.join(args.dir, "heldout.jsonl"), args.row_limit, args.reps,
+                    deadline)
     del hmats
+    if len(held) < args.heldout:
+        log(f"[selector] tuning budget spent with {len(held)}/{args.heldout} held-out matrices; "
+            "run again to resume")
+        print(json.dumps({"tuned": len(train), "heldout_tuned": len(held)}), flush=True)
+        return None
     ho = selector.regret(sel, held)
     report = {"train_matrices": len(train), "train_rows": int(sum(len(r["trials"]) for r in train)),
               "train_s": round(t_train, 1), "train_loss": sel.train_loss, "fit": fit, "heldout": ho}

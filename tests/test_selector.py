@@ -109,3 +109,26 @@ def test_selector_beats_default_plan_on_heldout():
     print(r)
     assert r["mean_regret"] <= r["default_mean_regret"] + 1e-9
     assert r["selector_speedup_over_default"] >= 1.0
+
+
+@pytest.mark.gpu
+def test_shipped_selector_plan_runs(oracle_mod):
+    """The shipped network picks a plan for the c1 and c2 shapes that computes the
+    transform within the plain FP32 bar."""
+    import torch
+    import cuspfft_amd as sp
+    from cuspfft_amd import selector
+    for cfg in ("c1", "c2"):
+        m = sp.config_matrix(cfg)
+        opts = selector.choose_plan(m)
+        plan = sp.Plan(m.rows, m.cols, u_tile=m.rows, **opts)
+        out = plan.alloc_output()
+        plan.execute(torch.as_tensor(m.row_idx).cuda(), torch.as_tensor(m.col_idx).cuda(), out)
+        torch.cuda.synchronize()
+        plan.check_input()
+        rows = np.array([0, 1, m.rows // 2, m.rows - 1])
+        ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, u_list=rows, nthreads=8)
+        got = out[torch.as_tensor(rows).cuda(), : m.cols // 2 + 1].cpu().numpy()
+        tol = 1e-5 if opts.get("precision") == sp.PREC_BF16X3 else 9e-6
+        print(cfg, opts, oracle_mod.rel_l2(got, ref))
+        assert oracle_mod.rel_l2(got, ref) <= tol
