@@ -1,0 +1,87 @@
+// fft1_impl.h -- k_fft1 / k_ifft1, the length-M1 sub-FFTs of the four-step
+// Bluestein row pass (see rowpass.hip for the pipeline); each rowpass_fft1_*.hip
+// instantiates a group of M1 lengths.
+//
+// One (u, n2) column of the four-step matrix per thread; block = 256 threads =
+// 64 consecutive n2 x 4 rows u, so a wave has one row.  Rows are buffers (see
+// buffer_io.h): the per-lane part of an offset is n2, the strided part M2 n1 or
+// M2 k1 a scalar offset, and the bounds of a row do the zero padding of x
+// (n >= C), drop the writes of v > C/2 and of lanes past M2 (offset kOob).
+#pragma once
+#include "buffer_io.h"
+#include "rowpass_internal.h"
+
+namespace spfft {
+
+namespace {
+
+template <int M1>
+__global__ __launch_bounds__(kThreads) void k_fft1(RowPassArgs a) {
+    const int n2 = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int ul = blockIdx.y * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (ul >= a.tile_rows) return;  // whole wave
+    const int M2 = a.M2;
+    const int vo = n2 < M2 ? n2 * 8 : bio::kOob;
+    const bio::rsrc_t r_x = bio::make_rsrc(a.X0 + (size_t)ul * a.ldx, (uint32_t)a.C * 8u);
+    const bio::rsrc_t r_t = bio::make_rsrc(a.T + (size_t)ul * a.M, (uint32_t)a.M * 8u);
+    const bio::rsrc_t r_w = bio::make_rsrc(a.twM, (uint32_t)a.M * 8u);
+    float2 v[M1];
+#pragma unroll
+    for (int n1 = 0; n1 < M1; n1++) v[n1] = bio::load2_stream(r_x, vo, M2 * n1 * 8);
+    RegFft<M1>::run(v, false);
+    bio::store2_stream(v[0], r_t, vo, 0);
+#pragma unroll
+    for (int k1 = 1; k1 < M1; k1++)
+        bio::store2_stream(cmul(v[k1], bio::load2(r_w, vo, M2 * k1 * 8)), r_t, vo, M2 * k1 * 8);
+}
+
+template <int M1>
+__global__ __launch_bounds__(kThreads) void k_ifft1(RowPassArgs a) {
+    const int n2 = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int ul = blockIdx.y * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (ul >= a.tile_rows) return;  // whole wave
+    const int M2 = a.M2;
+    const bool lane_ok = n2 < M2;
+    const int vo = lane_ok ? n2 * 8 : bio::kOob;
+    const bio::rsrc_t r_t = bio::make_rsrc(a.T + (size_t)ul * a.M, (uint32_t)a.M * 8u);
+    const bio::rsrc_t r_w = bio::make_rsrc(a.twM, (uint32_t)a.M * 8u);
+    float2 v[M1];
+    v[0] = bio::load2_stream(r_t, vo, 0);
+#pragma unroll
+    for (int k1 = 1; k1 < M1; k1++)
+        v[k1] = cmulc(bio::load2_stream(r_t, vo, M2 * k1 * 8), bio::load2(r_w, vo, M2 * k1 * 8));
+    RegFft<M1>::run(v, true);
+    // X[u][w] = h[w] y[w] for every w < C.  The input is real, so
+    // X[R-u][v] = conj(X[u][(C - v) mod C]): row u also yields its mirror row.
+    // Row u: a buffer of C/2 + 1 entries.  Mirror row: X[R-u][C - w] at offset
+    // M - 1 - w = M2 (M1 - 1 - n1) + (M2 - 1 - n2) from a base M - 1 - C entries
+    // before it, a buffer ending at entry C/2 (writes of C - w > C/2 dropped);
+    // w >= C takes kOob, w = 0 is written apart.
+    const int u = a.u_base + ul;
+    const bool mirror = u > 0 && a.R - u > u;
+    const int dm = a.M - 1 - a.C;
+    const bio::rsrc_t r_h = bio::make_rsrc(a.h, (uint32_t)a.C * 8u);
+    const bio::rsrc_t r_o = bio::make_rsrc(a.out + (size_t)u * a.P, (uint32_t)a.V * 8u);
+    const bio::rsrc_t r_m =
+        bio::make_rsrc(a.out + (size_t)(mirror ? a.R - u : u) * a.P - dm, mirror ? (uint32_t)(dm + a.V) * 8u : 0u);
+    const bio::rsrc_t r_m0 = bio::make_rsrc(a.out + (size_t)(mirror ? a.R - u : u) * a.P, mirror ? 8u : 0u);
+    const int n1c = lane_ok ? (a.C - n2 + M2 - 1) / M2 : 0;  // columns w = n2 + M2 n1 < C
+    const int vm0 = (M2 - 1 - n2) * 8;
+#pragma unroll
+    for (int n1 = 0; n1 < M1; n1++) {
+        const float2 x = cmul(v[n1], bio::load2(r_h, vo, M2 * n1 * 8));
+        bio::store2_stream(x, r_o, vo, M2 * n1 * 8);
+        bio::store2_stream(make_float2(x.x, -x.y), r_m, n1 < n1c ? vm0 : bio::kOob, M2 * (M1 - 1 - n1) * 8);
+        if (n1 == 0 && n2 == 0) bio::store2_stream(make_float2(x.x, -x.y), r_m0, 0, 0);
+    }
+}
+
+}  // namespace
+
+}  // namespace spfft
+
+#define SPFFT_FFT1_LAUNCH_CASE(m)                                                    \
+    case m:                                                                          \
+        if (inverse) hipLaunchKernelGGL((k_ifft1<m>), grid, dim3(kThreads), 0, s, a); \
+        else hipLaunchKernelGGL((k_fft1<m>), grid, dim3(kThreads), 0, s, a);         \
+        break;

@@ -28,49 +28,11 @@
 // accesses would keep the whole column live in registers.  RPW > 1 puts RPW
 // short rows in one workgroup (one descriptor for all, explicit bounds).
 #pragma once
+#include "buffer_io.h"
 #include "rowpass_internal.h"
 
 namespace spfft {
 
-namespace fused {
-
-using rsrc_t = __amdgpu_buffer_rsrc_t;
-// raw buffer: stride 0, num_records in bytes; word 3 = 32-bit data format (gfx9)
-constexpr int kRsrcWord3 = 0x00020000;
-
-__device__ __forceinline__ rsrc_t make_rsrc(const void* base, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, kRsrcWord3);
-}
-// An offset at or past num_records makes a load return 0 and drops a store;
-// kOob is past every buffer made here (num_records <= 2^31 - 256).
-constexpr int kOob = 0x7fffff00;
-
-// The LLVM buffer intrinsics, bound by name: this compiler lowers the clang
-// builtins __builtin_amdgcn_raw_buffer_load_b64 / _b128 to a single 32-bit load.
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-__device__ f32x2 raw_load_v2f32(rsrc_t r, int voff, int soff, int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f32");
-__device__ void raw_store_v2f32(f32x2 v, rsrc_t r, int voff, int soff, int aux) __asm(
-    "llvm.amdgcn.raw.ptr.buffer.store.v2f32");
-
-// Cache policy of the accesses made once per transform (the pass-1 row read,
-// the output writes): non-temporal (bit 1 of the aux operand on gfx950), so they
-// do not evict the tables every row reuses from L2.
-constexpr int kStream = 2;
-
-__device__ __forceinline__ float2 load2(rsrc_t r, int voff, int soff) {
-    const f32x2 d = raw_load_v2f32(r, voff, soff, 0);
-    return make_float2(d[0], d[1]);
-}
-__device__ __forceinline__ float2 load2_stream(rsrc_t r, int voff, int soff) {
-    const f32x2 d = raw_load_v2f32(r, voff, soff, kStream);
-    return make_float2(d[0], d[1]);
-}
-__device__ __forceinline__ void store2_stream(float2 x, rsrc_t r, int voff, int soff) {
-    const f32x2 d = {x.x, x.y};
-    raw_store_v2f32(d, r, voff, soff, kStream);
-}
-
-}  // namespace fused
 
 // One in-place Stockham stage of radix R over the G rows (length L) in LDS,
 // NT threads.  Row g belongs to k1 = kbase + g % KB.  Reads of the whole stage
@@ -93,7 +55,7 @@ __device__ __forceinline__ float2 wm_pow(const float2* s_wm, int e) {
 }
 
 template <int R, int L, int NS, int MODE, bool TWIN, bool TWOUT, int G, int KB, int NT>
-__device__ __forceinline__ void fused_stage(fused::rsrc_t r_bh, float2* buf, const float2* s_tw,
+__device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const float2* s_tw,
                                             const float2* s_wm, int kbase) {
     constexpr int nb = L / R;
     constexpr int total = G * nb;
@@ -143,7 +105,7 @@ __device__ __forceinline__ void fused_stage(fused::rsrc_t r_bh, float2* buf, con
                 // with no earlier stages: outputs j R + r
 #pragma unroll
                 for (int r = 0; r < R; r++)
-                    v[b][r] = cmul(v[b][r], fused::load2(r_bh, (rowoff + base) * 8, (kbase * L + r * NS) * 8));
+                    v[b][r] = cmul(v[b][r], bio::load2(r_bh, (rowoff + base) * 8, (kbase * L + r * NS) * 8));
                 RegFft<R>::run(v[b], true);
                 base = j * R;
                 stride = 1;
@@ -164,7 +126,7 @@ __device__ __forceinline__ void fused_stage(fused::rsrc_t r_bh, float2* buf, con
 // stages Rn .. R1 (radices of 1 are no stage), the last forward and first
 // inverse stage merged (kPivot).
 template <int L, int G, int KB, int NT, int R1, int R2, int R3>
-__device__ __forceinline__ void fused_conv(fused::rsrc_t r_bh, float2* buf, const float2* s_tw,
+__device__ __forceinline__ void fused_conv(bio::rsrc_t r_bh, float2* buf, const float2* s_tw,
                                            const float2* s_wm, int kbase) {
     if constexpr (R3 > 1) {
         fused_stage<R1, L, 1, kFwd, true, false, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
@@ -207,7 +169,7 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
     }
     for (int i = t; i < M2; i += NT) s_tw[i] = a.tw2[i];
     for (int i = t; i < 256 + (a.M + 255) / 256; i += NT) s_wm[i] = a.wm2[i];
-    const fused::rsrc_t r_bh = fused::make_rsrc(a.bhat, (uint32_t)a.M * 8u);
+    const bio::rsrc_t r_bh = bio::make_rsrc(a.bhat, (uint32_t)a.M * 8u);
 
     // phase A
     float2 v[CPT][M1];
@@ -215,7 +177,7 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
         const int nrows = min(RPW, a.tile_rows - row0);
         // RPW == 1: the row as a buffer of C entries, so reads past it return 0 (the
         // zero padding) with the stride M2 n1 as the scalar offset
-        const fused::rsrc_t r_x = fused::make_rsrc(a.X0 + (size_t)row0 * a.ldx,
+        const bio::rsrc_t r_x = bio::make_rsrc(a.X0 + (size_t)row0 * a.ldx,
                                                    (uint32_t)(RPW == 1 ? a.C : nrows * a.ldx) * 8u);
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
@@ -224,10 +186,10 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
 #pragma unroll
                 for (int n1 = 0; n1 < M1; n1++) {
                     if constexpr (RPW == 1) {
-                        v[c][n1] = fused::load2_stream(r_x, n2[c] * 8, M2 * n1 * 8);
+                        v[c][n1] = bio::load2_stream(r_x, n2[c] * 8, M2 * n1 * 8);
                     } else {
                         const int w = n2[c] + M2 * n1;
-                        v[c][n1] = fused::load2_stream(r_x, w < a.C ? (rb + w) * 8 : fused::kOob, 0);
+                        v[c][n1] = bio::load2_stream(r_x, w < a.C ? (rb + w) * 8 : bio::kOob, 0);
                     }
                 }
                 RegFft<M1>::run(v[c], false);
@@ -259,7 +221,7 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
     });
 
     // phase C
-    const fused::rsrc_t r_h = fused::make_rsrc(a.h, (uint32_t)a.C * 8u);  // reads past C return 0
+    const bio::rsrc_t r_h = bio::make_rsrc(a.h, (uint32_t)a.C * 8u);  // reads past C return 0
     const int u0 = a.u_base + row0;
     if constexpr (RPW == 1) {
         // Row u as a buffer of C/2 + 1 entries: writes of v > C/2 are dropped.  Mirror
@@ -270,9 +232,10 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
         const int u = u0;
         const bool mirror = u > 0 && a.R - u > u;
         const int dm = a.M - 1 - a.C;
-        const fused::rsrc_t r_o = fused::make_rsrc(a.out + (size_t)u * a.P, (uint32_t)a.V * 8u);
-        const fused::rsrc_t r_m = fused::make_rsrc(a.out + (size_t)(a.R - u) * a.P - dm,
+        const bio::rsrc_t r_o = bio::make_rsrc(a.out + (size_t)u * a.P, (uint32_t)a.V * 8u);
+        const bio::rsrc_t r_m = bio::make_rsrc(a.out + (size_t)(a.R - u) * a.P - dm,
                                                    mirror ? (uint32_t)(dm + a.V) * 8u : 0u);
+        const bio::rsrc_t r_m0 = bio::make_rsrc(a.out + (size_t)(a.R - u) * a.P, mirror ? 8u : 0u);
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
             if (live[c]) {
@@ -284,11 +247,11 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
                 const int vm0 = (M2 - 1 - n2[c]) * 8;
 #pragma unroll
                 for (int n1 = 0; n1 < M1; n1++) {
-                    const float2 x = cmul(v[c][n1], fused::load2(r_h, n2[c] * 8, M2 * n1 * 8));
-                    fused::store2_stream(x, r_o, n2[c] * 8, M2 * n1 * 8);
-                    fused::store2_stream(make_float2(x.x, -x.y), r_m, n1 < n1c ? vm0 : fused::kOob,
+                    const float2 x = cmul(v[c][n1], bio::load2(r_h, n2[c] * 8, M2 * n1 * 8));
+                    bio::store2_stream(x, r_o, n2[c] * 8, M2 * n1 * 8);
+                    bio::store2_stream(make_float2(x.x, -x.y), r_m, n1 < n1c ? vm0 : bio::kOob,
                                   M2 * (M1 - 1 - n1) * 8);
-                    if (n1 == 0 && n2[c] == 0) fused::store2_stream(make_float2(x.x, -x.y), r_m, dm * 8, 0);
+                    if (n1 == 0 && n2[c] == 0) bio::store2_stream(make_float2(x.x, -x.y), r_m0, 0, 0);
                 }
             }
         }
@@ -297,8 +260,8 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
         // their mirror rows R - u (the lowest is mlo); explicit bounds
         const int nrows = min(RPW, a.tile_rows - row0);
         const int mlo = a.R - (u0 + nrows - 1);
-        const fused::rsrc_t r_o = fused::make_rsrc(a.out + (size_t)u0 * a.P, (uint32_t)(nrows * a.P) * 8u);
-        const fused::rsrc_t r_m = fused::make_rsrc(a.out + (size_t)mlo * a.P, (uint32_t)(nrows * a.P) * 8u);
+        const bio::rsrc_t r_o = bio::make_rsrc(a.out + (size_t)u0 * a.P, (uint32_t)(nrows * a.P) * 8u);
+        const bio::rsrc_t r_m = bio::make_rsrc(a.out + (size_t)mlo * a.P, (uint32_t)(nrows * a.P) * 8u);
         if (live[0]) {
             const float2* col = buf + g[0] * KB * M2 + n2[0];
 #pragma unroll
@@ -312,10 +275,10 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
             for (int n1 = 0; n1 < M1; n1++) {
                 const int w = n2[0] + M2 * n1;
                 const bool in = w < a.C;
-                const float2 x = cmul(v[0][n1], fused::load2(r_h, n2[0] * 8, M2 * n1 * 8));
-                fused::store2_stream(x, r_o, w < a.V ? (ob + w) * 8 : fused::kOob, 0);
+                const float2 x = cmul(v[0][n1], bio::load2(r_h, n2[0] * 8, M2 * n1 * 8));
+                bio::store2_stream(x, r_o, w < a.V ? (ob + w) * 8 : bio::kOob, 0);
                 const int vm = w == 0 ? 0 : a.C - w;
-                fused::store2_stream(make_float2(x.x, -x.y), r_m, in && mirror && vm < a.V ? (mb + vm) * 8 : fused::kOob, 0);
+                bio::store2_stream(make_float2(x.x, -x.y), r_m, in && mirror && vm < a.V ? (mb + vm) * 8 : bio::kOob, 0);
             }
         }
     }
