@@ -2,6 +2,11 @@
 // Bluestein row pass (see rowpass.hip for the pipeline); each rowpass_fft1_*.hip
 // instantiates a group of M1 lengths.
 //
+// M >= 2C - 1, so the entries w = n2 + M2 n1 < C have n1 < NZ = (M1 + 1) / 2:
+// only those inputs of k_fft1 can be non-zero and only those outputs of
+// k_ifft1 are written; the codelets are pruned to them at compile time (the
+// files are built without signed zeros, so x + 0 folds to x).
+//
 // One (u, n2) column of the four-step matrix per thread; block = 256 threads =
 // 64 consecutive n2 x 4 rows u, so a wave has one row.  Rows are buffers (see
 // buffer_io.h): the per-lane part of an offset is n2, the strided part M2 n1 or
@@ -25,9 +30,11 @@ __global__ __launch_bounds__(kThreads) void k_fft1(RowPassArgs a) {
     const bio::rsrc_t r_x = bio::make_rsrc(a.X0 + (size_t)ul * a.ldx, (uint32_t)a.C * 8u);
     const bio::rsrc_t r_t = bio::make_rsrc(a.T + (size_t)ul * a.M, (uint32_t)a.M * 8u);
     const bio::rsrc_t r_w = bio::make_rsrc(a.twM, (uint32_t)a.M * 8u);
+    constexpr int NZ = (M1 + 1) / 2;
     float2 v[M1];
 #pragma unroll
-    for (int n1 = 0; n1 < M1; n1++) v[n1] = bio::load2_stream(r_x, vo, M2 * n1 * 8);
+    for (int n1 = 0; n1 < M1; n1++)
+        v[n1] = n1 < NZ ? bio::load2_stream(r_x, vo, M2 * n1 * 8) : make_float2(0.f, 0.f);
     RegFft<M1>::run(v, false);
     bio::store2_stream(v[0], r_t, vo, 0);
 #pragma unroll
@@ -68,7 +75,7 @@ __global__ __launch_bounds__(kThreads) void k_ifft1(RowPassArgs a) {
     const int n1c = lane_ok ? (a.C - n2 + M2 - 1) / M2 : 0;  // columns w = n2 + M2 n1 < C
     const int vm0 = (M2 - 1 - n2) * 8;
 #pragma unroll
-    for (int n1 = 0; n1 < M1; n1++) {
+    for (int n1 = 0; n1 < (M1 + 1) / 2; n1++) {
         const float2 x = cmul(v[n1], bio::load2(r_h, vo, M2 * n1 * 8));
         bio::store2_stream(x, r_o, vo, M2 * n1 * 8);
         bio::store2_stream(make_float2(x.x, -x.y), r_m, n1 < n1c ? vm0 : bio::kOob, M2 * (M1 - 1 - n1) * 8);

@@ -36,7 +36,9 @@ namespace spfft {
 
 // One in-place Stockham stage of radix R over the G rows (length L) in LDS,
 // NT threads.  Row g belongs to k1 = kbase + g % KB.  Reads of the whole stage
-// finish at a barrier before any write; the stage ends with a barrier.
+// finish at a barrier before any write; the stage ends with a barrier.  (Doing
+// the arithmetic before that barrier measured no faster and needs more
+// registers.)
 //   MODE kFwd / kInv: a forward / inverse stage after stages with product NS.
 //   MODE kPivot: the last forward stage (NS = L / R), the product with B^, and
 //     the first inverse stage, which runs the inverse radices in reverse order
@@ -152,6 +154,11 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
     static_assert(M1 % NB == 0, "batches must divide M1");
     static_assert(RPW == 1 || CPT == 1, "several rows per workgroup: one column per thread");
     static_assert(R1 > 1 && (R2 > 1 || R3 == 1), "stage radices first");
+    // M >= 2C - 1, so the entries w = n2 + M2 n1 < C have n1 < NZ: only those
+    // inputs can be non-zero and only those outputs are written (the codelets
+    // are pruned to them at compile time; this file is built without signed
+    // zeros, so x + 0 folds to x)
+    constexpr int NZ = (M1 + 1) / 2;
     extern __shared__ float2 smem[];
     float2* s_tw = smem;                    // W_M2^i, i < M2
     float2* s_wm = s_tw + M2;               // W_M^e: 256 low + (M + 255) / 256 high entries
@@ -185,7 +192,9 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
                 const int rb = g[c] * a.ldx;
 #pragma unroll
                 for (int n1 = 0; n1 < M1; n1++) {
-                    if constexpr (RPW == 1) {
+                    if (n1 >= NZ) {
+                        v[c][n1] = make_float2(0.f, 0.f);
+                    } else if constexpr (RPW == 1) {
                         v[c][n1] = bio::load2_stream(r_x, n2[c] * 8, M2 * n1 * 8);
                     } else {
                         const int w = n2[c] + M2 * n1;
@@ -246,7 +255,7 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
                 const int n1c = (a.C - n2[c] + M2 - 1) / M2;  // columns w = n2 + M2 n1 < C
                 const int vm0 = (M2 - 1 - n2[c]) * 8;
 #pragma unroll
-                for (int n1 = 0; n1 < M1; n1++) {
+                for (int n1 = 0; n1 < NZ; n1++) {
                     const float2 x = cmul(v[c][n1], bio::load2(r_h, n2[c] * 8, M2 * n1 * 8));
                     bio::store2_stream(x, r_o, n2[c] * 8, M2 * n1 * 8);
                     bio::store2_stream(make_float2(x.x, -x.y), r_m, n1 < n1c ? vm0 : bio::kOob,
@@ -272,7 +281,7 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
             const int ob = g[0] * a.P;                          // row u in r_o
             const int mb = mirror ? (a.R - u - mlo) * a.P : 0;  // row R - u in r_m
 #pragma unroll
-            for (int n1 = 0; n1 < M1; n1++) {
+            for (int n1 = 0; n1 < NZ; n1++) {
                 const int w = n2[0] + M2 * n1;
                 const bool in = w < a.C;
                 const float2 x = cmul(v[0][n1], bio::load2(r_h, n2[0] * 8, M2 * n1 * 8));
