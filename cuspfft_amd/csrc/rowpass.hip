@@ -90,7 +90,7 @@ __global__ __launch_bounds__(64) void k_colsched(const int* __restrict__ colstar
 }
 
 // ---------------------------------------------------------------------------
-// k_colpass: a tile of TC = 64/U columns x 64*U rows per workgroup (8 waves).
+// k_colpass: a tile of TC columns x 64*U rows per workgroup (8 waves).
 // The tile's columns are shared out to the waves by k_colsched, once per
 // transform; a wave takes its columns one at a time, so the column is uniform over
 // the wave; lane l takes rows u0 + l + 64t (t < U).  A column's row indices are
@@ -106,14 +106,13 @@ __global__ __launch_bounds__(64) void k_colsched(const int* __restrict__ colstar
 //   acc[0] += w, acc[t] += w * w64^t
 // The chirp h[c] is applied and the sums go through an LDS tile (re and im
 // planes, odd row stride TC + 1: conflict-free both ways), so X0 is written in
-// whole TC-column row segments (512, 256 or 128 bytes).
+// whole TC-column row segments (TC * 8 bytes).
 // ---------------------------------------------------------------------------
-template <bool PACKED, bool FULL, int U>
+template <bool PACKED, bool FULL, int U, int TC>
 __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
     extern __shared__ float2 smem[];
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    constexpr int TC = 64 / U;        // columns of the tile
     constexpr int TS = TC + 1;        // tile row stride
     const int c0 = blockIdx.x * TC;
     const int u0l = blockIdx.y * 64 * U;
@@ -122,7 +121,7 @@ __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
     // the tables first, so that their offsets fit the LDS instructions' immediate field
     float2* s_lo = smem;  // FULL: the whole W_R table
     float2* s_hi = s_lo + nlo;
-    float* t_re = reinterpret_cast<float*>(FULL ? s_lo + R : s_hi + a.n_hi);  // [64U][65]
+    float* t_re = reinterpret_cast<float*>(FULL ? s_lo + R : s_hi + a.n_hi);  // [64U][TS]
     float* t_im = t_re + 64 * U * TS;                                          // [64U][TS]
     int* s_cs = reinterpret_cast<int*>(t_im + 64 * U * TS);                  // colstart[c0 .. c0+TC]
     if (FULL) {
@@ -202,7 +201,7 @@ __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
     }
     __syncthreads();
     // whole TC-column row segments of X0 (ldx is a multiple of 64)
-    for (int e = threadIdx.x; e < 64 * 64; e += kColThreads) {
+    for (int e = threadIdx.x; e < 64 * U * TC; e += kColThreads) {
         const int row = e / TC, col = e % TC;
         const int ul = u0l + row;
         if (ul < a.tile_rows) {  // X0 is read once, by the row pass: non-temporal
@@ -224,45 +223,52 @@ bool rowpass_supports_m1(int m1) {
     }
 }
 
+int colpass_tile_cols(int rows_per_lane) {
+    switch (rows_per_lane) {
+        case 1: return 64;
+        case 2: return 32;
+        case 4: return 16;
+        default: return 8;  // 8 or 16 rows per lane
+    }
+}
+
 size_t colpass_lds_bytes(const RowPassArgs& a) {
     const int U = a.colpass_rows > 0 ? a.colpass_rows : 1;
-    const int TC = 64 / U;
+    const int TC = colpass_tile_cols(U);
     const size_t tile = (size_t)2 * 64 * U * (TC + 1) * sizeof(float) + (TC + 2) * sizeof(int);
     if (a.wr_full) return tile + (size_t)a.R * sizeof(float2);
     return tile + ((size_t)(1 << a.hi_shift) + a.n_hi) * sizeof(float2);
 }
 
 size_t colsched_elems(int cols, size_t* off_elems) {
-    // sized for the narrowest tile (16 columns, U = 4)
-    const size_t tiles = ((size_t)cols + 15) / 16;
+    // sized for the narrowest tile (8 columns)
+    const size_t tiles = ((size_t)cols + 7) / 8;
     *off_elems = tiles * (kColWaves + 1);
-    return tiles * 16;
+    return tiles * 8;
 }
 
 hipError_t launch_colsched(const int* colstart, int cols, int rows_per_lane, int* sched, int* off,
                            hipStream_t s) {
     if (cols <= 0) return hipSuccess;
-    const int TC = 64 / rows_per_lane;
+    const int TC = colpass_tile_cols(rows_per_lane);
     hipLaunchKernelGGL(k_colsched, dim3((cols + TC - 1) / TC), dim3(64), 0, s, colstart, cols, TC, sched, off);
     return hipGetLastError();
 }
 
+// The instantiated (rows per lane U, tile columns TC) pairs.
+#define SPFFT_COLPASS_SHAPES(X) X(1, 64) X(2, 32) X(4, 16) X(8, 8) X(16, 8)
+
 hipError_t colpass_prepare(size_t lds_limit) {
     // raise the dynamic LDS limit of every k_colpass instantiation (the tile is
-    // 33 KB per 64 rows; with U = 2 or 4 it passes 64 KB)
+    // 2 * 64U * (TC + 1) floats: past 64 KB for some shapes)
     const void* ks[] = {
-        reinterpret_cast<const void*>(&k_colpass<true, true, 1>),
-        reinterpret_cast<const void*>(&k_colpass<true, true, 2>),
-        reinterpret_cast<const void*>(&k_colpass<true, true, 4>),
-        reinterpret_cast<const void*>(&k_colpass<true, false, 1>),
-        reinterpret_cast<const void*>(&k_colpass<true, false, 2>),
-        reinterpret_cast<const void*>(&k_colpass<true, false, 4>),
-        reinterpret_cast<const void*>(&k_colpass<false, true, 1>),
-        reinterpret_cast<const void*>(&k_colpass<false, true, 2>),
-        reinterpret_cast<const void*>(&k_colpass<false, true, 4>),
-        reinterpret_cast<const void*>(&k_colpass<false, false, 1>),
-        reinterpret_cast<const void*>(&k_colpass<false, false, 2>),
-        reinterpret_cast<const void*>(&k_colpass<false, false, 4>),
+#define SPFFT_COLPASS_KERNELS(u, tc)                                         \
+    reinterpret_cast<const void*>(&k_colpass<true, true, u, tc>),           \
+        reinterpret_cast<const void*>(&k_colpass<true, false, u, tc>),      \
+        reinterpret_cast<const void*>(&k_colpass<false, true, u, tc>),      \
+        reinterpret_cast<const void*>(&k_colpass<false, false, u, tc>),
+        SPFFT_COLPASS_SHAPES(SPFFT_COLPASS_KERNELS)
+#undef SPFFT_COLPASS_KERNELS
     };
     for (const void* k : ks) {
         const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit);
@@ -274,13 +280,13 @@ hipError_t colpass_prepare(size_t lds_limit) {
 template <bool PACKED, bool FULL>
 static void launch_colpass_u(const RowPassArgs& a, hipStream_t s) {
     const int U = a.colpass_rows;
-    const dim3 gp((a.C + 64 / U - 1) / (64 / U), (a.tile_rows + 64 * U - 1) / (64 * U));
+    const int TC = colpass_tile_cols(U);
+    const dim3 gp((a.C + TC - 1) / TC, (a.tile_rows + 64 * U - 1) / (64 * U));
     const size_t lds = colpass_lds_bytes(a);
-    switch (U) {
-        case 1: hipLaunchKernelGGL((k_colpass<PACKED, FULL, 1>), gp, dim3(kColThreads), lds, s, a); break;
-        case 2: hipLaunchKernelGGL((k_colpass<PACKED, FULL, 2>), gp, dim3(kColThreads), lds, s, a); break;
-        default: hipLaunchKernelGGL((k_colpass<PACKED, FULL, 4>), gp, dim3(kColThreads), lds, s, a); break;
-    }
+#define SPFFT_COLPASS_LAUNCH(u, tc) \
+    if (U == u && TC == tc) hipLaunchKernelGGL((k_colpass<PACKED, FULL, u, tc>), gp, dim3(kColThreads), lds, s, a);
+    SPFFT_COLPASS_SHAPES(SPFFT_COLPASS_LAUNCH)
+#undef SPFFT_COLPASS_LAUNCH
 }
 
 static hipError_t launch_conv(const RowPassArgs& a, hipStream_t s) {

@@ -323,13 +323,12 @@ spfft_status do_csc_build(spfft_plan_s& p, const int* d_row, const int* d_col, i
         SPFFT_HIP(hipEventRecord(ev[0], s), "hipEventRecord");
     }
     SPFFT_HIP(spfft::launch_csc_build(a, s), "csc build launch");
-    // column-pass rows per lane (the option, or from the average column degree)
+    // column-pass rows per lane (the option, or from the number of rows): the
+    // extra rows cost one complex product per nonzero instead of a phase
+    // reduction and a table lookup; eight rows per lane (8-column tiles of 512
+    // rows) once the tile has at least four such row blocks, else four
     p.colpass_rows = p.base.colpass_rows;
-    if (p.colpass_rows == 0) {
-        // four rows per lane (16-column tiles): the extra rows cost one complex
-        // product per nonzero instead of a phase reduction and a table lookup
-        p.colpass_rows = 4;
-    }
+    if (p.colpass_rows == 0) p.colpass_rows = std::min(p.opts.u_tile, p.R / 2 + 1) >= 2048 ? 8 : 4;
     SPFFT_HIP(spfft::launch_colsched(a.colstart, p.C, p.colpass_rows, p.col_sched.as<int>(),
                                      p.col_sched_off.as<int>(), s),
               "column schedule launch");
@@ -553,8 +552,8 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     // just below 1/R: floor(x * inv_r) is floor(x / R) or one less for x < 2^28
     a.inv_r = (float)((1.0 / (double)rows) * (1.0 - 0x1p-20));
     a.precision = mfma ? 1 : 0;
-    if (o.colpass_rows < 0 || o.colpass_rows == 3 || o.colpass_rows > 4)
-        return fail(SPFFT_ERR_INVALID_ARG, "colpass_rows must be 0, 1, 2 or 4");
+    if (o.colpass_rows < 0 || o.colpass_rows > 16 || (o.colpass_rows & (o.colpass_rows - 1)))
+        return fail(SPFFT_ERR_INVALID_ARG, "colpass_rows must be 0, 1, 2, 4, 8 or 16");
     a.colpass_rows = o.colpass_rows ? o.colpass_rows : 0;  // 0: set from the average degree at CSC build
     if (o.colpass_table < 0 || o.colpass_table > 2)
         return fail(SPFFT_ERR_INVALID_ARG, "colpass_table must be 0, 1 or 2");

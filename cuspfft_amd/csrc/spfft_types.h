@@ -33,7 +33,7 @@ struct RowPassArgs {
     int ldx;                   // row stride of X0 (elements)
     int G2;                    // k_conv: (u, k1) rows per workgroup
     int conv_kind;             // 1: k_conv_fast<f2.radix[0], f2.radix[1]>, 0: k_conv_generic
-    int colpass_rows;          // k_colpass: rows per lane (1, 2 or 4; 64 * rows per workgroup)
+    int colpass_rows;          // k_colpass: rows per lane (1 .. 16; 64 * rows per workgroup)
     int hi_shift;              // two-level W_R table: W_R^p = hi[p >> s] * lo[p & (2^s - 1)]
     int n_hi;                  // entries in hi table
     float inv_r;               // (1 - 2^-20) / R: a phase quotient estimate that never exceeds

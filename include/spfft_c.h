@@ -51,7 +51,7 @@ typedef struct spfft_opts {
     int fft_len;     /* 0: from length_rule; else an explicit 7-smooth M >= 2*cols-1 */
     int m1;          /* 0: automatic; else the first factor of M = m1 * m2 (a 7-smooth m1 <= 64) */
     int conv_rows;   /* 0: automatic; else rows per workgroup of the convolution kernel (<= 64) */
-    int colpass_rows;  /* 0: automatic; else 1, 2 or 4 output rows per lane in the column pass */
+    int colpass_rows;  /* 0: automatic; else 1, 2, 4, 8 or 16 output rows per lane in the column pass */
     int colpass_table; /* 0: automatic; 1: two-level W_R table; 2: whole W_R table in LDS      */
     int row_limit;     /* 0: all rows.  Else only output rows u < row_limit (and their mirror
                           rows) are computed: a sampled run for timing (the autotuner) */

@@ -343,7 +343,7 @@ def test_fused_row_kernel(gpu, oracle_mod, M, rows, tile):
         assert oracle_mod.rel_l2(X, X4) <= 2 * REL_L2_PLAIN
 
 
-@pytest.mark.parametrize("rows_per_lane", [1, 2, 4])
+@pytest.mark.parametrize("rows_per_lane", [1, 2, 4, 8, 16])
 @pytest.mark.parametrize("table", [1, 2])
 @pytest.mark.parametrize("shape", [(2310, 2310, 16000, 128), (300, 4000, 9000, 100), (4095, 33, 2000, 4095)])
 def test_colpass_variants(gpu, oracle_mod, rows_per_lane, table, shape):
