@@ -20,6 +20,29 @@ namespace spfft {
 
 namespace {
 
+// W_M^{k1 n2} for k1 < M1 from A + B - 2 entries of the table twM[k1][n2] =
+// W_M^{k1 n2}: with k1 = A i + j (j < A), W_M^{k1 n2} = twM[A i][n2] twM[j][n2],
+// so a thread loads rows j < A and A i of its column and forms the rest with
+// one complex product each (instead of one table load per k1).
+template <int M1>
+struct TwSplit {
+    static constexpr int A = M1 >= 49 ? 8 : M1 >= 25 ? 6 : M1 >= 9 ? 4 : 2;
+    static constexpr int B = (M1 + A - 1) / A;
+    float2 lo[A], hi[B];
+    __device__ __forceinline__ void load(bio::rsrc_t r_w, int vo, int M2) {
+        lo[0] = hi[0] = make_float2(1.f, 0.f);
+#pragma unroll
+        for (int j = 1; j < A; j++) lo[j] = bio::load2(r_w, vo, M2 * j * 8);
+#pragma unroll
+        for (int i = 1; i < B; i++) hi[i] = bio::load2(r_w, vo, M2 * A * i * 8);
+    }
+    // W_M^{k1 n2}, k1 a compile-time index after unrolling
+    __device__ __forceinline__ float2 at(int k1) const {
+        const int i = k1 / A, j = k1 - (k1 / A) * A;
+        return i == 0 ? lo[j] : j == 0 ? hi[i] : cmul(hi[i], lo[j]);
+    }
+};
+
 template <int M1>
 __global__ __launch_bounds__(kThreads) void k_fft1(RowPassArgs a) {
     const int n2 = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -35,15 +58,19 @@ __global__ __launch_bounds__(kThreads) void k_fft1(RowPassArgs a) {
 #pragma unroll
     for (int n1 = 0; n1 < M1; n1++)
         v[n1] = n1 < NZ ? bio::load2_stream(r_x, vo, M2 * n1 * 8) : make_float2(0.f, 0.f);
+    TwSplit<M1> tw;
+    tw.load(r_w, vo, M2);
     RegFft<M1>::run(v, false);
     bio::store2_stream(v[0], r_t, vo, 0);
 #pragma unroll
-    for (int k1 = 1; k1 < M1; k1++)
-        bio::store2_stream(cmul(v[k1], bio::load2(r_w, vo, M2 * k1 * 8)), r_t, vo, M2 * k1 * 8);
+    for (int k1 = 1; k1 < M1; k1++) bio::store2_stream(cmul(v[k1], tw.at(k1)), r_t, vo, M2 * k1 * 8);
 }
 
+// At least two workgroups (two waves per SIMD) per CU: the length-64 column and
+// its loads otherwise take a few registers more and leave one wave per SIMD
+// to wait on HBM.
 template <int M1>
-__global__ __launch_bounds__(kThreads) void k_ifft1(RowPassArgs a) {
+__global__ __launch_bounds__(kThreads, 2) void k_ifft1(RowPassArgs a) {
     const int n2 = blockIdx.x * 64 + (threadIdx.x & 63);
     const int ul = blockIdx.y * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (ul >= a.tile_rows) return;  // whole wave
@@ -52,11 +79,13 @@ __global__ __launch_bounds__(kThreads) void k_ifft1(RowPassArgs a) {
     const int vo = lane_ok ? n2 * 8 : bio::kOob;
     const bio::rsrc_t r_t = bio::make_rsrc(a.T + (size_t)ul * a.M, (uint32_t)a.M * 8u);
     const bio::rsrc_t r_w = bio::make_rsrc(a.twM, (uint32_t)a.M * 8u);
+    TwSplit<M1> tw;
+    tw.load(r_w, vo, M2);
     float2 v[M1];
-    v[0] = bio::load2_stream(r_t, vo, 0);
 #pragma unroll
-    for (int k1 = 1; k1 < M1; k1++)
-        v[k1] = cmulc(bio::load2_stream(r_t, vo, M2 * k1 * 8), bio::load2(r_w, vo, M2 * k1 * 8));
+    for (int k1 = 0; k1 < M1; k1++) v[k1] = bio::load2_stream(r_t, vo, M2 * k1 * 8);
+#pragma unroll
+    for (int k1 = 1; k1 < M1; k1++) v[k1] = cmulc(v[k1], tw.at(k1));
     RegFft<M1>::run(v, true);
     // X[u][w] = h[w] y[w] for every w < C.  The input is real, so
     // X[R-u][v] = conj(X[u][(C - v) mod C]): row u also yields its mirror row.
