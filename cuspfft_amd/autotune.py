@@ -23,7 +23,7 @@ from . import Plan, SpfftError, describe_plan, fused_lengths, PREC_BF16X3, ROW_F
 # M1 lengths tried as alternatives to the planner's choice (register codelets
 # with a fast convolution kernel for M / M1).
 ALT_M1 = (16, 18, 24, 27, 32, 36, 40, 48, 54, 64, 20, 25, 28, 42, 45, 49, 56, 60, 63, 12, 8, 9)
-COLPASS_ROWS = (1, 2, 4)
+COLPASS_ROWS = (1, 2, 4, 8)
 
 
 def plan_key(opts: dict) -> str:

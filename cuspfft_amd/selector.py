@@ -13,7 +13,7 @@ Features
           fraction of active columns and the 8-bin log2 column-degree histogram
   plan:   log2 M, M / (2 cols - 1), the exponents of 2, 3, 5, 7 in M, log2 M1,
           log2 M2, fast convolution kernel, convolution stages, arithmetic form,
-          column-pass rows per lane (one-hot 1 / 2 / 4, auto = all zero), fused
+          column-pass rows per lane (one-hot 1 / 2 / 4 / 8, auto = all zero), fused
           row kernel
 """
 from __future__ import annotations
@@ -27,7 +27,7 @@ import numpy as np
 
 from . import _check, _vp, describe_plan, load
 
-N_FEATURES = 28
+N_FEATURES = 29
 # the network trained on the autotuner timings of 2000 seeded matrices (train_selector)
 DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "selector.json")
 
@@ -61,7 +61,7 @@ def features(rec: dict, opts: dict, info: dict | None = None) -> np.ndarray:
     f += [math.log2(M), M / (2.0 * cols - 1.0)] + _exponents(M)
     f += [math.log2(m1), math.log2(max(m2, 1)), float(info["conv_fast"]), float(info["conv_stages"]),
           float(info["precision"])]
-    f += [1.0 if cpr == 1 else 0.0, 1.0 if cpr == 2 else 0.0, 1.0 if cpr == 4 else 0.0]
+    f += [1.0 if cpr == r else 0.0 for r in (1, 2, 4, 8)]
     f += [1.0 if info.get("row_kernel") == 2 else 0.0]
     assert len(f) == N_FEATURES
     return np.asarray(f, np.float32)
