@@ -142,20 +142,6 @@ def run_batch_bench(args):
     dt_tuned, arena = run(tuned, graph=True)
     print(f"[c4] tuned graph: {dt_tuned * 1e3:.2f} ms per batch (tuning {t_tune:.0f} s)",
           file=sys.stderr, flush=True)
-    # HBM bytes per launch of the dominant kernel, from the rocprofv3 FETCH_SIZE /
-    # WRITE_SIZE runs recorded in profiles/traffic.json (same workload and stage)
-    traffic = None
-    try:
-        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")) as f:
-            rec = json.load(f).get(args.config, {}).get(dom)
-        default_plan = (args.tile <= 0 and args.fft_len == 0 and args.m1 == 0 and args.precision == 0
-                        and args.rule == 3 and args.colpass_rows == 0 and args.colpass_table == 0
-                        and args.row_kernel == 0)
-        if rec and default_plan:
-            traffic = rec["bytes"]
-    except (OSError, ValueError):
-        traffic = None
-
     rel = None
     if not args.no_check:
         import oracle

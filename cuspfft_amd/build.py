@@ -31,7 +31,9 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
 LIB_SOURCES = ["csc_build.hip", "rowpass.hip", "rowpass_conv_a.hip", "rowpass_conv_b.hip",
                "rowpass_conv_c.hip", "rowpass_conv_d.hip", "rowpass_conv_e.hip",
                "rowpass_fft1_a.hip", "rowpass_fft1_b.hip", "rowpass_fft1_c.hip", "rowpass_mfma.hip",
-               "rowpass_fused_a.hip", "rowpass_fused_b.hip",
+               "rowpass_fused_a.hip", "rowpass_fused_b.hip", "rowpass_fused_c.hip",
+               "rowpass_fused_d.hip", "rowpass_fused_e.hip", "rowpass_fused_f.hip",
+               "rowpass_fused_g.hip", "rowpass_fused_h.hip",
                "spfft_plan.cpp", "spfft_batch.cpp", "mlp.hip",
                "spfft_gen.cpp", "sparse_fft_shim.cpp"]
 CLI_SOURCES = ["cli_main.cpp"]
@@ -39,7 +41,9 @@ CLI_SOURCES = ["cli_main.cpp"]
 
 # The row-FFT files prune their codelets to the inputs and outputs that can be
 # non-zero / are needed: with signed zeros ignored, x + 0 folds to x.
-NO_SIGNED_ZEROS = {"rowpass_fused_a.hip", "rowpass_fused_b.hip", "rowpass_fft1_a.hip",
+NO_SIGNED_ZEROS = {"rowpass_fused_a.hip", "rowpass_fused_b.hip", "rowpass_fused_c.hip",
+                   "rowpass_fused_d.hip", "rowpass_fused_e.hip", "rowpass_fused_f.hip",
+                   "rowpass_fused_g.hip", "rowpass_fused_h.hip", "rowpass_fft1_a.hip",
                    "rowpass_fft1_b.hip", "rowpass_fft1_c.hip"}
 
 

@@ -294,8 +294,11 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
 }
 
 // Instantiated shapes: X(M1, NB, RPW, NT, R1, R2, R3), M = M1 * R1 * R2 * R3 (a
-// radix of 1 is no stage), NT threads per workgroup; each fits its registers
-// without spills.  Group A: the long rows (several LDS batches); group B: one batch.
+// radix of 1 is no stage), NT threads per workgroup, at most 160 KiB of LDS.
+// Group A: the c3 lengths and the power-of-two convolution lengths of its
+// columns; group B: the c1/c2 lengths; groups C-H: a ladder over the 3-smooth
+// lengths from 1024 to 32768 (one workgroup holds RPW rows of M <= 20000 in one
+// LDS batch, longer rows in NB batches).
 //   34992 = 36 * 972   (3-smooth length of 2 * 16706 - 1)
 //   33600 = 35 * 960   (7-smooth length of 2 * 16706 - 1)
 //   36864 = 36 * 1024, 35840 = 35 * 1024 (power-of-two convolution lengths)
@@ -306,7 +309,37 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
 #define SPFFT_FUSED_GROUP_B(X)                                                                  \
     X(24, 1, 1, 768, 27, 27, 1) X(21, 1, 1, 1024, 28, 28, 1) X(16, 1, 3, 1024, 18, 18, 1) \
     X(21, 1, 4, 1024, 8, 28, 1)
-#define SPFFT_FUSED_LIST(X) SPFFT_FUSED_GROUP_A(X) SPFFT_FUSED_GROUP_B(X)
+//   1024 .. 1944
+#define SPFFT_FUSED_GROUP_C(X)                                                                     \
+    X(16, 1, 16, 1024, 8, 8, 1) X(16, 1, 14, 1024, 9, 8, 1) X(16, 1, 12, 1024, 9, 9, 1)          \
+    X(18, 1, 12, 1024, 9, 9, 1) X(16, 1, 10, 1024, 12, 8, 1) X(16, 1, 9, 1024, 12, 9, 1)         \
+    X(18, 1, 9, 1024, 12, 9, 1)
+//   2048 .. 3888
+#define SPFFT_FUSED_GROUP_D(X)                                                                     \
+    X(16, 1, 8, 1024, 16, 8, 1) X(16, 1, 7, 1024, 12, 12, 1) X(16, 1, 6, 1024, 18, 9, 1)         \
+    X(18, 1, 6, 1024, 18, 9, 1) X(16, 1, 5, 1024, 16, 12, 1) X(18, 1, 5, 1024, 16, 12, 1)        \
+    X(16, 1, 4, 1024, 27, 9, 1)
+//   4096 .. 7776
+#define SPFFT_FUSED_GROUP_E(X)                                                                     \
+    X(16, 1, 4, 1024, 16, 16, 1) X(18, 1, 4, 1024, 27, 9, 1) X(18, 1, 4, 1024, 16, 16, 1)        \
+    X(18, 1, 3, 1024, 18, 18, 1) X(24, 1, 3, 768, 16, 16, 1) X(27, 1, 2, 512, 16, 16, 1)         \
+    X(16, 1, 2, 1024, 27, 18, 1)
+//   8192 .. 13122
+#define SPFFT_FUSED_GROUP_F(X)                                                                     \
+    X(16, 1, 2, 1024, 8, 8, 8) X(18, 1, 2, 1024, 27, 18, 1) X(18, 1, 2, 1024, 8, 8, 8)           \
+    X(16, 1, 1, 704, 27, 24, 1) X(18, 1, 1, 704, 27, 24, 1) X(16, 1, 1, 768, 32, 24, 1)          \
+    X(18, 1, 1, 768, 27, 27, 1)
+//   13824 .. 23328
+#define SPFFT_FUSED_GROUP_G(X)                                                                     \
+    X(18, 1, 1, 768, 32, 24, 1) X(16, 1, 1, 1024, 36, 27, 1) X(16, 1, 1, 1024, 32, 32, 1)        \
+    X(18, 1, 1, 1024, 32, 32, 1) X(24, 2, 1, 896, 8, 12, 9) X(24, 2, 1, 512, 9, 12, 9)
+//   24576 .. 32768
+#define SPFFT_FUSED_GROUP_H(X)                                                                     \
+    X(24, 2, 1, 512, 16, 8, 8) X(27, 3, 1, 512, 9, 12, 9) X(27, 3, 1, 512, 16, 8, 8)             \
+    X(32, 2, 1, 512, 9, 12, 9) X(32, 4, 1, 512, 16, 8, 8)
+#define SPFFT_FUSED_LIST(X)                                                                        \
+    SPFFT_FUSED_GROUP_A(X) SPFFT_FUSED_GROUP_B(X) SPFFT_FUSED_GROUP_C(X) SPFFT_FUSED_GROUP_D(X)     \
+    SPFFT_FUSED_GROUP_E(X) SPFFT_FUSED_GROUP_F(X) SPFFT_FUSED_GROUP_G(X) SPFFT_FUSED_GROUP_H(X)
 
 constexpr size_t fused_lds_bytes_of(int m1, int nb, int rpw, int m2) {
     return ((size_t)m2 * (1 + (size_t)rpw * (m1 / nb)) + kFusedWmEntries) * sizeof(float2);
@@ -315,6 +348,12 @@ constexpr size_t fused_lds_bytes_of(int m1, int nb, int rpw, int m2) {
 // Each returns hipErrorInvalidValue when the shape is not in its group.
 hipError_t launch_fused_group_a(const RowPassArgs& a, hipStream_t s, bool prepare_only);
 hipError_t launch_fused_group_b(const RowPassArgs& a, hipStream_t s, bool prepare_only);
+hipError_t launch_fused_group_c(const RowPassArgs& a, hipStream_t s, bool prepare_only);
+hipError_t launch_fused_group_d(const RowPassArgs& a, hipStream_t s, bool prepare_only);
+hipError_t launch_fused_group_e(const RowPassArgs& a, hipStream_t s, bool prepare_only);
+hipError_t launch_fused_group_f(const RowPassArgs& a, hipStream_t s, bool prepare_only);
+hipError_t launch_fused_group_g(const RowPassArgs& a, hipStream_t s, bool prepare_only);
+hipError_t launch_fused_group_h(const RowPassArgs& a, hipStream_t s, bool prepare_only);
 
 #define SPFFT_FUSED_LAUNCH_CASE(m1, nb, rpw, nt, r1, r2, r3)                                       \
     if (a.M1 == m1 && a.fused_nb == nb && a.fused_rpw == rpw && a.fused_nt == nt &&                 \

@@ -21,8 +21,12 @@ bool fused_shape(int i, FusedShape* out) {
 }
 
 hipError_t launch_fused(const RowPassArgs& a, hipStream_t s, bool prepare_only) {
-    hipError_t e = launch_fused_group_a(a, s, prepare_only);
-    if (e == hipErrorInvalidValue) e = launch_fused_group_b(a, s, prepare_only);
+    hipError_t (*const groups[])(const RowPassArgs&, hipStream_t, bool) = {
+        launch_fused_group_a, launch_fused_group_b, launch_fused_group_c, launch_fused_group_d,
+        launch_fused_group_e, launch_fused_group_f, launch_fused_group_g, launch_fused_group_h};
+    hipError_t e = hipErrorInvalidValue;
+    for (auto* g : groups)
+        if ((e = g(a, s, prepare_only)) != hipErrorInvalidValue) break;
     return e;
 }
 

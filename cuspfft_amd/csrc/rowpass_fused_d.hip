@@ -1,0 +1,11 @@
+// rowpass_fused_d.hip -- k_rowfused instantiations of group D (see fused_impl.h).
+#include "fused_impl.h"
+
+namespace spfft {
+
+hipError_t launch_fused_group_d(const RowPassArgs& a, hipStream_t s, bool prepare_only) {
+    SPFFT_FUSED_GROUP_D(SPFFT_FUSED_LAUNCH_CASE)
+    return hipErrorInvalidValue;
+}
+
+}  // namespace spfft

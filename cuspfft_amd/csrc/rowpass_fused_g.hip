@@ -1,0 +1,11 @@
+// rowpass_fused_g.hip -- k_rowfused instantiations of group G (see fused_impl.h).
+#include "fused_impl.h"
+
+namespace spfft {
+
+hipError_t launch_fused_group_g(const RowPassArgs& a, hipStream_t s, bool prepare_only) {
+    SPFFT_FUSED_GROUP_G(SPFFT_FUSED_LAUNCH_CASE)
+    return hipErrorInvalidValue;
+}
+
+}  // namespace spfft

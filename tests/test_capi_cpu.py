@@ -83,11 +83,11 @@ def test_planner_fused_row_kernel():
         four = sp.describe_plan(cols, cols, length_rule=rule, row_kernel=sp.ROW_FOUR_STEP)
         assert four["row_kernel"] == sp.ROW_FOUR_STEP and four["fft_len"] == M
     # no shape for this length, the BF16x3 form, explicit convolution rows -> four-step
-    assert sp.describe_plan(100, 1000, fft_len=2048)["row_kernel"] == sp.ROW_FOUR_STEP
+    assert sp.describe_plan(100, 1000, fft_len=2000)["row_kernel"] == sp.ROW_FOUR_STEP
     assert sp.describe_plan(2310, 2310, precision=sp.PREC_BF16X3)["row_kernel"] == sp.ROW_FOUR_STEP
     assert sp.describe_plan(2310, 2310, conv_rows=2)["row_kernel"] == sp.ROW_FOUR_STEP
     with pytest.raises(sp.SpfftError) as ei:
-        sp.describe_plan(100, 1000, fft_len=2048, row_kernel=sp.ROW_FUSED)
+        sp.describe_plan(100, 1000, fft_len=2000, row_kernel=sp.ROW_FUSED)
     assert ei.value.status == 4
     with pytest.raises(sp.SpfftError) as ei:
         sp.describe_plan(100, 2310, row_kernel=sp.ROW_FUSED, precision=sp.PREC_BF16X3)
@@ -95,6 +95,24 @@ def test_planner_fused_row_kernel():
     with pytest.raises(sp.SpfftError) as ei:
         sp.describe_plan(100, 100, row_kernel=5)
     assert ei.value.status == 1
+
+
+def test_fused_length_ladder():
+    """Every 3-smooth length from 1024 to 36864 except the powers of 3 has a fused
+    row-kernel shape, so the default (3-smooth) plan of any matrix with
+    512 < cols <= 18432 runs the fused kernel; the GPU tests cover exactly the
+    library's list."""
+    import cuspfft_amd as sp
+    from test_gpu_path import FUSED_LENGTHS
+    have = sp.fused_lengths()
+    assert sorted(have) == sorted(FUSED_LENGTHS)
+    smooth3 = sorted({2 ** a * 3 ** b for a in range(16) for b in range(10)
+                      if 1024 <= 2 ** a * 3 ** b <= 36864})
+    missing = [m for m in smooth3 if m not in have]
+    assert missing == [2187, 6561, 19683]
+    for cols in (513, 700, 1500, 4000, 9000, 12000, 16000, 18432):
+        info = sp.describe_plan(cols, cols)
+        assert info["fft_len"] in have and info["row_kernel"] == sp.ROW_FUSED
 
 
 def test_planner_errors():

@@ -317,7 +317,11 @@ def test_m1_splits(gpu, oracle_mod, cols, fft_len, m1):
 
 
 # Bluestein lengths with a fused row-kernel shape (one per instantiated shape)
-FUSED_LENGTHS = [34992, 33600, 17496, 16464, 5184, 4704]
+FUSED_LENGTHS = [34992, 33600, 36864, 35840, 17496, 16464, 5184, 4704,
+                 1024, 1152, 1296, 1458, 1536, 1728, 1944, 2048, 2304, 2592, 2916, 3072, 3456,
+                 3888, 4096, 4374, 4608, 5832, 6144, 6912, 7776, 8192, 8748, 9216, 10368, 11664,
+                 12288, 13122, 13824, 15552, 16384, 18432, 20736, 23328, 24576, 26244, 27648,
+                 31104, 32768]
 
 
 @pytest.mark.parametrize("M", FUSED_LENGTHS)
