@@ -4,17 +4,18 @@
 // The four-step split M = M1 * M2 (M2 = R1 * R2 * R3) of k_fft1 / k_conv /
 // k_ifft1, without the tile buffer T in HBM:
 //   phase A  each column n2 < M2 (CPT columns per thread) loads x[M2 n1 + n2]
-//            (n1 < M1) from the pass-1 row and runs the length-M1 register
-//            codelet; the M1 results stay in registers.
+//            (n1 < M1) from the pass-1 row, runs the length-M1 register
+//            codelet and multiplies by W_M^{k1 n2}; the M1 results stay in
+//            registers.
 //   phase B  the M1 rows k1 (length M2) go through LDS in NB batches of
 //            KB = M1 / NB rows: each thread swaps its columns of the batch with
 //            the rows processed in the previous batch, then all threads run the
-//            Stockham stages R1, R2, R3 forward (the first one times W_M^{k1 n2}
-//            on its reads, the last one times B^[k1 + M1 k2] / M) and R3, R2, R1
-//            inverse (the last one times W_M^{-k1 n2} on its writes), in place
-//            in LDS; the last forward and first inverse stage share one pass.
-//   phase C  each column runs the inverse length-M1 codelet, applies the chirp
-//            and writes row u and its mirror row R - u
+//            Stockham stages R1, R2, R3 forward (the last one times
+//            B^[k1 + M1 k2] / M) and R3, R2, R1 inverse, in place in LDS; the
+//            last forward and first inverse stage share one pass.
+//   phase C  each column multiplies by W_M^{-k1 n2}, runs the inverse
+//            length-M1 codelet, applies the chirp and writes row u and its
+//            mirror row R - u
 //            (X[R-u][v] = conj(X[u][(C - v) mod C]) for a real input).
 // HBM traffic per row: the pass-1 row in, the two half-spectrum rows out.
 //
@@ -44,9 +45,6 @@ namespace spfft {
 //     the first inverse stage, which runs the inverse radices in reverse order
 //     so that it takes the same R values as the butterfly that produced them
 //     (j + r L/R): one LDS round trip for both.
-// TWIN multiplies the values read by W_M^{k1 n2} (first forward stage), TWOUT
-// the values written by W_M^{-k1 n2} (last inverse stage), both from an LDS
-// table (a global table read here would stall the stage on its latency).
 enum { kFwd = 0, kInv = 1, kPivot = 2 };
 // the W_M^e table: 256 low entries and (M + 255) / 256 high ones, M <= 65536
 constexpr int kFusedWmEntries = 256 + 256;
@@ -56,9 +54,31 @@ __device__ __forceinline__ float2 wm_pow(const float2* s_wm, int e) {
     return cmul(s_wm[256 + (e >> 8)], s_wm[e & 255]);
 }
 
-template <int R, int L, int NS, int MODE, bool TWIN, bool TWOUT, int G, int KB, int NT>
-__device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const float2* s_tw,
-                                            const float2* s_wm, int kbase) {
+// v[k1] times W_M^{k1 n2} (INV: its conjugate) for the thread's column n2, k1 < M1:
+// with k1 = A i + j (j < A), W_M^{k1 n2} = W_M^{A i n2} W_M^{j n2}, so a column
+// takes A + B - 2 table lookups and one product per k1 instead of a lookup per k1.
+// (Applied in registers here rather than on the first and last LDS stage: 5%
+// fewer VALU and 19% fewer LDS instructions, 7% faster on the c2 and c3 rows.)
+template <int M1, bool INV>
+__device__ __forceinline__ void fused_twiddle(float2* v, int n2, const float2* s_wm) {
+    constexpr int A = M1 >= 49 ? 8 : M1 >= 25 ? 6 : M1 >= 9 ? 4 : 2;
+    constexpr int B = (M1 + A - 1) / A;
+    float2 lo[A], hi[B];
+    lo[0] = hi[0] = make_float2(1.f, 0.f);
+#pragma unroll
+    for (int j = 1; j < A; j++) lo[j] = wm_pow(s_wm, j * n2);
+#pragma unroll
+    for (int i = 1; i < B; i++) hi[i] = wm_pow(s_wm, A * i * n2);
+#pragma unroll
+    for (int k1 = 1; k1 < M1; k1++) {
+        const int i = k1 / A, j = k1 % A;
+        const float2 w = i == 0 ? lo[j] : j == 0 ? hi[i] : cmul(hi[i], lo[j]);
+        v[k1] = INV ? cmulc(v[k1], w) : cmul(v[k1], w);
+    }
+}
+
+template <int R, int L, int NS, int MODE, int G, int KB, int NT>
+__device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const float2* s_tw, int kbase) {
     constexpr int nb = L / R;
     constexpr int total = G * nb;
     constexpr int MB = (total + NT - 1) / NT;
@@ -87,11 +107,6 @@ __device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const
             const int jq = j / NS;
             const int k = j - jq * NS;
             const int rowoff = (g % KB) * L;  // (k1 - kbase) * L
-            const int k1 = kbase + g % KB;
-            if (TWIN) {
-#pragma unroll
-                for (int r = 0; r < R; r++) v[b][r] = cmul(v[b][r], wm_pow(s_wm, k1 * (j + r * nb)));
-            }
             if (NS > 1 && k != 0) {
 #pragma unroll
                 for (int r = 1; r < R; r++) {
@@ -112,10 +127,6 @@ __device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const
                 base = j * R;
                 stride = 1;
             }
-            if (TWOUT) {
-#pragma unroll
-                for (int r = 0; r < R; r++) v[b][r] = cmulc(v[b][r], wm_pow(s_wm, k1 * (base + r * stride)));
-            }
             float2* dst = buf + g * L + base;
 #pragma unroll
             for (int r = 0; r < R; r++) dst[r * stride] = v[b][r];
@@ -128,20 +139,19 @@ __device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const
 // stages Rn .. R1 (radices of 1 are no stage), the last forward and first
 // inverse stage merged (kPivot).
 template <int L, int G, int KB, int NT, int R1, int R2, int R3>
-__device__ __forceinline__ void fused_conv(bio::rsrc_t r_bh, float2* buf, const float2* s_tw,
-                                           const float2* s_wm, int kbase) {
+__device__ __forceinline__ void fused_conv(bio::rsrc_t r_bh, float2* buf, const float2* s_tw, int kbase) {
     if constexpr (R3 > 1) {
-        fused_stage<R1, L, 1, kFwd, true, false, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
-        fused_stage<R2, L, R1, kFwd, false, false, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
-        fused_stage<R3, L, R1 * R2, kPivot, false, false, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
-        fused_stage<R2, L, R3, kInv, false, false, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
-        fused_stage<R1, L, R3 * R2, kInv, false, true, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
+        fused_stage<R1, L, 1, kFwd, G, KB, NT>(r_bh, buf, s_tw, kbase);
+        fused_stage<R2, L, R1, kFwd, G, KB, NT>(r_bh, buf, s_tw, kbase);
+        fused_stage<R3, L, R1 * R2, kPivot, G, KB, NT>(r_bh, buf, s_tw, kbase);
+        fused_stage<R2, L, R3, kInv, G, KB, NT>(r_bh, buf, s_tw, kbase);
+        fused_stage<R1, L, R3 * R2, kInv, G, KB, NT>(r_bh, buf, s_tw, kbase);
     } else if constexpr (R2 > 1) {
-        fused_stage<R1, L, 1, kFwd, true, false, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
-        fused_stage<R2, L, R1, kPivot, false, false, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
-        fused_stage<R1, L, R2, kInv, false, true, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
+        fused_stage<R1, L, 1, kFwd, G, KB, NT>(r_bh, buf, s_tw, kbase);
+        fused_stage<R2, L, R1, kPivot, G, KB, NT>(r_bh, buf, s_tw, kbase);
+        fused_stage<R1, L, R2, kInv, G, KB, NT>(r_bh, buf, s_tw, kbase);
     } else {
-        fused_stage<R1, L, 1, kPivot, true, true, G, KB, NT>(r_bh, buf, s_tw, s_wm, kbase);
+        fused_stage<R1, L, 1, kPivot, G, KB, NT>(r_bh, buf, s_tw, kbase);
     }
 }
 
@@ -174,11 +184,10 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
         n2[c] = q - g[c] * M2;
         live[c] = (RPW == 1 ? q < M2 : g[c] < RPW) && row0 + g[c] < a.tile_rows;
     }
-    for (int i = t; i < M2; i += NT) s_tw[i] = a.tw2[i];
-    for (int i = t; i < 256 + (a.M + 255) / 256; i += NT) s_wm[i] = a.wm2[i];
     const bio::rsrc_t r_bh = bio::make_rsrc(a.bhat, (uint32_t)a.M * 8u);
 
-    // phase A
+    // phase A: the loads go out first, then the tables are filled (phase A's
+    // twiddles read W_M from LDS after a barrier)
     float2 v[CPT][M1];
     {
         const int nrows = min(RPW, a.tile_rows - row0);
@@ -201,7 +210,16 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
                         v[c][n1] = bio::load2_stream(r_x, w < a.C ? (rb + w) * 8 : bio::kOob, 0);
                     }
                 }
+            }
+        }
+        for (int i = t; i < M2; i += NT) s_tw[i] = a.tw2[i];
+        for (int i = t; i < 256 + (a.M + 255) / 256; i += NT) s_wm[i] = a.wm2[i];
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            if (live[c]) {
                 RegFft<M1>::run(v[c], false);
+                fused_twiddle<M1, false>(v[c], n2[c], s_wm);
             }
         }
     }
@@ -226,7 +244,7 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
             }
         }
         __syncthreads();
-        fused_conv<M2, G, KB, NT, R1, R2, R3>(r_bh, buf, s_tw, s_wm, b * KB);
+        fused_conv<M2, G, KB, NT, R1, R2, R3>(r_bh, buf, s_tw, b * KB);
     });
 
     // phase C
@@ -251,6 +269,7 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
                 const float2* col = buf + n2[c];
 #pragma unroll
                 for (int k = 0; k < KB; k++) v[c][(NB - 1) * KB + k] = col[k * M2];
+                fused_twiddle<M1, true>(v[c], n2[c], s_wm);
                 RegFft<M1>::run(v[c], true);
                 const int n1c = (a.C - n2[c] + M2 - 1) / M2;  // columns w = n2 + M2 n1 < C
                 const int vm0 = (M2 - 1 - n2[c]) * 8;
@@ -275,6 +294,7 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
             const float2* col = buf + g[0] * KB * M2 + n2[0];
 #pragma unroll
             for (int k = 0; k < KB; k++) v[0][(NB - 1) * KB + k] = col[k * M2];
+            fused_twiddle<M1, true>(v[0], n2[0], s_wm);
             RegFft<M1>::run(v[0], true);
             const int u = u0 + g[0];
             const bool mirror = u > 0 && a.R - u > u;
