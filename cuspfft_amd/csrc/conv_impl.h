@@ -2,8 +2,8 @@
 // product with B^[k1 + M1 k2] / M, and the inverse length-M2 FFT, through LDS.
 //
 // Fast path k_conv_fast<R1, R2>: M2 = R1 * R2 with both radices compile-time
-// register codelets; four Stockham stages (forward R1, R2 with the B^ product
-// fused into the R2 stage, inverse R1, R2).  The first stage reads T straight
+// register codelets; three passes (forward R1; forward R2, the B^ product and
+// inverse R2 in one pass; inverse R1).  The first stage reads T straight
 // from HBM and the last writes it straight back, so T is read and written once
 // and LDS holds the row between the stages.  Twiddles between the stages come
 // from an LDS copy of the W_M2 table.
@@ -19,7 +19,10 @@ enum StageIO { kGlobalToLds = 0, kLdsToLds = 1, kLdsToGlobal = 2 };
 
 // One Stockham stage of radix R over G2 rows of length L held in LDS (or
 // streamed from / to T).  NS = product of the radices of the earlier stages.
-template <int R, int L, int NS, int IO, bool INV, bool MUL>
+// PIVOT: the last forward stage (NS = L / R), the product with B^, and the
+// first inverse stage on the same R values (outputs j R + r), one LDS round
+// trip for both; the inverse then runs its radices in reverse order.
+template <int R, int L, int NS, int IO, bool INV, bool MUL, bool PIVOT = false>
 __device__ __forceinline__ void cstage(const RowPassArgs& a, float2* buf, const float2* s_tw,
                                        const int* s_k1, int row0) {
     constexpr int MB = MaxBfly<R>::value;
@@ -65,23 +68,30 @@ __device__ __forceinline__ void cstage(const RowPassArgs& a, float2* buf, const 
             }
         }
         RegFft<R>::run(v[b], INV);
-        const int base = jq * NS * R + k;
+        int base = jq * NS * R + k;
+        int stride = NS;
         if (MUL) {
             const int k1 = s_k1[g];
             const float2* bh = a.bhat + (size_t)(k1 < 0 ? 0 : k1) * L;
 #pragma unroll
             for (int r = 0; r < R; r++) v[b][r] = cmul(v[b][r], bh[base + r * NS]);
         }
+        if (PIVOT) {
+            static_assert(!PIVOT || (MUL && !INV && NS * R == L), "the pivot is the last forward stage");
+            RegFft<R>::run(v[b], true);
+            base = j * R;
+            stride = 1;
+        }
         if (IO == kLdsToGlobal) {
             if (s_k1[g] >= 0) {
                 float2* dst = a.T + (size_t)(row0 + g) * L;
 #pragma unroll
-                for (int r = 0; r < R; r++) dst[base + r * NS] = v[b][r];
+                for (int r = 0; r < R; r++) dst[base + r * stride] = v[b][r];
             }
         } else {
             float2* dst = buf + g * L;
 #pragma unroll
-            for (int r = 0; r < R; r++) dst[base + r * NS] = v[b][r];
+            for (int r = 0; r < R; r++) dst[base + r * stride] = v[b][r];
         }
     }
     if (IO != kLdsToGlobal) __syncthreads();
@@ -110,9 +120,8 @@ __global__ __launch_bounds__(kThreads) void k_conv_fast(RowPassArgs a) {
         cstage<R1, L, 1, kLdsToGlobal, true, false>(a, buf, s_tw, s_k1, row0);
     } else {
         cstage<R1, L, 1, kGlobalToLds, false, false>(a, buf, s_tw, s_k1, row0);
-        cstage<R2, L, R1, kLdsToLds, false, true>(a, buf, s_tw, s_k1, row0);
-        cstage<R1, L, 1, kLdsToLds, true, false>(a, buf, s_tw, s_k1, row0);
-        cstage<R2, L, R1, kLdsToGlobal, true, false>(a, buf, s_tw, s_k1, row0);
+        cstage<R2, L, R1, kLdsToLds, false, true, true>(a, buf, s_tw, s_k1, row0);
+        cstage<R1, L, R2, kLdsToGlobal, true, false>(a, buf, s_tw, s_k1, row0);
     }
 }
 
