@@ -39,9 +39,11 @@ constexpr int kSortWaves = kSortThreads / 64;
 constexpr int kSortIters = 8;
 constexpr int kSortBlock = kSortThreads * kSortIters;  // 2048 elements per block
 
-__global__ void k_fill_i32(int* __restrict__ p, long n, int v) {
+// Zeroes p[0, n) and q[0] (the column counts and the error flag) in one launch.
+__global__ void k_zero_counts(int* __restrict__ p, long n, int* __restrict__ q) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    if (i < n) p[i] = 0;
+    if (i == 0) q[0] = 0;
 }
 
 __global__ void k_count_cols(const int* __restrict__ row, const int* __restrict__ col, int nnz,
@@ -164,6 +166,55 @@ __global__ void k_scan_apply(const int* __restrict__ in, int n, const int2* __re
     }
 }
 
+// Single-block scan for n + 1 <= kOneBlock elements: k_scan_reduce, k_scan_top
+// and k_scan_apply in one launch (the same outputs, totals included).  Small
+// scans are bound by launch latency; larger ones run faster on many blocks.
+constexpr int kOneThreads = 1024;
+constexpr int kOneItems = 4;
+constexpr int kOneBlock = kOneThreads * kOneItems;
+
+template <int MODE, bool PACKED>
+__global__ __launch_bounds__(kOneThreads) void k_scan_single(const int* __restrict__ in, int n,
+                                                             int* __restrict__ totals, int* __restrict__ out,
+                                                             void* __restrict__ sparse_cols,
+                                                             int* __restrict__ col_ptr) {
+    __shared__ int s_w[kOneThreads / 64];
+    const int base = threadIdx.x * kOneItems;
+    int v[kOneItems];
+    int sx = 0, sy = 0;
+#pragma unroll
+    for (int k = 0; k < kOneItems; k++) {
+        const int i = base + k;
+        v[k] = i < n ? in[i] : 0;
+        sx += v[k];
+        sy += (MODE == 1 && v[k] > 0) ? 1 : 0;
+    }
+    int tx, ty;
+    int ex = block_exclusive_scan(sx, s_w, tx);
+    int ey = block_exclusive_scan(sy, s_w, ty);
+#pragma unroll
+    for (int k = 0; k < kOneItems; k++) {
+        const int i = base + k;
+        if (i < n) {
+            out[i] = ex;
+            if (MODE == 1 && v[k] > 0) {
+                if (PACKED) ((uint16_t*)sparse_cols)[ey] = (uint16_t)i;
+                else ((int*)sparse_cols)[ey] = i;
+                col_ptr[ey] = ex;
+                ey += 1;
+            }
+        } else if (i == n) {
+            out[n] = ex;
+            if (MODE == 1) col_ptr[ey] = ex;
+        }
+        ex += v[k];
+    }
+    if (threadIdx.x == 0 && totals) {
+        totals[0] = tx;
+        totals[1] = ty;
+    }
+}
+
 __device__ __forceinline__ int digit_of(int key, int shift) { return (int)(((uint32_t)key >> shift) & 255u); }
 
 __global__ void k_radix_hist(const int* __restrict__ keys, int nnz, int shift, int* __restrict__ hist,
@@ -251,6 +302,11 @@ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 template <int MODE, bool PACKED>
 hipError_t run_scan(const int* in, int n, int2* bsum, int* totals, int* out, void* sparse_cols,
                     int* col_ptr, hipStream_t s) {
+    if ((long)n + 1 <= kOneBlock) {
+        hipLaunchKernelGGL((k_scan_single<MODE, PACKED>), dim3(1), dim3(kOneThreads), 0, s, in, n, totals, out,
+                           sparse_cols, col_ptr);
+        return hipGetLastError();
+    }
     // one extra slot so that index n (the total) is written by some block
     const int nb = cdiv((long)n + 1, kScanBlock);
     hipLaunchKernelGGL((k_scan_reduce<MODE>), dim3(nb), dim3(kScanThreads), 0, s, in, n, bsum);
@@ -275,8 +331,7 @@ int csc_sort_passes(int cols) {
 hipError_t launch_csc_build(const CscBuildArgs& a, hipStream_t s) {
     const int R = a.R, C = a.C, nnz = a.nnz;
     // 1. counts + validation
-    hipLaunchKernelGGL(k_fill_i32, dim3(cdiv((long)C + 1, 256)), dim3(256), 0, s, a.counts, (long)C + 1, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, a.err, 1L, 0);
+    hipLaunchKernelGGL(k_zero_counts, dim3(cdiv((long)C + 1, 256)), dim3(256), 0, s, a.counts, (long)C + 1, a.err);
     if (nnz > 0)
         hipLaunchKernelGGL(k_count_cols, dim3(cdiv(nnz, 256)), dim3(256), 0, s, a.row, a.col, nnz, R, C,
                            a.counts, a.err);
