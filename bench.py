@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--precision", type=int, default=0, help="0 plain FP32, 1 BF16x3 MFMA")
     ap.add_argument("--colpass-rows", type=int, default=0)
     ap.add_argument("--colpass-table", type=int, default=0)
+    ap.add_argument("--colpass-pair", type=int, default=0,
+                    help="0 auto (rows u and R/2 - u share the column pass when R is even), 1 off")
     ap.add_argument("--row-kernel", type=int, default=0,
                     help="0 auto (fused row kernel when M has a shape), 1 four-step kernels, 2 fused")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -218,7 +220,8 @@ def main():
     tile = args.tile if args.tile > 0 else m.rows
     plan = sp.Plan(m.rows, m.cols, length_rule=args.rule, u_tile=tile, m1=args.m1, fft_len=args.fft_len,
                    precision=args.precision, row_kernel=args.row_kernel,
-                   colpass_rows=args.colpass_rows, colpass_table=args.colpass_table)
+                   colpass_rows=args.colpass_rows, colpass_table=args.colpass_table,
+                   colpass_pair=args.colpass_pair)
     d_row = torch.as_tensor(m.row_idx).to(dev)
     d_col = torch.as_tensor(m.col_idx).to(dev)
     out = plan.alloc_output()
@@ -260,7 +263,7 @@ def main():
             rec = json.load(f).get(args.config, {}).get(dom)
         default_plan = (args.tile <= 0 and args.fft_len == 0 and args.m1 == 0 and args.precision == 0
                         and args.rule == 3 and args.colpass_rows == 0 and args.colpass_table == 0
-                        and args.row_kernel == 0)
+                        and args.row_kernel == 0 and args.colpass_pair == 0)
         if rec and default_plan:
             traffic = rec["bytes"]
     except (OSError, ValueError):
@@ -323,6 +326,7 @@ def main():
             "m2": info["m2"], "u_tile": info["u_tile"], "conv_rows": info["conv_rows"],
             "conv_radix": info["conv_radix"], "colpass_rows": info["colpass_rows"],
             "colpass_full_table": info["colpass_full_table"],
+            "colpass_paired": info["colpass_paired"],
             "row_kernel": {1: "four-step", 2: "fused"}[info["row_kernel"]],
         },
         "rel_l2_vs_fp64": rel,

@@ -25,6 +25,8 @@ PREC_FP32 = 0
 PREC_BF16X3 = 1
 # row-pass kernels (spfft_opts.row_kernel)
 ROW_AUTO, ROW_FOUR_STEP, ROW_FUSED = 0, 1, 2
+# column-pass half-period row pairing (spfft_opts.colpass_pair)
+PAIR_AUTO, PAIR_OFF = 0, 1
 
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "HIP", 3: "ALLOC", 4: "UNSUPPORTED", 5: "BAD_INPUT"}
 
@@ -40,7 +42,7 @@ class Opts(ctypes.Structure):
                 ("precision", ctypes.c_int), ("fft_len", ctypes.c_int), ("m1", ctypes.c_int),
                 ("conv_rows", ctypes.c_int), ("colpass_rows", ctypes.c_int),
                 ("colpass_table", ctypes.c_int), ("row_limit", ctypes.c_int),
-                ("row_kernel", ctypes.c_int)]
+                ("row_kernel", ctypes.c_int), ("colpass_pair", ctypes.c_int)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -51,7 +53,7 @@ class PlanInfo(ctypes.Structure):
                 ("conv_stages", ctypes.c_int), ("conv_radix", ctypes.c_int * 8),
                 ("colpass_rows", ctypes.c_int), ("colpass_full_table", ctypes.c_int),
                 ("precision", ctypes.c_int), ("workspace_bytes", ctypes.c_size_t),
-                ("row_kernel", ctypes.c_int)]
+                ("row_kernel", ctypes.c_int), ("colpass_paired", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -257,13 +259,14 @@ def _stream_handle(stream=None):
 
 
 def _opts(length_rule=SMOOTH_3, u_tile=128, fft_len=0, m1=0, conv_rows=0, precision=PREC_FP32,
-          colpass_rows=0, colpass_table=0, row_limit=0, row_kernel=ROW_AUTO):
+          colpass_rows=0, colpass_table=0, row_limit=0, row_kernel=ROW_AUTO, colpass_pair=PAIR_AUTO):
     o = Opts()
     load().spfft_opts_default(ctypes.byref(o))
     o.length_rule, o.u_tile, o.precision = length_rule, u_tile, precision
     o.fft_len, o.m1, o.conv_rows = fft_len, m1, conv_rows
     o.colpass_rows, o.colpass_table, o.row_limit = colpass_rows, colpass_table, row_limit
     o.row_kernel = row_kernel
+    o.colpass_pair = colpass_pair
     return o
 
 
@@ -288,10 +291,10 @@ class Plan:
     def __init__(self, rows: int, cols: int, length_rule: int = SMOOTH_3, u_tile: int = 128,
                  fft_len: int = 0, m1: int = 0, conv_rows: int = 0, precision: int = PREC_FP32,
                  colpass_rows: int = 0, colpass_table: int = 0, row_limit: int = 0,
-                 row_kernel: int = ROW_AUTO):
+                 row_kernel: int = ROW_AUTO, colpass_pair: int = PAIR_AUTO):
         L = load()
         o = _opts(length_rule, u_tile, fft_len, m1, conv_rows, precision, colpass_rows,
-                  colpass_table, row_limit, row_kernel)
+                  colpass_table, row_limit, row_kernel, colpass_pair)
         self._h = _vp()
         _check(L.spfft_plan_create(int(rows), int(cols), ctypes.byref(o), ctypes.byref(self._h)))
         inf = PlanInfo()

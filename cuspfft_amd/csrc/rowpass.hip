@@ -89,8 +89,23 @@ __global__ __launch_bounds__(64) void k_colsched(const int* __restrict__ colstar
     }
 }
 
+// acc[0] += w, acc[t] += w s^t (t < N): a lane's rows 64 apart share one phase
+// lookup; each further row costs one complex product
+template <int N>
+__device__ __forceinline__ void accumulate_rows(float2 (&acc)[N], float2 w, float2 s) {
+    acc[0].x += w.x;
+    acc[0].y += w.y;
+#pragma unroll
+    for (int t = 1; t < N; t++) {
+        w = cmul(w, s);
+        acc[t].x += w.x;
+        acc[t].y += w.y;
+    }
+}
+
 // ---------------------------------------------------------------------------
-// k_colpass: a tile of TC columns x 64*U rows per workgroup (8 waves).
+// k_colpass: a tile of TC columns x 64*U rows per workgroup (8 waves; PAIR:
+// 64*U rows u and their 64*U partner rows).
 // The tile's columns are shared out to the waves by k_colsched, once per
 // transform; a wave takes its columns one at a time, so the column is uniform over
 // the wave; lane l takes rows u0 + l + 64t (t < U).  A column's row indices are
@@ -107,8 +122,16 @@ __global__ __launch_bounds__(64) void k_colsched(const int* __restrict__ colstar
 // The chirp h[c] is applied and the sums go through an LDS tile (re and im
 // planes, odd row stride TC + 1: conflict-free both ways), so X0 is written in
 // whole TC-column row segments (TC * 8 bytes).
+//
+// PAIR (R even, one tile of rows [0, H], H = R/2): W_R^H = -1, so row u + H
+// sums (-1)^r W_R^{u r}.  A lane keeps the even-r and odd-r sums E, O of its
+// rows u <= H/2 (each 64-nonzero chunk is permuted across lanes so that its
+// even rows come first: two straight loops, no branch per nonzero) and emits
+//   S[u] = E + O,   S[H - u] = conj(S[u + H]) = conj(E - O)
+// (S[R - u'] = conj(S[u']) for a real input), through a second LDS tile.  One
+// phase reduction and lookup per nonzero now serves twice the rows.
 // ---------------------------------------------------------------------------
-template <bool PACKED, bool FULL, int U, int TC>
+template <bool PACKED, bool FULL, int U, int TC, bool PAIR>
 __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
     extern __shared__ float2 smem[];
     const int lane = threadIdx.x & 63;
@@ -121,9 +144,10 @@ __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
     // the tables first, so that their offsets fit the LDS instructions' immediate field
     float2* s_lo = smem;  // FULL: the whole W_R table
     float2* s_hi = s_lo + nlo;
-    float* t_re = reinterpret_cast<float*>(FULL ? s_lo + R : s_hi + a.n_hi);  // [64U][TS]
-    float* t_im = t_re + 64 * U * TS;                                          // [64U][TS]
-    int* s_cs = reinterpret_cast<int*>(t_im + 64 * U * TS);                  // colstart[c0 .. c0+TC]
+    constexpr int NTILE = PAIR ? 2 : 1;  // row tiles: rows u, and (PAIR) rows H - u
+    float* t_re = reinterpret_cast<float*>(FULL ? s_lo + R : s_hi + a.n_hi);  // [NTILE][64U][TS]
+    float* t_im = t_re + NTILE * 64 * U * TS;                                  // [NTILE][64U][TS]
+    int* s_cs = reinterpret_cast<int*>(t_im + NTILE * 64 * U * TS);          // colstart[c0 .. c0+TC]
     if (FULL) {
         for (int i = threadIdx.x; i < (int)R; i += kColThreads) s_lo[i] = a.wr_full[i];
     } else {
@@ -146,9 +170,9 @@ __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
     for (int i = i0; i < i1; i++) {
         const int cc = __builtin_amdgcn_readfirstlane(sched[i]);  // column of the tile
         const int c = c0 + cc;
-        float2 acc[U];
+        float2 acc[U], acc_o[U];  // PAIR: acc sums even r, acc_o odd r
 #pragma unroll
-        for (int t = 0; t < U; t++) acc[t] = make_float2(0.f, 0.f);
+        for (int t = 0; t < U; t++) acc[t] = acc_o[t] = make_float2(0.f, 0.f);
         if (c < a.C) {
             const int beg = s_cs[cc], end = s_cs[cc + 1];
             for (int p0 = beg; p0 < end; p0 += 64) {
@@ -158,53 +182,83 @@ __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
                 if (lane < n)
                     r = PACKED ? (uint32_t)((const uint16_t*)a.row_idx)[p0 + lane]
                                : (uint32_t)((const int*)a.row_idx)[p0 + lane];
+                int n_even = n;
+                if (PAIR) {
+                    // the chunk's even rows first, then its odd rows (one forward permute)
+                    const uint64_t valid = n >= 64 ? ~0ull : (1ull << n) - 1;
+                    const uint64_t om = __ballot((r & 1u) != 0) & valid, em = ~om & valid;
+                    n_even = n - __popcll(om);
+                    const uint64_t m = (r & 1u) ? om : em;
+                    int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                    pos = lane >= n ? lane : (r & 1u) ? n_even + pos : pos;
+                    r = (uint32_t)__builtin_amdgcn_ds_permute(pos << 2, (int)r);
+                }
                 const uint32_t base = PACKED ? fastmod_u32(u0 * r, a.r_magic, R)
                                              : (uint32_t)(((uint64_t)u0 * r) % (uint64_t)R);
                 const uint32_t d64 = fastmod_u32(r << 6, a.r_magic, R);
                 float2 w64 = make_float2(1.f, 0.f);
                 if (U > 1) w64 = wr(d64);
+                // nonzeros jb .. je of the chunk into the lane's U row sums A
+                auto run = [&](int jb, int je, float2(&A)[U]) {
 #pragma unroll 2
-                for (int j = 0; j < n; j++) {
-                    const uint32_t rj = (uint32_t)__builtin_amdgcn_readlane((int)r, j);
-                    const uint32_t bj = (uint32_t)__builtin_amdgcn_readlane((int)base, j);
-                    // (u0 + lane) r mod R = (bj + lane rj) mod R, bj + lane rj < 64 R
-                    const uint32_t x = bj + __umul24((uint32_t)lane, rj);
-                    const uint32_t q = (uint32_t)((float)x * a.inv_r);
-                    const uint32_t y = x - __umul24(q, R);  // in [0, 2R)
-                    const uint32_t ph = min(y, y - R);      // y - R wraps when y < R
-                    float2 w = wr(ph);
-                    acc[0].x += w.x;
-                    acc[0].y += w.y;
-                    if (U > 1) {
-                        const float2 s = make_float2(__builtin_bit_cast(float, __builtin_amdgcn_readlane(
-                                                         __builtin_bit_cast(int, w64.x), j)),
-                                                     __builtin_bit_cast(float, __builtin_amdgcn_readlane(
-                                                         __builtin_bit_cast(int, w64.y), j)));
-#pragma unroll
-                        for (int t = 1; t < U; t++) {
-                            w = cmul(w, s);
-                            acc[t].x += w.x;
-                            acc[t].y += w.y;
-                        }
+                    for (int j = jb; j < je; j++) {
+                        const uint32_t rj = (uint32_t)__builtin_amdgcn_readlane((int)r, j);
+                        const uint32_t bj = (uint32_t)__builtin_amdgcn_readlane((int)base, j);
+                        // (u0 + lane) r mod R = (bj + lane rj) mod R, bj + lane rj < 64 R
+                        const uint32_t x = bj + __umul24((uint32_t)lane, rj);
+                        const uint32_t q = (uint32_t)((float)x * a.inv_r);
+                        const uint32_t y = x - __umul24(q, R);  // in [0, 2R)
+                        const uint32_t ph = min(y, y - R);      // y - R wraps when y < R
+                        float2 sj = make_float2(1.f, 0.f);
+                        if (U > 1)
+                            sj = make_float2(__builtin_bit_cast(float, __builtin_amdgcn_readlane(
+                                                 __builtin_bit_cast(int, w64.x), j)),
+                                             __builtin_bit_cast(float, __builtin_amdgcn_readlane(
+                                                 __builtin_bit_cast(int, w64.y), j)));
+                        accumulate_rows<U>(A, wr(ph), sj);
                     }
-                }
+                };
+                run(0, n_even, acc);
+                if (PAIR) run(n_even, n, acc_o);
             }
             const float2 hc = a.h[c];
 #pragma unroll
-            for (int t = 0; t < U; t++) acc[t] = cmul(acc[t], hc);
+            for (int t = 0; t < U; t++) {
+                if (PAIR) {
+                    const float2 e = acc[t], o = acc_o[t];
+                    acc[t] = make_float2(e.x + o.x, e.y + o.y);
+                    acc_o[t] = cmul(make_float2(e.x - o.x, o.y - e.y), hc);
+                }
+                acc[t] = cmul(acc[t], hc);
+            }
         }
 #pragma unroll
         for (int t = 0; t < U; t++) {
             t_re[(t * 64 + lane) * TS + cc] = acc[t].x;
             t_im[(t * 64 + lane) * TS + cc] = acc[t].y;
+            if (PAIR) {
+                t_re[((U + t) * 64 + lane) * TS + cc] = acc_o[t].x;
+                t_im[((U + t) * 64 + lane) * TS + cc] = acc_o[t].y;
+            }
         }
     }
     __syncthreads();
     // whole TC-column row segments of X0 (ldx is a multiple of 64)
-    for (int e = threadIdx.x; e < 64 * U * TC; e += kColThreads) {
+    const int half = a.R >> 1, qmax = half >> 1;  // PAIR: rows u <= qmax and H - u
+    for (int e = threadIdx.x; e < NTILE * 64 * U * TC; e += kColThreads) {
         const int row = e / TC, col = e % TC;
-        const int ul = u0l + row;
-        if (ul < a.tile_rows) {  // X0 is read once, by the row pass: non-temporal
+        int ul;
+        bool ok;
+        if (PAIR) {
+            const bool partner = row >= 64 * U;
+            const int u = u0l + (partner ? row - 64 * U : row);
+            ul = partner ? half - u : u;
+            ok = u <= qmax && (!partner || ul != u);  // H - u == u only for u = H/2: written once
+        } else {
+            ul = u0l + row;
+            ok = ul < a.tile_rows;
+        }
+        if (ok) {  // X0 is read once, by the row pass: non-temporal
             typedef float f32x2 __attribute__((ext_vector_type(2)));
             const f32x2 x = {t_re[row * TS + col], t_im[row * TS + col]};
             __builtin_nontemporal_store(x, reinterpret_cast<f32x2*>(&a.X0[(size_t)ul * a.ldx + c0 + col]));
@@ -235,6 +289,7 @@ int colpass_tile_cols(int rows_per_lane) {
 size_t colpass_lds_bytes(const RowPassArgs& a) {
     const int U = a.colpass_rows > 0 ? a.colpass_rows : 1;
     const int TC = colpass_tile_cols(U);
+    // (paired: U/2 rows per lane and their U/2 partner rows: the same tile)
     const size_t tile = (size_t)2 * 64 * U * (TC + 1) * sizeof(float) + (TC + 2) * sizeof(int);
     if (a.wr_full) return tile + (size_t)a.R * sizeof(float2);
     return tile + ((size_t)(1 << a.hi_shift) + a.n_hi) * sizeof(float2);
@@ -257,17 +312,26 @@ hipError_t launch_colsched(const int* colstart, int cols, int rows_per_lane, int
 
 // The instantiated (rows per lane U, tile columns TC) pairs.
 #define SPFFT_COLPASS_SHAPES(X) X(1, 64) X(2, 32) X(4, 16) X(8, 8) X(16, 8)
+// The paired kernels: output rows per lane U (U/2 rows and their partners), same tiles.
+#define SPFFT_COLPASS_PAIR_SHAPES(X) X(2, 32) X(4, 16) X(8, 8) X(16, 8)
 
 hipError_t colpass_prepare(size_t lds_limit) {
     // raise the dynamic LDS limit of every k_colpass instantiation (the tile is
     // 2 * 64U * (TC + 1) floats: past 64 KB for some shapes)
     const void* ks[] = {
 #define SPFFT_COLPASS_KERNELS(u, tc)                                         \
-    reinterpret_cast<const void*>(&k_colpass<true, true, u, tc>),           \
-        reinterpret_cast<const void*>(&k_colpass<true, false, u, tc>),      \
-        reinterpret_cast<const void*>(&k_colpass<false, true, u, tc>),      \
-        reinterpret_cast<const void*>(&k_colpass<false, false, u, tc>),
+    reinterpret_cast<const void*>(&k_colpass<true, true, u, tc, false>),    \
+        reinterpret_cast<const void*>(&k_colpass<true, false, u, tc, false>),  \
+        reinterpret_cast<const void*>(&k_colpass<false, true, u, tc, false>),  \
+        reinterpret_cast<const void*>(&k_colpass<false, false, u, tc, false>),
         SPFFT_COLPASS_SHAPES(SPFFT_COLPASS_KERNELS)
+#undef SPFFT_COLPASS_KERNELS
+#define SPFFT_COLPASS_KERNELS(u, tc)                                         \
+    reinterpret_cast<const void*>(&k_colpass<true, true, u / 2, tc, true>),     \
+        reinterpret_cast<const void*>(&k_colpass<true, false, u / 2, tc, true>),  \
+        reinterpret_cast<const void*>(&k_colpass<false, true, u / 2, tc, true>),  \
+        reinterpret_cast<const void*>(&k_colpass<false, false, u / 2, tc, true>),
+        SPFFT_COLPASS_PAIR_SHAPES(SPFFT_COLPASS_KERNELS)
 #undef SPFFT_COLPASS_KERNELS
     };
     for (const void* k : ks) {
@@ -281,10 +345,20 @@ template <bool PACKED, bool FULL>
 static void launch_colpass_u(const RowPassArgs& a, hipStream_t s) {
     const int U = a.colpass_rows;
     const int TC = colpass_tile_cols(U);
-    const dim3 gp((a.C + TC - 1) / TC, (a.tile_rows + 64 * U - 1) / (64 * U));
     const size_t lds = colpass_lds_bytes(a);
+    if (a.pair_half) {
+        // rows u <= R/4, U/2 per lane, each also giving row R/2 - u; the tile is rows [0, R/2]
+        const int rows = (a.R >> 2) + 1, per_blk = 64 * (U / 2);
+        const dim3 gp((a.C + TC - 1) / TC, (rows + per_blk - 1) / per_blk);
 #define SPFFT_COLPASS_LAUNCH(u, tc) \
-    if (U == u && TC == tc) hipLaunchKernelGGL((k_colpass<PACKED, FULL, u, tc>), gp, dim3(kColThreads), lds, s, a);
+    if (U == u && TC == tc) hipLaunchKernelGGL((k_colpass<PACKED, FULL, u / 2, tc, true>), gp, dim3(kColThreads), lds, s, a);
+        SPFFT_COLPASS_PAIR_SHAPES(SPFFT_COLPASS_LAUNCH)
+#undef SPFFT_COLPASS_LAUNCH
+        return;
+    }
+    const dim3 gp((a.C + TC - 1) / TC, (a.tile_rows + 64 * U - 1) / (64 * U));
+#define SPFFT_COLPASS_LAUNCH(u, tc) \
+    if (U == u && TC == tc) hipLaunchKernelGGL((k_colpass<PACKED, FULL, u, tc, false>), gp, dim3(kColThreads), lds, s, a);
     SPFFT_COLPASS_SHAPES(SPFFT_COLPASS_LAUNCH)
 #undef SPFFT_COLPASS_LAUNCH
 }

@@ -338,6 +338,14 @@ spfft_status do_csc_build(spfft_plan_s& p, const int* d_row, const int* d_col, i
     return SPFFT_OK;
 }
 
+// Half-period row pairing in the column pass (SPFFT_PAIR_AUTO): R even, one tile
+// holds every computed row [0, R/2], no sampled row limit, and not one row per
+// lane (a paired lane owns U/2 rows and their U/2 partners).
+bool pair_eligible(const spfft_plan_s& p) {
+    return p.opts.colpass_pair == SPFFT_PAIR_AUTO && p.R % 2 == 0 && p.opts.row_limit == 0 &&
+           p.opts.u_tile >= p.R / 2 + 1 && p.base.colpass_rows != 1;
+}
+
 spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t s, float2* T,
                         float2* X0) {
     if (!p.have_csc) return fail(SPFFT_ERR_INVALID_ARG, "no CSC built yet");
@@ -346,8 +354,14 @@ spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t
     RowPassArgs a = p.base;
     a.P = (int)pitch;
     a.colpass_rows = p.colpass_rows;  // resolved with the column schedule at the CSC build
+    const int tile = p.opts.u_tile;
+    const int rows = p.opts.row_limit > 0 ? std::min(p.R / 2 + 1, p.opts.row_limit) : p.R / 2 + 1;
+    // rows u and R/2 - u from one phase lookup: R even, one tile of all rows, and
+    // at least two output rows per lane (half of them partner rows)
+    a.pair_half = pair_eligible(p) && a.colpass_rows >= 2 ? 1 : 0;
     // the whole W_R table only while it fits in LDS next to the row tile
     if (a.wr_full && spfft::colpass_lds_bytes(a) > g_lds_limit) a.wr_full = nullptr;
+
     a.colstart = p.colstart.as<int>();
     a.col_sched = p.col_sched.as<int>();
     a.col_sched_off = p.col_sched_off.as<int>();
@@ -358,8 +372,6 @@ spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t
     if ((!a.T && !a.fused) || !a.X0) return fail(SPFFT_ERR_INVALID_ARG, "plan has no row-pass workspace");
     // Rows u in [0, R/2] are computed; each also writes its mirror row R - u
     // (X[R-u][v] = conj(X[u][(C - v) mod C]) for a real input).
-    const int tile = p.opts.u_tile;
-    const int rows = p.opts.row_limit > 0 ? std::min(p.R / 2 + 1, p.opts.row_limit) : p.R / 2 + 1;
     for (int u0 = 0; u0 < rows; u0 += tile) {
         a.u_base = u0;
         a.tile_rows = std::min(tile, rows - u0);
@@ -557,6 +569,8 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     a.colpass_rows = o.colpass_rows ? o.colpass_rows : 0;  // 0: set from the average degree at CSC build
     if (o.colpass_table < 0 || o.colpass_table > 2)
         return fail(SPFFT_ERR_INVALID_ARG, "colpass_table must be 0, 1 or 2");
+    if (o.colpass_pair != SPFFT_PAIR_AUTO && o.colpass_pair != SPFFT_PAIR_OFF)
+        return fail(SPFFT_ERR_INVALID_ARG, "colpass_pair must be 0 or 1");
     const bool full_fits = (size_t)rows * 8 <= budget;
     if (o.colpass_table == 2 && !full_fits)
         return fail(SPFFT_ERR_UNSUPPORTED, "W_R table does not fit in LDS");
@@ -587,6 +601,7 @@ void fill_info(const spfft_plan_s* p, spfft_plan_info* info) {
     info->colpass_full_table = p->colpass_full ? 1 : 0;
     info->workspace_bytes = p->table_bytes;
     info->row_kernel = p->base.fused ? SPFFT_ROW_FUSED : SPFFT_ROW_FOUR_STEP;
+    info->colpass_paired = pair_eligible(*p) ? 1 : 0;
 }
 
 }  // namespace

@@ -103,6 +103,8 @@ spfft_status do_csc_build(spfft_plan_s& p, const int* d_row, const int* d_col, i
 // T / X0: workspace to use instead of the plan's own (NULL: the plan's own).
 spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t s, float2* T,
                         float2* X0);
+// The column pass pairs rows u and R/2 - u (SPFFT_PAIR_AUTO, R even, one tile of rows).
+bool pair_eligible(const spfft_plan_s& p);
 // Plan without its own row-pass workspace (T, X0) when own_workspace is false.
 spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts, bool own_workspace,
                             spfft_plan_s** out);

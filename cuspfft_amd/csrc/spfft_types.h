@@ -34,6 +34,8 @@ struct RowPassArgs {
     int G2;                    // k_conv: (u, k1) rows per workgroup
     int conv_kind;             // 1: k_conv_fast<f2.radix[0], f2.radix[1]>, 0: k_conv_generic
     int colpass_rows;          // k_colpass: rows per lane (1 .. 16; 64 * rows per workgroup)
+    int pair_half;             // k_colpass: 1 = rows u <= R/4 each also give row R/2 - u (R even,
+                               //   one tile of rows [0, R/2])
     int hi_shift;              // two-level W_R table: W_R^p = hi[p >> s] * lo[p & (2^s - 1)]
     int n_hi;                  // entries in hi table
     float inv_r;               // (1 - 2^-20) / R: a phase quotient estimate that never exceeds

@@ -56,6 +56,7 @@ typedef struct spfft_opts {
     int row_limit;     /* 0: all rows.  Else only output rows u < row_limit (and their mirror
                           rows) are computed: a sampled run for timing (the autotuner) */
     int row_kernel;    /* SPFFT_ROW_*: how the Bluestein row pass runs                       */
+    int colpass_pair;  /* SPFFT_PAIR_*: half-period row pairing in the column pass            */
 } spfft_opts;
 
 /* Row-pass kernels.  FOUR_STEP: k_fft1 + k_conv + k_ifft1 with the tile buffer T
@@ -63,6 +64,14 @@ typedef struct spfft_opts {
  * held in registers and LDS (no T; the M of a fixed list of shapes, FP32 form).
  * AUTO: FUSED when M has a fused shape and the options allow it, else FOUR_STEP. */
 enum { SPFFT_ROW_AUTO = 0, SPFFT_ROW_FOUR_STEP = 1, SPFFT_ROW_FUSED = 2 };
+
+/* Column-pass row pairing.  For even rows R, W_R^{(u + R/2) r} = (-1)^r W_R^{u r}:
+ * one phase lookup per nonzero serves row u (even-r sum + odd-r sum) and row
+ * R/2 - u (the conjugate of their difference), so the pass computes rows
+ * u <= R/4 only.  AUTO: paired when R is even, one tile holds every computed row
+ * (u_tile >= R/2 + 1), row_limit is 0 and colpass_rows is not 1 (a lane's output
+ * rows are then half its own, half their partners).  OFF: every row on its own. */
+enum { SPFFT_PAIR_AUTO = 0, SPFFT_PAIR_OFF = 1 };
 
 typedef struct spfft_plan_info {
     int rows, cols;
@@ -81,6 +90,7 @@ typedef struct spfft_plan_info {
     int precision;          /* SPFFT_PREC_* of the length-m1 sub-FFTs                   */
     size_t workspace_bytes; /* device bytes held by the plan (tables + tile buffer) */
     int row_kernel;         /* SPFFT_ROW_FOUR_STEP or SPFFT_ROW_FUSED                   */
+    int colpass_paired;     /* 1 if the column pass pairs rows u and R/2 - u            */
 } spfft_plan_info;
 
 typedef struct spfft_plan_s* spfft_plan;

@@ -69,6 +69,23 @@ def test_planner_config_choices():
         assert a["conv_fast"] == 1 and b["conv_fast"] == 1
 
 
+def test_planner_colpass_pairing():
+    """Rows u and R/2 - u share the column pass only for even R with one tile of
+    every computed row and no sampled row limit."""
+    import cuspfft_amd as sp
+    assert sp.describe_plan(16706, 16706, u_tile=16706)["colpass_paired"] == 1
+    assert sp.describe_plan(16706, 16706, u_tile=8354)["colpass_paired"] == 1  # = R/2 + 1
+    assert sp.describe_plan(16706, 16706, u_tile=8353)["colpass_paired"] == 0
+    assert sp.describe_plan(8219, 8219, u_tile=8219)["colpass_paired"] == 0   # odd R
+    assert sp.describe_plan(2, 5, u_tile=2)["colpass_paired"] == 1
+    assert sp.describe_plan(2310, 2310, u_tile=2310, row_limit=64)["colpass_paired"] == 0
+    assert sp.describe_plan(2310, 2310, u_tile=2310, colpass_pair=sp.PAIR_OFF)["colpass_paired"] == 0
+    assert sp.describe_plan(2310, 2310, u_tile=2310, colpass_rows=1)["colpass_paired"] == 0
+    assert sp.describe_plan(2310, 2310, u_tile=2310, colpass_rows=2)["colpass_paired"] == 1
+    with pytest.raises(sp.SpfftError):
+        sp.describe_plan(2310, 2310, colpass_pair=2)
+
+
 def test_planner_fused_row_kernel():
     """The planner picks the fused row kernel for the lengths it has a shape for
     (the configurations' 3- and 7-smooth lengths), the four-step kernels

@@ -364,6 +364,58 @@ def test_colpass_variants(gpu, oracle_mod, rows_per_lane, table, shape):
     assert oracle_mod.rel_l2(X, ref) <= REL_L2_PLAIN
 
 
+# Half-period row pairing (even R, one tile of all rows): rows u <= R/4 and R/2 - u
+# from one phase lookup.  R/2 odd and even (the self-paired row u = R/4), the
+# smallest R, and int32 indices (R > 65536).
+@pytest.mark.parametrize("rows_per_lane", [2, 4, 8, 16])
+@pytest.mark.parametrize("table", [1, 2])
+@pytest.mark.parametrize("shape", [(2310, 2310, 16000), (300, 4000, 9000), (2, 7, 9), (4, 3, 7),
+                                   (6, 40, 100), (1026, 513, 5000), (70000, 50, 3000)])
+def test_colpass_pairing(gpu, oracle_mod, rows_per_lane, table, shape):
+    sp = gpu
+    rows, cols, nnz = shape
+    if table == 2 and rows * 8 > 64 * 1024:
+        pytest.skip("whole W_R table only up to 8192 rows")
+    rng = np.random.default_rng(rows * 7 + cols)
+    r, c = _random_coo(rng, rows, cols, nnz)
+    print(f"[pair] {shape} rows/lane={rows_per_lane} table={table}", flush=True)
+    plan, X = _run(sp, rows, cols, r, c, u_tile=rows, colpass_rows=rows_per_lane,
+                   colpass_table=table)
+    assert plan.info["colpass_paired"] == 1
+    _, X0 = _run(sp, rows, cols, r, c, u_tile=rows, colpass_rows=rows_per_lane,
+                 colpass_table=table, colpass_pair=sp.PAIR_OFF)
+    ref = oracle_mod.sparse_rfft2_f64(rows, cols, r, c)
+    err, err0 = oracle_mod.rel_l2(X, ref), oracle_mod.rel_l2(X0, ref)
+    print(f"  rel_l2 paired={err:.3e} unpaired={err0:.3e}")
+    assert err <= REL_L2_PLAIN and err0 <= REL_L2_PLAIN
+    assert oracle_mod.rel_l2(X, X0) <= 1e-6  # two summation orders of the same terms
+    assert abs(X[0, 0] - nnz) <= 1e-5 * nnz
+
+
+def test_full_config_paired(gpu, oracle_mod):
+    """c3 (16706 rows: R/2 = 8353 odd) with the default one-tile plan, which pairs
+    rows u and R/2 - u in the column pass: spot rows on both sides of the pairing
+    boundary R/4 against the FP64 oracle, plus the invariants."""
+    import torch
+    sp = gpu
+    m = sp.config_matrix("c3")
+    print("[fft] c3 paired", flush=True)
+    plan = sp.Plan(m.rows, m.cols, u_tile=m.rows)
+    assert plan.info["colpass_paired"] == 1
+    out = plan.alloc_output()
+    plan.execute(_dev(m.row_idx), _dev(m.col_idx), out)
+    torch.cuda.synchronize()
+    plan.check_input()
+    q, h = m.rows // 4, m.rows // 2
+    rows = np.array([0, 1, q - 1, q, q + 1, h - 1, h, h + 1, m.rows - q, m.rows - 1], np.int32)
+    ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, u_list=rows, nthreads=8)
+    got = out[torch.as_tensor(rows).cuda(), : m.cols // 2 + 1].cpu().numpy()
+    err = oracle_mod.rel_l2(got, ref)
+    print(f"c3 paired spot rel_l2={err:.3e}")
+    assert err <= REL_L2_PLAIN
+    _invariants(out, m)
+
+
 # ---- precision form: BF16x3 MFMA length-16/32 sub-FFTs -------------------------
 REL_L2_BF16X3 = 1e-5
 
