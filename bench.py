@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--streams", type=int, default=4, help="c4: worker streams")
     ap.add_argument("--forms", action="store_true",
                     help="also time the graph-replayed plain and BF16x3 MFMA forms")
+    ap.add_argument("--to-host", type=int, default=0, metavar="TILE",
+                    help="also time the output streamed to pinned host memory with u-tiles of TILE rows")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     return ap.parse_args()
@@ -297,6 +299,28 @@ def main():
         forms["mfma_over_plain"] = round(forms["bf16x3_mfma_graph"]["fft_per_s"]
                                          / forms["plain_fp32_graph"]["fft_per_s"], 3)
 
+    streamed = None
+    if args.to_host:
+        # the output streamed to pinned host memory (device-to-host copies included)
+        splan = sp.Plan(m.rows, m.cols, length_rule=args.rule, u_tile=args.to_host)
+        host = splan.alloc_host_output()
+        for _ in range(args.warmup):
+            splan.execute_to_host(d_row, d_col, host)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            splan.execute_to_host(d_row, d_col, host)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        err = None
+        if not args.no_check:
+            err = oracle.rel_l2(host[torch.as_tensor(rows), :V].numpy(), ref)
+        streamed = {"fft_per_s": round(1.0 / dt, 3), "ms": round(dt * 1e3, 4), "u_tile": args.to_host,
+                    "d2h_gb_per_s": round(m.rows * V * 8 / dt / 1e9, 2),
+                    "device_chunk_bytes": 2 * 2 * min(args.to_host, m.rows // 2 + 1) * splan.pitch * 8,
+                    "rel_l2_vs_fp64": err}
+        del splan, host
+
     cpu = None
     if not args.no_cpu:
         # about 10 s of single-thread work on the host
@@ -338,6 +362,7 @@ def main():
             "avg_launch_ms": round(dom_ms / max(dom_n, 1), 4),
         },
         "forms": forms,
+        "streamed_to_host": streamed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

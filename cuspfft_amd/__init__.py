@@ -114,6 +114,10 @@ def load():
         "spfft_csc_get": (_i, [_vp, _vp, ctypes.POINTER(CscView)]),
         "spfft_rowpass": (_i, [_vp, _vp, ctypes.c_int64, _vp]),
         "spfft_execute": (_i, [_vp, _vp, _vp, _i, _vp, ctypes.c_int64, _vp]),
+        "spfft_execute_to_host": (_i, [_vp, _vp, _vp, _i, _vp, ctypes.c_int64, _vp]),
+        "spfft_rowpass_to_host": (_i, [_vp, _vp, ctypes.c_int64, _vp]),
+        "spfft_run_host_coo_to_host": (_i, [_i, _i, _i, _vp, _vp, ctypes.POINTER(Opts), _vp,
+                                            ctypes.c_int64, ctypes.POINTER(Result)]),
         "spfft_check_input": (_i, [_vp, _vp]),
         "spfft_csc_bluestein_mixed_radix": (_i, [_i, _i, _i, _vp, _vp, _i, ctypes.POINTER(Result)]),
         "spfft_csc_bluestein_cufft_smooth": (_i, [_i, _i, _i, _vp, _vp, _i, ctypes.POINTER(Result)]),
@@ -323,6 +327,30 @@ class Plan:
                                     int(d_row.numel()), _vp(d_out.data_ptr()), int(d_out.shape[1]),
                                     _stream_handle(stream)))
 
+    def alloc_host_output(self):
+        """Pinned host memory [rows, pitch] complex64 for execute_to_host."""
+        import torch
+        return torch.empty((self.rows, self.pitch), dtype=torch.complex64, pin_memory=True)
+
+    def execute_to_host(self, d_row, d_col, h_out, stream=None):
+        """All three stages on device COO with the output streamed to host memory h_out
+        (a contiguous complex64 CPU tensor [rows, >= cols/2+1], pinned for overlap):
+        two device chunks of 2 * u_tile rows, copied on the plan's copy stream while the
+        next tile computes (the reference's streaming drivers,
+        src/sparse_fft_bluestein.cu:912-1145).  Asynchronous on the stream: synchronise
+        before reading h_out."""
+        self._check_coo(d_row, d_col)
+        self._check_host_out(h_out)
+        _check(load().spfft_execute_to_host(self._h, _vp(d_row.data_ptr()), _vp(d_col.data_ptr()),
+                                            int(d_row.numel()), _vp(h_out.data_ptr()),
+                                            int(h_out.shape[1]), _stream_handle(stream)))
+
+    def rowpass_to_host(self, h_out, stream=None):
+        """Stages 2 + 3 of execute_to_host on the CSC of the last csc_build."""
+        self._check_host_out(h_out)
+        _check(load().spfft_rowpass_to_host(self._h, _vp(h_out.data_ptr()), int(h_out.shape[1]),
+                                            _stream_handle(stream)))
+
     def csc_build(self, d_row, d_col, stream=None):
         self._check_coo(d_row, d_col)
         _check(load().spfft_csc_build(self._h, _vp(d_row.data_ptr()), _vp(d_col.data_ptr()),
@@ -367,6 +395,13 @@ class Plan:
                 raise ValueError("COO arrays must be contiguous int32 CUDA tensors")
         if d_row.numel() != d_col.numel():
             raise ValueError("row and col arrays differ in length")
+
+    def _check_host_out(self, h_out):
+        import torch
+        if (h_out.dtype != torch.complex64 or h_out.is_cuda or h_out.dim() != 2
+                or h_out.shape[0] != self.rows or h_out.shape[1] < self.info["out_cols"]
+                or not h_out.is_contiguous()):
+            raise ValueError("host output must be a contiguous complex64 CPU tensor [rows, >= cols/2+1]")
 
     def _check_io(self, d_row, d_col, d_out):
         import torch
@@ -507,12 +542,14 @@ def device_ptr_to_host(ptr, rows: int, pitch: int) -> np.ndarray:
 @dataclass
 class SparseFFTResult:
     """Mirror of the reference's SparseFFTResult (include/sparse_fft.h:7-13);
-    d_output is a torch tensor [rows, output_cols] (pitch P)."""
+    d_output is a torch tensor [rows, output_cols] (pitch P), or None for the
+    streamed variants, whose output is h_output (pinned host tensor, same shape)."""
     d_output: object
     output_cols: int
     ms: float
     mem_bytes: int
     ms_e2e: float = 0.0
+    h_output: object = None
 
 
 def _run_reference_shaped(coo: COOMatrix, u_tile: int, length_rule: int) -> SparseFFTResult:
@@ -571,3 +608,39 @@ def sparse_fft_csc_bluestein_mixed_radix_graph(coo: COOMatrix, u_tile: int = 128
     free1, _ = torch.cuda.mem_get_info()
     ms = e0.elapsed_time(e1)
     return SparseFFTResult(out, plan.pitch, ms, max(free0 - free1, 0), ms)
+
+
+def sparse_fft_csc_stockham_streaming(coo: COOMatrix, u_tile: int = 128,
+                                      fft_backend: int = 0) -> SparseFFTResult:
+    """The transform with the output streamed to pinned host memory: two device
+    chunks of 2 * u_tile rows; ms covers the tile loop with its device-to-host
+    copies.  fft_backend 3: the 7-smooth length (the reference's cuFFT backend),
+    else the 3-smooth one.  Reference include/sparse_fft.h:35-37,
+    src/sparse_fft_bluestein.cu:912-1149."""
+    import torch
+    if u_tile <= 0:
+        raise RuntimeError("CSC tile size must be positive")
+    free0, _ = torch.cuda.mem_get_info()
+    plan = Plan(coo.rows, coo.cols, length_rule=SMOOTH_7 if fft_backend == 3 else SMOOTH_3,
+                u_tile=u_tile)
+    d_row = torch.as_tensor(np.ascontiguousarray(coo.row_idx, np.int32)).cuda()
+    d_col = torch.as_tensor(np.ascontiguousarray(coo.col_idx, np.int32)).cuda()
+    host = plan.alloc_host_output()
+    torch.cuda.synchronize()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record()
+    plan.csc_build(d_row, d_col)
+    e1.record()
+    plan.rowpass_to_host(host)
+    e2.record()
+    e2.synchronize()
+    plan.check_input()
+    free1, _ = torch.cuda.mem_get_info()
+    return SparseFFTResult(None, plan.pitch, e1.elapsed_time(e2), max(free0 - free1, 0),
+                           e0.elapsed_time(e2), host)
+
+
+def sparse_fft_csc_bluestein_cufft_streaming(coo: COOMatrix, u_tile: int = 128) -> SparseFFTResult:
+    """sparse_fft_csc_stockham_streaming at the 7-smooth length (reference
+    include/sparse_fft.h:37)."""
+    return sparse_fft_csc_stockham_streaming(coo, u_tile, 3)

@@ -7,6 +7,8 @@
 //     --csc-mixed-radix   Bluestein at next_3_smooth(2*cols-1)
 //     --csc-mixed-radix-graph  the same, captured into one hipGraph and launched once
 //     --csc-cufft-smooth  Bluestein at next_7_smooth(2*cols-1) (hand-written FFTs, no library)
+//     --csc-stockham-stream        output streamed to pinned host memory (3-smooth length)
+//     --csc-stockham-cufft-stream  the same at the 7-smooth length (reference main.cu:612-628)
 //     --csc-tile N        output rows per tile (default 1024, as the reference)
 //     --repeat N          1 warm-up + N timed runs; median, min, max
 //     --sparse-only       accepted for compatibility (no dense baseline is built)
@@ -80,7 +82,7 @@ void print_bench(const char* label, float ms, size_t mem, int n, float mn, float
 void usage(const char* p) {
     fprintf(stderr,
             "Usage: %s <matrix.mtx | gen:c1|c2|c3|c5> [--csc-mixed-radix] [--csc-mixed-radix-graph]\n"
-            "          [--csc-cufft-smooth]\n"
+            "          [--csc-cufft-smooth] [--csc-stockham-stream] [--csc-stockham-cufft-stream]\n"
             "          [--csc-tile N] [--repeat N] [--sparse-only] [--cpu-reference]\n",
             p);
 }
@@ -91,6 +93,7 @@ int main(int argc, char** argv) {
     if (argc < 2) { usage(argv[0]); return 1; }
     if (!std::strcmp(argv[1], "--help") || !std::strcmp(argv[1], "-h")) { usage(argv[0]); return 0; }
     bool mixed = false, mixed_graph = false, smooth = false, cpu_ref = false;
+    bool stream3 = false, stream7 = false;
     int tile = 1024, repeat = 1;
     try {
         for (int i = 2; i < argc; i++) {
@@ -99,6 +102,8 @@ int main(int argc, char** argv) {
             else if (f == "--csc-mixed-radix") mixed = true;
             else if (f == "--csc-mixed-radix-graph") mixed_graph = true;
             else if (f == "--csc-cufft-smooth") smooth = true;
+            else if (f == "--csc-stockham-stream") stream3 = true;
+            else if (f == "--csc-stockham-cufft-stream") stream7 = true;
             else if (f == "--cpu-reference") cpu_ref = true;
             else if (f == "--csc-tile") {
                 if (++i >= argc) throw std::runtime_error("--csc-tile requires an integer value");
@@ -124,7 +129,9 @@ int main(int argc, char** argv) {
         struct V { bool on; const char* label; int kind; };
         const V vs[] = {{mixed, "Sparse CSC (mixed-radix)", 0},
                         {mixed_graph, "Sparse CSC (mr-graph)", 1},
-                        {smooth, "Sparse CSC (cufft smooth)", 2}};
+                        {smooth, "Sparse CSC (cufft smooth)", 2},
+                        {stream3, "Sparse CSC (Stockham stream)", 3},
+                        {stream7, "Sparse CSC (cufft stream)", 4}};
         for (const V& v : vs) {
             if (!v.on) continue;
             try {
@@ -132,13 +139,16 @@ int main(int argc, char** argv) {
                 SparseFFTResult r = run_repeated(
                     repeat,
                     [&] {
+                        if (v.kind == 4) return sparse_fft_csc_bluestein_cufft_streaming(coo, tile);
+                        if (v.kind == 3) return sparse_fft_csc_stockham_streaming(coo, tile, 0);
                         if (v.kind == 2) return sparse_fft_csc_bluestein_cufft_smooth(coo, tile);
                         if (v.kind == 1) return sparse_fft_csc_bluestein_mixed_radix_graph(coo, tile);
                         return sparse_fft_csc_bluestein_mixed_radix(coo, tile);
                     },
                     &mn, &mx);
                 print_bench(v.label, r.ms, r.mem_bytes, repeat, mn, mx);
-                printf("  rate %-20s  %.2f FFTs/s (tile loop)\n", v.label, 1000.0 / r.ms);
+                printf("  rate %-20s  %.2f FFTs/s (tile loop%s)\n", v.label, 1000.0 / r.ms,
+                       v.kind >= 3 ? " with device-to-host copies" : "");
                 (void)hipFree(r.d_output);
             } catch (const std::exception& e) {
                 printf("%-26s  %10s  %12s  [%s]\n", v.label, "OOM", "OOM", e.what());

@@ -99,8 +99,8 @@ __global__ __launch_bounds__(kThreads, 2) void k_ifft1(RowPassArgs a) {
     const bio::rsrc_t r_h = bio::make_rsrc(a.h, (uint32_t)a.C * 8u);
     const bio::rsrc_t r_o = bio::make_rsrc(a.out + (size_t)u * a.P, (uint32_t)a.V * 8u);
     const bio::rsrc_t r_m =
-        bio::make_rsrc(a.out + (size_t)(mirror ? a.R - u : u) * a.P - dm, mirror ? (uint32_t)(dm + a.V) * 8u : 0u);
-    const bio::rsrc_t r_m0 = bio::make_rsrc(a.out + (size_t)(mirror ? a.R - u : u) * a.P, mirror ? 8u : 0u);
+        bio::make_rsrc(mirror ? a.out_m + (size_t)(a.R - u) * a.P - dm : a.out, mirror ? (uint32_t)(dm + a.V) * 8u : 0u);
+    const bio::rsrc_t r_m0 = bio::make_rsrc(mirror ? a.out_m + (size_t)(a.R - u) * a.P : a.out, mirror ? 8u : 0u);
     const int n1c = lane_ok ? (a.C - n2 + M2 - 1) / M2 : 0;  // columns w = n2 + M2 n1 < C
     const int vm0 = (M2 - 1 - n2) * 8;
 #pragma unroll

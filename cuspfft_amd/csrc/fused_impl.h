@@ -260,9 +260,10 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
         const bool mirror = u > 0 && a.R - u > u;
         const int dm = a.M - 1 - a.C;
         const bio::rsrc_t r_o = bio::make_rsrc(a.out + (size_t)u * a.P, (uint32_t)a.V * 8u);
-        const bio::rsrc_t r_m = bio::make_rsrc(a.out + (size_t)(a.R - u) * a.P - dm,
+        const bio::rsrc_t r_m = bio::make_rsrc(mirror ? a.out_m + (size_t)(a.R - u) * a.P - dm : a.out,
                                                    mirror ? (uint32_t)(dm + a.V) * 8u : 0u);
-        const bio::rsrc_t r_m0 = bio::make_rsrc(a.out + (size_t)(a.R - u) * a.P, mirror ? 8u : 0u);
+        const bio::rsrc_t r_m0 = bio::make_rsrc(mirror ? a.out_m + (size_t)(a.R - u) * a.P : a.out,
+                                                    mirror ? 8u : 0u);
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
             if (live[c]) {
@@ -289,7 +290,7 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
         const int nrows = min(RPW, a.tile_rows - row0);
         const int mlo = a.R - (u0 + nrows - 1);
         const bio::rsrc_t r_o = bio::make_rsrc(a.out + (size_t)u0 * a.P, (uint32_t)(nrows * a.P) * 8u);
-        const bio::rsrc_t r_m = bio::make_rsrc(a.out + (size_t)mlo * a.P, (uint32_t)(nrows * a.P) * 8u);
+        const bio::rsrc_t r_m = bio::make_rsrc(a.out_m + (size_t)mlo * a.P, (uint32_t)(nrows * a.P) * 8u);
         if (live[0]) {
             const float2* col = buf + g[0] * KB * M2 + n2[0];
 #pragma unroll

@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kThreads) void k_ifft1_mfma(RowPassArgs a) {
     const int u = a.u_base + ul;
     const bool mirror = u > 0 && a.R - u > u;
     float2* o = a.out + (size_t)u * a.P;
-    float2* om = a.out + (size_t)(mirror ? a.R - u : u) * a.P;
+    float2* om = mirror ? a.out_m + (size_t)(a.R - u) * a.P : a.out + (size_t)u * a.P;
     using OM = OutMap<L>;
 #pragma unroll
     for (int t = 0; t < OM::RE_T; t++)

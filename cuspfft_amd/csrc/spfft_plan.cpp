@@ -346,16 +346,16 @@ bool pair_eligible(const spfft_plan_s& p) {
            p.opts.u_tile >= p.R / 2 + 1 && p.base.colpass_rows != 1;
 }
 
-spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t s, float2* T,
-                        float2* X0) {
+// The plan's row-pass arguments for output pitch `pitch` (the output rows and the
+// tile range are set per tile); *rows: the computed rows u < *rows.
+static spfft_status rowpass_setup(spfft_plan_s& p, int64_t pitch, float2* T, float2* X0, RowPassArgs& a,
+                                  int* rows_out) {
     if (!p.have_csc) return fail(SPFFT_ERR_INVALID_ARG, "no CSC built yet");
-    if (!d_out) return fail(SPFFT_ERR_INVALID_ARG, "NULL output pointer");
     if (pitch < p.V) return fail(SPFFT_ERR_INVALID_ARG, "output pitch smaller than cols/2+1");
-    RowPassArgs a = p.base;
+    a = p.base;
     a.P = (int)pitch;
     a.colpass_rows = p.colpass_rows;  // resolved with the column schedule at the CSC build
-    const int tile = p.opts.u_tile;
-    const int rows = p.opts.row_limit > 0 ? std::min(p.R / 2 + 1, p.opts.row_limit) : p.R / 2 + 1;
+    *rows_out = p.opts.row_limit > 0 ? std::min(p.R / 2 + 1, p.opts.row_limit) : p.R / 2 + 1;
     // rows u and R/2 - u from one phase lookup: R even, one tile of all rows, and
     // at least two output rows per lane (half of them partner rows)
     a.pair_half = pair_eligible(p) && a.colpass_rows >= 2 ? 1 : 0;
@@ -366,24 +366,95 @@ spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t
     a.col_sched = p.col_sched.as<int>();
     a.col_sched_off = p.col_sched_off.as<int>();
     a.row_idx = p.row_idx.p;
-    a.out = static_cast<float2*>(d_out);
     if (T) a.T = T;
     if (X0) a.X0 = X0;
     if ((!a.T && !a.fused) || !a.X0) return fail(SPFFT_ERR_INVALID_ARG, "plan has no row-pass workspace");
+    return SPFFT_OK;
+}
+
+// one row-pass tile (rows u0 .. u0 + n - 1 and their mirror rows), with the
+// plan's profiling events when enabled
+static spfft_status rowpass_tile(spfft_plan_s& p, RowPassArgs& a, int u0, int n, hipStream_t s) {
+    a.u_base = u0;
+    a.tile_rows = n;
+    hipEvent_t* ev = nullptr;
+    if (p.profiling) {
+        const size_t at = p.ev_used;
+        ev = p.take_events(5);
+        if (!ev) return fail(SPFFT_ERR_HIP, "hipEventCreate failed");
+        p.ev_groups.push_back({1, at});
+    }
+    SPFFT_HIP(spfft::launch_rowpass_tile(a, p.packed, s, ev), "row pass launch");
+    return SPFFT_OK;
+}
+
+spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t s, float2* T,
+                        float2* X0) {
+    if (!d_out) return fail(SPFFT_ERR_INVALID_ARG, "NULL output pointer");
+    RowPassArgs a;
+    int rows = 0;
+    spfft_status st = rowpass_setup(p, pitch, T, X0, a, &rows);
+    if (st != SPFFT_OK) return st;
+    a.out = a.out_m = static_cast<float2*>(d_out);
     // Rows u in [0, R/2] are computed; each also writes its mirror row R - u
     // (X[R-u][v] = conj(X[u][(C - v) mod C]) for a real input).
-    for (int u0 = 0; u0 < rows; u0 += tile) {
-        a.u_base = u0;
-        a.tile_rows = std::min(tile, rows - u0);
-        hipEvent_t* ev = nullptr;
-        if (p.profiling) {
-            const size_t at = p.ev_used;
-            ev = p.take_events(5);
-            if (!ev) return fail(SPFFT_ERR_HIP, "hipEventCreate failed");
-            p.ev_groups.push_back({1, at});
-        }
-        SPFFT_HIP(spfft::launch_rowpass_tile(a, p.packed, s, ev), "row pass launch");
+    for (int u0 = 0; u0 < rows; u0 += p.opts.u_tile)
+        if ((st = rowpass_tile(p, a, u0, std::min(p.opts.u_tile, rows - u0), s)) != SPFFT_OK) return st;
+    return SPFFT_OK;
+}
+
+// p + e elements as a device address (e may leave the allocation: a kernel only
+// adds offsets that land inside it)
+static float2* elem_shift(float2* p, int64_t e) {
+    return reinterpret_cast<float2*>(reinterpret_cast<uintptr_t>(p) + (uintptr_t)(e * (int64_t)sizeof(float2)));
+}
+
+// Streamed output, after the reference's double-buffered streaming drivers
+// (src/sparse_fft_bluestein.cu:912-1145): tile k of computed rows u0 .. u0+n-1
+// writes chunk k % 2 on the caller's stream -- rows u at chunk row u - u0, their
+// mirror rows R - u (u in [1, (R-1)/2]) at chunk row n + (u0 + n - 1 - u), so both
+// groups are contiguous -- and the copy stream moves them to h_out while the next
+// tile computes.  A chunk is rewritten only after its copies are done.
+spfft_status do_rowpass_to_host(spfft_plan_s& p, void* h_out, int64_t pitch, hipStream_t s) {
+    if (!h_out) return fail(SPFFT_ERR_INVALID_ARG, "NULL host output pointer");
+    if (pitch < p.V) return fail(SPFFT_ERR_INVALID_ARG, "output pitch smaller than cols/2+1");
+    RowPassArgs a;
+    int rows = 0;
+    spfft_status st = rowpass_setup(p, p.P, nullptr, nullptr, a, &rows);
+    if (st != SPFFT_OK) return st;
+    const int tile = p.opts.u_tile;
+    const size_t chunk_bytes = (size_t)2 * std::min(tile, rows) * p.P * sizeof(float2);
+    for (DevBuf& c : p.chunk)
+        if (c.bytes < chunk_bytes) SPFFT_HIP(c.alloc(chunk_bytes), "hipMalloc(output chunk)");
+    if (!p.copy_stream) SPFFT_HIP(hipStreamCreateWithFlags(&p.copy_stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (hipEvent_t* e : {&p.chunk_ready[0], &p.chunk_ready[1], &p.chunk_free[0], &p.chunk_free[1], &p.copies_done})
+        if (!*e) SPFFT_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate");
+    float2* host = static_cast<float2*>(h_out);
+    const size_t hp = (size_t)pitch * sizeof(float2), dp = (size_t)p.P * sizeof(float2);
+    const size_t width = (size_t)p.V * sizeof(float2);
+    int k = 0;
+    for (int u0 = 0; u0 < rows; u0 += tile, k++) {
+        const int n = std::min(tile, rows - u0), b = k & 1;
+        float2* ch = p.chunk[b].as<float2>();
+        if (k >= 2) SPFFT_HIP(hipStreamWaitEvent(s, p.chunk_free[b], 0), "hipStreamWaitEvent");
+        a.out = elem_shift(ch, -(int64_t)u0 * p.P);
+        a.out_m = elem_shift(ch, (int64_t)(2 * n + u0 - 1 - p.R) * p.P);
+        if ((st = rowpass_tile(p, a, u0, n, s)) != SPFFT_OK) return st;
+        SPFFT_HIP(hipEventRecord(p.chunk_ready[b], s), "hipEventRecord");
+        SPFFT_HIP(hipStreamWaitEvent(p.copy_stream, p.chunk_ready[b], 0), "hipStreamWaitEvent");
+        SPFFT_HIP(hipMemcpy2DAsync(host + (size_t)u0 * pitch, hp, ch, dp, width, n, hipMemcpyDeviceToHost,
+                                   p.copy_stream),
+                  "hipMemcpy2DAsync(rows)");
+        const int lo = std::max(u0, 1), hi = std::min(u0 + n - 1, (p.R - 1) / 2);
+        if (lo <= hi)
+            SPFFT_HIP(hipMemcpy2DAsync(host + (size_t)(p.R - hi) * pitch, hp, ch + (size_t)(2 * n + u0 - 1 - hi) * p.P,
+                                       dp, width, hi - lo + 1, hipMemcpyDeviceToHost, p.copy_stream),
+                      "hipMemcpy2DAsync(mirror rows)");
+        SPFFT_HIP(hipEventRecord(p.chunk_free[b], p.copy_stream), "hipEventRecord");
     }
+    // work queued on the caller's stream after this call sees the copies done
+    SPFFT_HIP(hipEventRecord(p.copies_done, p.copy_stream), "hipEventRecord");
+    SPFFT_HIP(hipStreamWaitEvent(s, p.copies_done, 0), "hipStreamWaitEvent");
     return SPFFT_OK;
 }
 
@@ -814,6 +885,21 @@ spfft_status spfft_execute(spfft_plan p, const int* d_row, const int* d_col, int
     return do_rowpass(*p, d_out, pitch, (hipStream_t)stream, nullptr, nullptr);
 }
 
+spfft_status spfft_rowpass_to_host(spfft_plan p, void* h_out, int64_t pitch, void* stream) {
+    if (!p) return fail(SPFFT_ERR_INVALID_ARG, "NULL plan");
+    return do_rowpass_to_host(*p, h_out, pitch, (hipStream_t)stream);
+}
+
+spfft_status spfft_execute_to_host(spfft_plan p, const int* d_row, const int* d_col, int nnz, void* h_out,
+                                   int64_t pitch, void* stream) {
+    if (!p) return fail(SPFFT_ERR_INVALID_ARG, "NULL plan");
+    if (!h_out) return fail(SPFFT_ERR_INVALID_ARG, "NULL host output pointer");
+    if (pitch < p->V) return fail(SPFFT_ERR_INVALID_ARG, "output pitch smaller than cols/2+1");
+    spfft_status st = do_csc_build(*p, d_row, d_col, nnz, (hipStream_t)stream);
+    if (st != SPFFT_OK) return st;
+    return do_rowpass_to_host(*p, h_out, pitch, (hipStream_t)stream);
+}
+
 spfft_status spfft_profile_enable(spfft_plan p, int on) {
     if (!p) return fail(SPFFT_ERR_INVALID_ARG, "NULL plan");
     p->profiling = on != 0;
@@ -851,8 +937,10 @@ spfft_status spfft_free_output(void* d) {
     return SPFFT_OK;
 }
 
-spfft_status spfft_run_host_coo(int rows, int cols, int nnz, const int* h_row, const int* h_col,
-                                const spfft_opts* opts, spfft_result* res) {
+// Host COO in; the output in device memory (h_out NULL: res->d_output, the caller
+// frees it) or streamed to host memory h_out at pitch h_pitch.
+static spfft_status run_host_coo(int rows, int cols, int nnz, const int* h_row, const int* h_col,
+                                 const spfft_opts* opts, void* h_out, int64_t h_pitch, spfft_result* res) {
     if (!res) return fail(SPFFT_ERR_INVALID_ARG, "NULL result");
     std::memset(res, 0, sizeof(*res));
     spfft_opts o;
@@ -875,7 +963,7 @@ spfft_status spfft_run_host_coo(int rows, int cols, int nnz, const int* h_row, c
         SPFFT_HIP(hipMemcpy(drow.p, h_row, (size_t)nnz * 4, hipMemcpyHostToDevice), "hipMemcpy(COO)");
         SPFFT_HIP(hipMemcpy(dcol.p, h_col, (size_t)nnz * 4, hipMemcpyHostToDevice), "hipMemcpy(COO)");
     }
-    SPFFT_HIP(dout.alloc((size_t)rows * p->P * sizeof(float2)), "hipMalloc(output)");
+    if (!h_out) SPFFT_HIP(dout.alloc((size_t)rows * p->P * sizeof(float2)), "hipMalloc(output)");
     st = ensure_csc_capacity(*p, nnz);
     if (st != SPFFT_OK) return st;
     SPFFT_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
@@ -891,7 +979,7 @@ spfft_status spfft_run_host_coo(int rows, int cols, int nnz, const int* h_row, c
     st = do_csc_build(*p, drow.as<int>(), dcol.as<int>(), nnz, 0);
     if (st != SPFFT_OK) return st;
     SPFFT_HIP(hipEventRecord(e1, 0), "hipEventRecord");
-    st = do_rowpass(*p, dout.p, p->P, 0, nullptr, nullptr);
+    st = h_out ? do_rowpass_to_host(*p, h_out, h_pitch, 0) : do_rowpass(*p, dout.p, p->P, 0, nullptr, nullptr);
     if (st != SPFFT_OK) return st;
     SPFFT_HIP(hipEventRecord(e2, 0), "hipEventRecord");
     SPFFT_HIP(hipEventSynchronize(e2), "hipEventSynchronize");
@@ -902,13 +990,28 @@ spfft_status spfft_run_host_coo(int rows, int cols, int nnz, const int* h_row, c
     if (st != SPFFT_OK) return st;
     size_t free1 = 0;
     SPFFT_HIP(hipMemGetInfo(&free1, &total), "hipMemGetInfo");
-    res->d_output = dout.p;
-    dout.p = nullptr;  // ownership passes to the caller
-    res->output_cols = p->P;
+    res->d_output = dout.p;  // NULL when streamed
+    dout.p = nullptr;        // ownership passes to the caller
+    res->output_cols = h_out ? (int)h_pitch : p->P;
     res->ms = ms_loop;
     res->ms_e2e = ms_all;
     res->mem_bytes = free1 < free0 ? free0 - free1 : 0;
     return SPFFT_OK;
+}
+
+spfft_status spfft_run_host_coo(int rows, int cols, int nnz, const int* h_row, const int* h_col,
+                                const spfft_opts* opts, spfft_result* res) {
+    return run_host_coo(rows, cols, nnz, h_row, h_col, opts, nullptr, 0, res);
+}
+
+spfft_status spfft_run_host_coo_to_host(int rows, int cols, int nnz, const int* h_row, const int* h_col,
+                                        const spfft_opts* opts, void* h_out, int64_t pitch,
+                                        spfft_result* res) {
+    if (!h_out) {
+        if (res) std::memset(res, 0, sizeof(*res));
+        return fail(SPFFT_ERR_INVALID_ARG, "NULL host output pointer");
+    }
+    return run_host_coo(rows, cols, nnz, h_row, h_col, opts, h_out, pitch, res);
 }
 
 spfft_status spfft_csc_bluestein_mixed_radix(int rows, int cols, int nnz, const int* h_row,

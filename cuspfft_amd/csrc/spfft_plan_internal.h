@@ -80,8 +80,18 @@ struct spfft_plan_s {
     // event groups recorded since the last read: kind 0 = CSC build (2 events),
     // kind 1 = row-pass tile (4 events)
     std::vector<std::pair<int, size_t>> ev_groups;
+    // streamed output (spfft_execute_to_host): two device chunks of 2 * u_tile rows
+    // (a tile's rows and their mirror rows), a copy stream, and per chunk an event
+    // "written" (compute stream) and "copied" (copy stream)
+    DevBuf chunk[2];
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t chunk_ready[2] = {nullptr, nullptr}, chunk_free[2] = {nullptr, nullptr};
+    hipEvent_t copies_done = nullptr;
     ~spfft_plan_s() {
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+        for (hipEvent_t e : {chunk_ready[0], chunk_ready[1], chunk_free[0], chunk_free[1], copies_done})
+            if (e) (void)hipEventDestroy(e);
+        if (copy_stream) (void)hipStreamDestroy(copy_stream);
     }
     hipEvent_t* take_events(int n) {
         while (ev_pool.size() < ev_used + n) {
@@ -103,6 +113,8 @@ spfft_status do_csc_build(spfft_plan_s& p, const int* d_row, const int* d_col, i
 // T / X0: workspace to use instead of the plan's own (NULL: the plan's own).
 spfft_status do_rowpass(spfft_plan_s& p, void* d_out, int64_t pitch, hipStream_t s, float2* T,
                         float2* X0);
+// The row pass with the output streamed to host memory h_out (pitch elements per row).
+spfft_status do_rowpass_to_host(spfft_plan_s& p, void* h_out, int64_t pitch, hipStream_t s);
 // The column pass pairs rows u and R/2 - u (SPFFT_PAIR_AUTO, R even, one tile of rows).
 bool pair_eligible(const spfft_plan_s& p);
 // Plan without its own row-pass workspace (T, X0) when own_workspace is false.

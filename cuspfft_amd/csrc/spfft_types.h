@@ -61,7 +61,9 @@ struct RowPassArgs {
     int fused_nt;              // k_rowfused: threads per workgroup
     float2* X0;                // tile pass-1 output [u_local][ldx]
     float2* T;                 // tile workspace [u_local][k1][n2]
-    float2* out;               // output base (row 0), pitch P
+    float2* out;               // output base (row 0), pitch P: row u at out + u P
+    float2* out_m;             // mirror rows: row R - u at out_m + (R - u) P (= out, or a
+                               //   streamed tile's chunk)
 };
 
 }  // namespace spfft

@@ -12,7 +12,7 @@ struct SparseFFTResult {
     int output_cols;            // row pitch P = 4 * ceil((cols/2 + 1) / 4)
     float ms;                   // GPU time of the tile loop (the reference's timed region)
     size_t mem_bytes;           // device free-memory delta over the call
-    std::vector<hipFloatComplex> h_output;  // unused (no streamed variant)
+    std::vector<hipFloatComplex> h_output;  // streamed variants: rows * output_cols, host memory
 };
 
 // Bluestein length next_3_smooth(2*cols - 1).  Replaces
@@ -29,3 +29,14 @@ SparseFFTResult sparse_fft_csc_bluestein_mixed_radix_graph(const COOMatrix& coo,
 // sub-FFTs in place of cuFFT.  Replaces sparse_fft_csc_bluestein_cufft_smooth
 // (reference src/sparse_fft_bluestein.cu:2557-2719).
 SparseFFTResult sparse_fft_csc_bluestein_cufft_smooth(const COOMatrix& coo, int u_tile = 128);
+
+// Streamed output: only two chunks of 2 * u_tile output rows on the device; every
+// tile's rows go to pinned host memory on a copy stream while the next tile
+// computes.  d_output is NULL and h_output holds rows * output_cols entries; ms
+// includes the copies.  fft_backend 3 takes the 7-smooth Bluestein length (the
+// reference's cuFFT backend), any other value the 3-smooth one (its Stockham
+// kernels).  Replaces sparse_fft_csc_stockham_streaming and
+// sparse_fft_csc_bluestein_cufft_streaming (reference include/sparse_fft.h:35-37,
+// src/sparse_fft_bluestein.cu:912-1149).
+SparseFFTResult sparse_fft_csc_stockham_streaming(const COOMatrix& coo, int u_tile = 128, int fft_backend = 0);
+SparseFFTResult sparse_fft_csc_bluestein_cufft_streaming(const COOMatrix& coo, int u_tile = 128);

@@ -143,6 +143,18 @@ spfft_status spfft_rowpass(spfft_plan plan, void* d_out, int64_t out_pitch, void
 /* All three stages, asynchronous on `stream`.  Inputs and output in device memory. */
 spfft_status spfft_execute(spfft_plan plan, const int* d_row, const int* d_col, int nnz, void* d_out,
                            int64_t out_pitch, void* stream);
+/* All three stages with the output streamed to host memory h_out (rows x out_pitch
+ * complex64; pinned memory from hipHostMalloc lets the copies overlap the compute).
+ * Only two chunks of 2 * u_tile output rows are held on the device, so outputs
+ * larger than device memory can be produced: each tile's rows and mirror rows are
+ * copied on a plan-owned copy stream while the next tile computes.  Asynchronous
+ * on `stream`: synchronise it before reading h_out.  Replaces the streaming
+ * variants sparse_fft_csc_stockham_streaming / sparse_fft_csc_bluestein_cufft_streaming
+ * (reference include/sparse_fft.h:35-37, src/sparse_fft_bluestein.cu:912-1145). */
+spfft_status spfft_execute_to_host(spfft_plan plan, const int* d_row, const int* d_col, int nnz,
+                                   void* h_out, int64_t out_pitch, void* stream);
+/* Stages 2 + 3 of spfft_execute_to_host on the CSC of the last spfft_csc_build. */
+spfft_status spfft_rowpass_to_host(spfft_plan plan, void* h_out, int64_t out_pitch, void* stream);
 /* Synchronises `stream` and reports SPFFT_ERR_BAD_INPUT if the last CSC build
  * saw an index outside the matrix. */
 spfft_status spfft_check_input(spfft_plan plan, void* stream);
@@ -175,6 +187,14 @@ spfft_status spfft_csc_bluestein_cufft_smooth(int rows, int cols, int nnz, const
 /* General form of the two above (any opts). */
 spfft_status spfft_run_host_coo(int rows, int cols, int nnz, const int* h_row, const int* h_col,
                                 const spfft_opts* opts, spfft_result* result);
+/* The streamed form of spfft_run_host_coo (spfft_execute_to_host): the output goes
+ * to the caller's host buffer h_out (rows x out_pitch complex64, pinned for
+ * overlap); result->d_output is NULL, result->output_cols = out_pitch, and ms
+ * includes the device-to-host copies, as the reference's streaming drivers time
+ * (src/sparse_fft_bluestein.cu:1035-1116). */
+spfft_status spfft_run_host_coo_to_host(int rows, int cols, int nnz, const int* h_row, const int* h_col,
+                                        const spfft_opts* opts, void* h_out, int64_t out_pitch,
+                                        spfft_result* result);
 spfft_status spfft_free_output(void* d_output);
 
 /* ---- hipGraph form ------------------------------------------------------

@@ -526,15 +526,62 @@ def test_cli_reference_flags(gpu):
     sp = gpu
     cli = os.path.join(os.path.dirname(sp.LIB_PATH), "spfft_cli")
     res = subprocess.run([cli, "gen:c1", "--csc-mixed-radix", "--csc-mixed-radix-graph",
-                          "--csc-cufft-smooth", "--csc-tile", "256", "--repeat", "3"],
+                          "--csc-cufft-smooth", "--csc-stockham-stream", "--csc-stockham-cufft-stream",
+                          "--csc-tile", "256", "--repeat", "3"],
                          capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stdout + res.stderr
-    for label in ("Sparse CSC (mixed-radix)", "Sparse CSC (mr-graph)", "Sparse CSC (cufft smooth)"):
+    for label in ("Sparse CSC (mixed-radix)", "Sparse CSC (mr-graph)", "Sparse CSC (cufft smooth)",
+                  "Sparse CSC (Stockham stream)", "Sparse CSC (cufft stream)"):
         assert label in res.stdout
     assert "OOM" not in res.stdout
     bad = subprocess.run([cli, "gen:c1", "--csc-mixed-radix", "--csc-tile", "0"],
                          capture_output=True, text=True, timeout=60)
     assert bad.returncode == 1 and "--csc-tile must be positive" in bad.stderr
+
+
+# ---- streamed output: device chunks copied to pinned host memory --------------
+STREAM_CASES = [((2310, 2310, 16000), 128, {}), ((2310, 2310, 16000), 2310, {}),
+                ((2310, 2310, 16000), 500, dict(row_kernel=1)),
+                ((2310, 2310, 16000), 300, dict(precision=1)),
+                ((7, 9, 30), 1, {}), ((8, 9, 30), 3, {}), ((2, 5, 6), 1, {}), ((1, 5, 3), 4, {}),
+                ((301, 1000, 5000), 50, {}), ((300, 4000, 9000), 151, {}), ((300, 4000, 9000), 77, {})]
+
+
+@pytest.mark.parametrize("shape,tile,kw", STREAM_CASES)
+def test_streamed_output(gpu, oracle_mod, shape, tile, kw):
+    """execute_to_host gives exactly the device-output transform: every row (tile
+    rows and mirror rows, host buffer pre-filled with NaN) on two calls of one plan."""
+    import torch
+    sp = gpu
+    rows, cols, nnz = shape
+    rng = np.random.default_rng(rows + 3 * cols + tile)
+    r, c = _random_coo(rng, rows, cols, nnz)
+    print(f"[stream] {shape} tile={tile} {kw}", flush=True)
+    plan, X = _run(sp, rows, cols, r, c, u_tile=tile, **kw)
+    V = cols // 2 + 1
+    dr, dc = _dev(r), _dev(c)
+    for _ in range(2):
+        host = plan.alloc_host_output()
+        host.fill_(complex(float("nan"), float("nan")))
+        plan.execute_to_host(dr, dc, host)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(host[:, :V].numpy(), X)
+    ref = oracle_mod.sparse_rfft2_f64(rows, cols, r, c)
+    assert oracle_mod.rel_l2(X, ref) <= (REL_L2_BF16X3 if kw.get("precision") else REL_L2_PLAIN)
+
+
+def test_streaming_entry_points(gpu, oracle_mod):
+    """The reference-shaped streamed variants: h_output set, d_output None, same result."""
+    sp = gpu
+    m = sp.config_matrix("c1")
+    V = m.cols // 2 + 1
+    ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, nthreads=8)
+    for fn in (sp.sparse_fft_csc_stockham_streaming, sp.sparse_fft_csc_bluestein_cufft_streaming):
+        res = fn(m, 200)
+        assert res.d_output is None and res.output_cols == (V + 3) // 4 * 4 and res.ms > 0
+        assert oracle_mod.rel_l2(res.h_output[:, :V].numpy(), ref) <= REL_L2_PLAIN
+    with pytest.raises(RuntimeError):
+        sp.sparse_fft_csc_stockham_streaming(m, 0)
 
 
 def _batch_mats(n, seed):
