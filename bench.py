@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--colpass-table", type=int, default=0)
     ap.add_argument("--colpass-pair", type=int, default=0,
                     help="0 auto (rows u and R/2 - u share the column pass when R is even), 1 off")
+    ap.add_argument("--row-transform", type=int, default=0,
+                    help="0 auto (direct length-cols FFT when cols has a shape), 1 Bluestein, 2 direct")
     ap.add_argument("--row-kernel", type=int, default=0,
                     help="0 auto (fused row kernel when M has a shape), 1 four-step kernels, 2 fused")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -223,7 +225,7 @@ def main():
     plan = sp.Plan(m.rows, m.cols, length_rule=args.rule, u_tile=tile, m1=args.m1, fft_len=args.fft_len,
                    precision=args.precision, row_kernel=args.row_kernel,
                    colpass_rows=args.colpass_rows, colpass_table=args.colpass_table,
-                   colpass_pair=args.colpass_pair)
+                   colpass_pair=args.colpass_pair, row_transform=args.row_transform)
     d_row = torch.as_tensor(m.row_idx).to(dev)
     d_col = torch.as_tensor(m.col_idx).to(dev)
     out = plan.alloc_output()
@@ -265,7 +267,7 @@ def main():
             rec = json.load(f).get(args.config, {}).get(dom)
         default_plan = (args.tile <= 0 and args.fft_len == 0 and args.m1 == 0 and args.precision == 0
                         and args.rule == 3 and args.colpass_rows == 0 and args.colpass_table == 0
-                        and args.row_kernel == 0 and args.colpass_pair == 0)
+                        and args.row_kernel == 0 and args.colpass_pair == 0 and args.row_transform == 0)
         if rec and default_plan:
             traffic = rec["bytes"]
     except (OSError, ValueError):
@@ -325,7 +327,7 @@ def main():
     if not args.no_cpu:
         # about 10 s of single-thread work on the host
         n = args.cpu_rows if args.cpu_rows > 0 else {"c1": 2310, "c2": 4000, "c3": 3000,
-                                                     "c5": 800}.get(args.config, 2000)
+                                                     "c5": 800, "p14": 3000}.get(args.config, 2000)
         cpu = cpu_baseline(m, args.rule, n)
 
     info = plan.info
@@ -352,6 +354,7 @@ def main():
             "colpass_full_table": info["colpass_full_table"],
             "colpass_paired": info["colpass_paired"],
             "row_kernel": {1: "four-step", 2: "fused"}[info["row_kernel"]],
+            "row_transform": "direct" if info["direct"] else "bluestein",
         },
         "rel_l2_vs_fp64": rel,
         "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 5),

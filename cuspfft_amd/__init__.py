@@ -27,6 +27,8 @@ PREC_BF16X3 = 1
 ROW_AUTO, ROW_FOUR_STEP, ROW_FUSED = 0, 1, 2
 # column-pass half-period row pairing (spfft_opts.colpass_pair)
 PAIR_AUTO, PAIR_OFF = 0, 1
+# row transform (spfft_opts.row_transform): Bluestein, or a direct length-cols FFT
+TRANSFORM_AUTO, TRANSFORM_BLUESTEIN, TRANSFORM_DIRECT = 0, 1, 2
 
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "HIP", 3: "ALLOC", 4: "UNSUPPORTED", 5: "BAD_INPUT"}
 
@@ -42,7 +44,8 @@ class Opts(ctypes.Structure):
                 ("precision", ctypes.c_int), ("fft_len", ctypes.c_int), ("m1", ctypes.c_int),
                 ("conv_rows", ctypes.c_int), ("colpass_rows", ctypes.c_int),
                 ("colpass_table", ctypes.c_int), ("row_limit", ctypes.c_int),
-                ("row_kernel", ctypes.c_int), ("colpass_pair", ctypes.c_int)]
+                ("row_kernel", ctypes.c_int), ("colpass_pair", ctypes.c_int),
+                ("row_transform", ctypes.c_int)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -53,7 +56,8 @@ class PlanInfo(ctypes.Structure):
                 ("conv_stages", ctypes.c_int), ("conv_radix", ctypes.c_int * 8),
                 ("colpass_rows", ctypes.c_int), ("colpass_full_table", ctypes.c_int),
                 ("precision", ctypes.c_int), ("workspace_bytes", ctypes.c_size_t),
-                ("row_kernel", ctypes.c_int), ("colpass_paired", ctypes.c_int)]
+                ("row_kernel", ctypes.c_int), ("colpass_paired", ctypes.c_int),
+                ("direct", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -140,6 +144,7 @@ def load():
         "spfft_next_7_smooth": (_i, [_i]),
         "spfft_factor_3_smooth": (_i, [_i, _vp, _i]),
         "spfft_fused_lengths": (_i, [_vp, _i]),
+        "spfft_direct_lengths": (_i, [_vp, _i]),
         "spfft_coo_generate": (_i, [ctypes.c_char_p, _i, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
                                     ctypes.POINTER(HostCoo)]),
@@ -180,6 +185,13 @@ def fused_lengths() -> list[int]:
     """Bluestein lengths M with a fused row-kernel shape (row_kernel=ROW_FUSED)."""
     buf = (ctypes.c_int * 256)()
     k = load().spfft_fused_lengths(ctypes.cast(buf, _vp), 256)
+    return list(buf[:min(k, 256)])
+
+
+def direct_lengths() -> list[int]:
+    """Widths cols with a direct length-cols transform (row_transform=TRANSFORM_DIRECT)."""
+    buf = (ctypes.c_int * 256)()
+    k = load().spfft_direct_lengths(ctypes.cast(buf, _vp), 256)
     return list(buf[:min(k, 256)])
 
 
@@ -238,6 +250,8 @@ CONFIGS = {
     "c2": ("banded", 8219, 242700, 300, 1200, 0.02),
     "c3": ("chunglu", 16706, 121250, 2.1, 360, 0),
     "c5": ("chunglu", 40421, 351400, 2.2, 0, 0),
+    # the shape of the reference's delaunay_n14 benchmark (README.md:659-660), synthetic
+    "p14": ("uniform", 16384, 49122, 0, 0, 0),
 }
 
 CONFIG_TEXT = {
@@ -245,6 +259,8 @@ CONFIG_TEXT = {
     "c2": "banded symmetric 8219x8219, p ~ exp(-|i-j|/300), cutoff 1200, +2% off-band, ~242700 nnz, seed 0",
     "c3": "Chung-Lu power-law 16706x16706, exponent 2.1, max degree 360, ~121250 nnz, seed 0",
     "c5": "Chung-Lu power-law 40421x40421, exponent 2.2, ~351400 nnz, seed 0",
+    "p14": "uniform symmetric 16384x16384, 49122 random i<j pairs mirrored (98244 nnz), seed 0 "
+           "(the size of delaunay_n14; a power-of-two width)",
 }
 
 
@@ -263,7 +279,8 @@ def _stream_handle(stream=None):
 
 
 def _opts(length_rule=SMOOTH_3, u_tile=128, fft_len=0, m1=0, conv_rows=0, precision=PREC_FP32,
-          colpass_rows=0, colpass_table=0, row_limit=0, row_kernel=ROW_AUTO, colpass_pair=PAIR_AUTO):
+          colpass_rows=0, colpass_table=0, row_limit=0, row_kernel=ROW_AUTO, colpass_pair=PAIR_AUTO,
+          row_transform=TRANSFORM_AUTO):
     o = Opts()
     load().spfft_opts_default(ctypes.byref(o))
     o.length_rule, o.u_tile, o.precision = length_rule, u_tile, precision
@@ -271,6 +288,7 @@ def _opts(length_rule=SMOOTH_3, u_tile=128, fft_len=0, m1=0, conv_rows=0, precis
     o.colpass_rows, o.colpass_table, o.row_limit = colpass_rows, colpass_table, row_limit
     o.row_kernel = row_kernel
     o.colpass_pair = colpass_pair
+    o.row_transform = row_transform
     return o
 
 
@@ -295,10 +313,11 @@ class Plan:
     def __init__(self, rows: int, cols: int, length_rule: int = SMOOTH_3, u_tile: int = 128,
                  fft_len: int = 0, m1: int = 0, conv_rows: int = 0, precision: int = PREC_FP32,
                  colpass_rows: int = 0, colpass_table: int = 0, row_limit: int = 0,
-                 row_kernel: int = ROW_AUTO, colpass_pair: int = PAIR_AUTO):
+                 row_kernel: int = ROW_AUTO, colpass_pair: int = PAIR_AUTO,
+                 row_transform: int = TRANSFORM_AUTO):
         L = load()
         o = _opts(length_rule, u_tile, fft_len, m1, conv_rows, precision, colpass_rows,
-                  colpass_table, row_limit, row_kernel, colpass_pair)
+                  colpass_table, row_limit, row_kernel, colpass_pair, row_transform)
         self._h = _vp()
         _check(L.spfft_plan_create(int(rows), int(cols), ctypes.byref(o), ctypes.byref(self._h)))
         inf = PlanInfo()

@@ -3,7 +3,7 @@
 // the flags, :45-83 for the warm-up + median rule and the table lines,
 // :486-524 and :649-659 for the variants).
 //
-//   spfft_cli <matrix.mtx | gen:c1 | gen:c2 | gen:c3 | gen:c5> [options]
+//   spfft_cli <matrix.mtx | gen:c1 | gen:c2 | gen:c3 | gen:c5 | gen:p14> [options]
 //     --csc-mixed-radix   Bluestein at next_3_smooth(2*cols-1)
 //     --csc-mixed-radix-graph  the same, captured into one hipGraph and launched once
 //     --csc-cufft-smooth  Bluestein at next_7_smooth(2*cols-1) (hand-written FFTs, no library)
@@ -37,6 +37,7 @@ COOMatrix load(const std::string& spec) {
     else if (k == "c2") st = spfft_coo_generate("banded", 8219, 242700, 300, 1200, 0.02, 0, &h);
     else if (k == "c3") st = spfft_coo_generate("chunglu", 16706, 121250, 2.1, 360, 0, 0, &h);
     else if (k == "c5") st = spfft_coo_generate("chunglu", 40421, 351400, 2.2, 0, 0, 0, &h);
+    else if (k == "p14") st = spfft_coo_generate("uniform", 16384, 49122, 0, 0, 0, 0, &h);  // delaunay_n14's size
     else throw std::runtime_error("unknown generator " + spec);
     if (st != SPFFT_OK) throw std::runtime_error("generator failed");
     COOMatrix c;
@@ -81,7 +82,7 @@ void print_bench(const char* label, float ms, size_t mem, int n, float mn, float
 
 void usage(const char* p) {
     fprintf(stderr,
-            "Usage: %s <matrix.mtx | gen:c1|c2|c3|c5> [--csc-mixed-radix] [--csc-mixed-radix-graph]\n"
+            "Usage: %s <matrix.mtx | gen:c1|c2|c3|c5|p14> [--csc-mixed-radix] [--csc-mixed-radix-graph]\n"
             "          [--csc-cufft-smooth] [--csc-stockham-stream] [--csc-stockham-cufft-stream]\n"
             "          [--csc-tile N] [--repeat N] [--sparse-only] [--cpu-reference]\n",
             p);

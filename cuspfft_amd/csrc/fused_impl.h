@@ -49,6 +49,68 @@ enum { kFwd = 0, kInv = 1, kPivot = 2 };
 // the W_M^e table: 256 low entries and (M + 255) / 256 high ones, M <= 65536
 constexpr int kFusedWmEntries = 256 + 256;
 
+// LDS row layout.  Power-of-two stage radices access a row with lane strides of
+// R entries (8 R bytes), which puts all lanes of a wave on a few LDS banks.  When
+// P = 1 the row is stored padded, entry i at i + floor(i / 32) (row pitch
+// L + L / 32), which spreads those strides over the banks.  Padding is used only
+// where it costs no instructions: every LDS access of the stages is a
+// per-thread base plus a compile-time offset, and x + y keeps that form in the
+// padded layout iff (x mod 32) + (y mod 32) < 32 for all its bases x and offsets
+// y -- checked at compile time for every stage of the shape (fused_pad_ok).
+template <int P>
+__host__ __device__ constexpr int lds_idx(int i) { return P ? i + (i >> 5) : i; }
+template <int L, int P>
+constexpr int kRowPitch = P ? L + L / 32 : L;
+
+// (x mod 32) + (y mod 32) < 32 for the bases x and offsets y of one stage:
+// reads j + r (L / R); writes ((j / NS) NS R + j % NS) + r NS, or (pivot: the
+// inverse half) j R + r.
+constexpr bool pad_stage_ok(int L, int R, int NS, bool pivot) {
+    const int nb = L / R;
+    int rd = 0, wr_base = 0, wr_off = 0, pv_base = 0;
+    for (int r = 0; r < R; r++) {
+        rd = (r * nb) % 32 > rd ? (r * nb) % 32 : rd;
+        wr_off = (r * NS) % 32 > wr_off ? (r * NS) % 32 : wr_off;
+    }
+    int jmax = 0;
+    for (int j = 0; j < nb; j++) {
+        jmax = j % 32 > jmax ? j % 32 : jmax;
+        const int b = (j / NS) * NS * R + j % NS;
+        wr_base = b % 32 > wr_base ? b % 32 : wr_base;
+        pv_base = (j * R) % 32 > pv_base ? (j * R) % 32 : pv_base;
+    }
+    const int pv_off = R - 1 < 31 ? R - 1 : 31;
+    return jmax + rd < 32 && wr_base + wr_off < 32 && (!pivot || pv_base + pv_off % 32 < 32);
+}
+// every stage of the Bluestein convolution (or of the forward transform) of M2 = R1 R2 R3
+constexpr bool fused_pad_ok(int R1, int R2, int R3) {
+    const int L = R1 * R2 * R3;
+    if (L % 32) return false;
+    bool ok = pad_stage_ok(L, R1, 1, false);
+    if (R2 > 1) ok = ok && pad_stage_ok(L, R2, R1, R3 == 1);
+    if (R3 > 1) ok = ok && pad_stage_ok(L, R3, R1 * R2, true);
+    if (R3 > 1) ok = ok && pad_stage_ok(L, R2, R3, false);  // inverse stages
+    if (R2 > 1) ok = ok && pad_stage_ok(L, R1, R2 * R3, false);
+    if (R2 == 1) ok = ok && pad_stage_ok(L, R1, 1, true);
+    return ok;
+}
+
+constexpr size_t fused_lds_bytes_pitch(int m1, int nb, int rpw, int m2, size_t pitch) {
+    // W_M2 table, the G = rpw * m1 / nb rows of a batch, the W_M table
+    return ((size_t)m2 + pitch * (size_t)rpw * (m1 / nb) + kFusedWmEntries) * sizeof(float2);
+}
+// padded rows when the stages allow it and the padded rows still fit in 160 KiB
+constexpr bool fused_pad(int m1, int nb, int rpw, int r1, int r2, int r3) {
+    const int m2 = r1 * r2 * r3;
+    return fused_pad_ok(r1, r2, r3) &&
+           fused_lds_bytes_pitch(m1, nb, rpw, m2, (size_t)m2 + m2 / 32) <= 160 * 1024;
+}
+constexpr size_t fused_lds_bytes_of(int m1, int nb, int rpw, int r1, int r2, int r3) {
+    const int m2 = r1 * r2 * r3;
+    return fused_lds_bytes_pitch(m1, nb, rpw, m2,
+                                 fused_pad(m1, nb, rpw, r1, r2, r3) ? (size_t)m2 + m2 / 32 : (size_t)m2);
+}
+
 // W_M^e, e = k1 n2 < M, from the two-level LDS table hi[e >> 8] * lo[e & 255].
 __device__ __forceinline__ float2 wm_pow(const float2* s_wm, int e) {
     return cmul(s_wm[256 + (e >> 8)], s_wm[e & 255]);
@@ -77,9 +139,10 @@ __device__ __forceinline__ void fused_twiddle(float2* v, int n2, const float2* s
     }
 }
 
-template <int R, int L, int NS, int MODE, int G, int KB, int NT>
+template <int R, int L, int NS, int MODE, int G, int KB, int NT, int P>
 __device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const float2* s_tw, int kbase) {
     constexpr int nb = L / R;
+    constexpr int LP = kRowPitch<L, P>;
     constexpr int total = G * nb;
     constexpr int MB = (total + NT - 1) / NT;
     constexpr int step = L / (NS * R);
@@ -92,9 +155,9 @@ __device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const
         if (t < total) {
             const int g = t / nb;
             const int j = t - g * nb;
-            const float2* src = buf + g * L + j;
+            const float2* src = buf + g * LP + lds_idx<P>(j);
 #pragma unroll
-            for (int r = 0; r < R; r++) v[b][r] = src[r * nb];
+            for (int r = 0; r < R; r++) v[b][r] = src[lds_idx<P>(r * nb)];
         }
     }
     __syncthreads();
@@ -127,9 +190,9 @@ __device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const
                 base = j * R;
                 stride = 1;
             }
-            float2* dst = buf + g * L + base;
+            float2* dst = buf + g * LP + lds_idx<P>(base);
 #pragma unroll
-            for (int r = 0; r < R; r++) dst[r * stride] = v[b][r];
+            for (int r = 0; r < R; r++) dst[lds_idx<P>(r * stride)] = v[b][r];
         }
     }
     __syncthreads();
@@ -138,24 +201,38 @@ __device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const
 // The convolution of the batch's rows: forward stages R1 .. Rn, B^, inverse
 // stages Rn .. R1 (radices of 1 are no stage), the last forward and first
 // inverse stage merged (kPivot).
-template <int L, int G, int KB, int NT, int R1, int R2, int R3>
+template <int L, int G, int KB, int NT, int R1, int R2, int R3, int P>
 __device__ __forceinline__ void fused_conv(bio::rsrc_t r_bh, float2* buf, const float2* s_tw, int kbase) {
     if constexpr (R3 > 1) {
-        fused_stage<R1, L, 1, kFwd, G, KB, NT>(r_bh, buf, s_tw, kbase);
-        fused_stage<R2, L, R1, kFwd, G, KB, NT>(r_bh, buf, s_tw, kbase);
-        fused_stage<R3, L, R1 * R2, kPivot, G, KB, NT>(r_bh, buf, s_tw, kbase);
-        fused_stage<R2, L, R3, kInv, G, KB, NT>(r_bh, buf, s_tw, kbase);
-        fused_stage<R1, L, R3 * R2, kInv, G, KB, NT>(r_bh, buf, s_tw, kbase);
+        fused_stage<R1, L, 1, kFwd, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
+        fused_stage<R2, L, R1, kFwd, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
+        fused_stage<R3, L, R1 * R2, kPivot, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
+        fused_stage<R2, L, R3, kInv, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
+        fused_stage<R1, L, R3 * R2, kInv, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
     } else if constexpr (R2 > 1) {
-        fused_stage<R1, L, 1, kFwd, G, KB, NT>(r_bh, buf, s_tw, kbase);
-        fused_stage<R2, L, R1, kPivot, G, KB, NT>(r_bh, buf, s_tw, kbase);
-        fused_stage<R1, L, R2, kInv, G, KB, NT>(r_bh, buf, s_tw, kbase);
+        fused_stage<R1, L, 1, kFwd, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
+        fused_stage<R2, L, R1, kPivot, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
+        fused_stage<R1, L, R2, kInv, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
     } else {
-        fused_stage<R1, L, 1, kPivot, G, KB, NT>(r_bh, buf, s_tw, kbase);
+        fused_stage<R1, L, 1, kPivot, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
     }
 }
 
-template <int M1, int NB, int RPW, int NT, int R1, int R2, int R3>
+// The forward stages only (the direct transform of a smooth C: no Bluestein).
+template <int L, int G, int KB, int NT, int R1, int R2, int R3, int P>
+__device__ __forceinline__ void fused_forward(bio::rsrc_t r_bh, float2* buf, const float2* s_tw) {
+    fused_stage<R1, L, 1, kFwd, G, KB, NT, P>(r_bh, buf, s_tw, 0);
+    if constexpr (R2 > 1) fused_stage<R2, L, R1, kFwd, G, KB, NT, P>(r_bh, buf, s_tw, 0);
+    if constexpr (R3 > 1) fused_stage<R3, L, R1 * R2, kFwd, G, KB, NT, P>(r_bh, buf, s_tw, 0);
+}
+
+// DIRECT: the row transform is one length-C FFT (M = C = M1 M2, a shape with the
+// whole row in LDS), with no Bluestein chirps: phase A as below on all M1 inputs,
+// phase B the forward stages only, then each thread reads its LDS column n2 back
+// -- entries X[k1 + M1 n2], k1 < M1 -- and writes those of row u (v <= C/2) and of
+// its mirror row R - u (X[R-u][C - w] = conj(X[u][w])).  The column pass then runs
+// with h = 1.
+template <int M1, int NB, int RPW, int NT, int R1, int R2, int R3, bool DIRECT>
 __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
     constexpr int M2 = R1 * R2 * R3;
     constexpr int KB = M1 / NB;
@@ -164,11 +241,14 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
     static_assert(M1 % NB == 0, "batches must divide M1");
     static_assert(RPW == 1 || CPT == 1, "several rows per workgroup: one column per thread");
     static_assert(R1 > 1 && (R2 > 1 || R3 == 1), "stage radices first");
-    // M >= 2C - 1, so the entries w = n2 + M2 n1 < C have n1 < NZ: only those
-    // inputs can be non-zero and only those outputs are written (the codelets
+    static_assert(!DIRECT || NB == 1, "the direct transform holds the whole row in LDS");
+    // Bluestein: M >= 2C - 1, so the entries w = n2 + M2 n1 < C have n1 < NZ: only
+    // those inputs can be non-zero and only those outputs are written (the codelets
     // are pruned to them at compile time; this file is built without signed
-    // zeros, so x + 0 folds to x)
-    constexpr int NZ = (M1 + 1) / 2;
+    // zeros, so x + 0 folds to x).  DIRECT: every entry.
+    constexpr int NZ = DIRECT ? M1 : (M1 + 1) / 2;
+    constexpr int P = fused_pad(M1, NB, RPW, R1, R2, R3) ? 1 : 0;  // padded LDS rows
+    constexpr int LP = kRowPitch<M2, P>;
     extern __shared__ float2 smem[];
     float2* s_tw = smem;                    // W_M2^i, i < M2
     float2* s_wm = s_tw + M2;               // W_M^e: 256 low + (M + 255) / 256 high entries
@@ -230,27 +310,50 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
             if (live[c]) {
-                float2* col = buf + g[c] * KB * M2 + n2[c];
+                float2* col = buf + g[c] * KB * LP + lds_idx<P>(n2[c]);
 #pragma unroll
                 for (int k = 0; k < KB; k++) {
                     if constexpr (b == 0) {
-                        col[k * M2] = v[c][k];
+                        col[k * LP] = v[c][k];
                     } else {
-                        const float2 done = col[k * M2];
-                        col[k * M2] = v[c][b * KB + k];
+                        const float2 done = col[k * LP];
+                        col[k * LP] = v[c][b * KB + k];
                         v[c][(b - 1) * KB + k] = done;
                     }
                 }
             }
         }
         __syncthreads();
-        fused_conv<M2, G, KB, NT, R1, R2, R3>(r_bh, buf, s_tw, b * KB);
+        if constexpr (DIRECT)
+            fused_forward<M2, G, KB, NT, R1, R2, R3, P>(r_bh, buf, s_tw);
+        else
+            fused_conv<M2, G, KB, NT, R1, R2, R3, P>(r_bh, buf, s_tw, b * KB);
     });
 
     // phase C
     const bio::rsrc_t r_h = bio::make_rsrc(a.h, (uint32_t)a.C * 8u);  // reads past C return 0
     const int u0 = a.u_base + row0;
-    if constexpr (RPW == 1) {
+    if constexpr (DIRECT) {
+        // lanes take consecutive n2: conflict-free LDS reads; a lane's M1 outputs are
+        // M1 consecutive entries of the row (the L2 merges the lanes' partial lines)
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            if (!live[c]) continue;
+            const float2* col = buf + g[c] * KB * LP + lds_idx<P>(n2[c]);
+            const int u = u0 + g[c];
+            const bool mirror = u > 0 && a.R - u > u;
+            float2* o = a.out + (size_t)u * a.P;
+            float2* om = a.out_m + (size_t)(mirror ? a.R - u : 0) * a.P;
+#pragma unroll
+            for (int k1 = 0; k1 < M1; k1++) {
+                const float2 x = col[k1 * LP];
+                const int w = k1 + M1 * n2[c];
+                if (w < a.V) o[w] = x;
+                const int vm = w == 0 ? 0 : a.C - w;
+                if (mirror && vm < a.V) om[vm] = make_float2(x.x, -x.y);
+            }
+        }
+    } else if constexpr (RPW == 1) {
         // Row u as a buffer of C/2 + 1 entries: writes of v > C/2 are dropped.  Mirror
         // row R - u: X[R-u][C - w] = conj(X[u][w]) for 0 < w < C, at offset
         // M - 1 - w = (M2 (M1 - 1 - n1) + (M2 - 1 - n2)) from a base M - 1 - C entries
@@ -267,9 +370,9 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
             if (live[c]) {
-                const float2* col = buf + n2[c];
+                const float2* col = buf + lds_idx<P>(n2[c]);
 #pragma unroll
-                for (int k = 0; k < KB; k++) v[c][(NB - 1) * KB + k] = col[k * M2];
+                for (int k = 0; k < KB; k++) v[c][(NB - 1) * KB + k] = col[k * LP];
                 fused_twiddle<M1, true>(v[c], n2[c], s_wm);
                 RegFft<M1>::run(v[c], true);
                 const int n1c = (a.C - n2[c] + M2 - 1) / M2;  // columns w = n2 + M2 n1 < C
@@ -292,9 +395,9 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
         const bio::rsrc_t r_o = bio::make_rsrc(a.out + (size_t)u0 * a.P, (uint32_t)(nrows * a.P) * 8u);
         const bio::rsrc_t r_m = bio::make_rsrc(a.out_m + (size_t)mlo * a.P, (uint32_t)(nrows * a.P) * 8u);
         if (live[0]) {
-            const float2* col = buf + g[0] * KB * M2 + n2[0];
+            const float2* col = buf + g[0] * KB * LP + lds_idx<P>(n2[0]);
 #pragma unroll
-            for (int k = 0; k < KB; k++) v[0][(NB - 1) * KB + k] = col[k * M2];
+            for (int k = 0; k < KB; k++) v[0][(NB - 1) * KB + k] = col[k * LP];
             fused_twiddle<M1, true>(v[0], n2[0], s_wm);
             RegFft<M1>::run(v[0], true);
             const int u = u0 + g[0];
@@ -362,9 +465,6 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
     SPFFT_FUSED_GROUP_A(X) SPFFT_FUSED_GROUP_B(X) SPFFT_FUSED_GROUP_C(X) SPFFT_FUSED_GROUP_D(X)     \
     SPFFT_FUSED_GROUP_E(X) SPFFT_FUSED_GROUP_F(X) SPFFT_FUSED_GROUP_G(X) SPFFT_FUSED_GROUP_H(X)
 
-constexpr size_t fused_lds_bytes_of(int m1, int nb, int rpw, int m2) {
-    return ((size_t)m2 * (1 + (size_t)rpw * (m1 / nb)) + kFusedWmEntries) * sizeof(float2);
-}
 
 // Each returns hipErrorInvalidValue when the shape is not in its group.
 hipError_t launch_fused_group_a(const RowPassArgs& a, hipStream_t s, bool prepare_only);
@@ -376,12 +476,17 @@ hipError_t launch_fused_group_f(const RowPassArgs& a, hipStream_t s, bool prepar
 hipError_t launch_fused_group_g(const RowPassArgs& a, hipStream_t s, bool prepare_only);
 hipError_t launch_fused_group_h(const RowPassArgs& a, hipStream_t s, bool prepare_only);
 
+// (a.direct: the direct-transform instantiation, single-batch shapes only)
 #define SPFFT_FUSED_LAUNCH_CASE(m1, nb, rpw, nt, r1, r2, r3)                                       \
     if (a.M1 == m1 && a.fused_nb == nb && a.fused_rpw == rpw && a.fused_nt == nt &&                 \
         a.f2.radix[0] == r1 && (a.f2.nstages > 1 ? a.f2.radix[1] : 1) == r2 &&                      \
         (a.f2.nstages > 2 ? a.f2.radix[2] : 1) == r3) {                                             \
-        constexpr size_t lds = fused_lds_bytes_of(m1, nb, rpw, r1 * r2 * r3);                       \
-        auto* k = &k_rowfused<m1, nb, rpw, nt, r1, r2, r3>;                                         \
+        constexpr size_t lds = fused_lds_bytes_of(m1, nb, rpw, r1, r2, r3);                         \
+        auto* k = &k_rowfused<m1, nb, rpw, nt, r1, r2, r3, false>;                                  \
+        if (a.direct) {                                                                             \
+            if constexpr (nb == 1) k = &k_rowfused<m1, nb, rpw, nt, r1, r2, r3, nb == 1>;           \
+            else return hipErrorInvalidValue;                                                       \
+        }                                                                                           \
         if (prepare_only)                                                                           \
             return hipFuncSetAttribute(reinterpret_cast<const void*>(k),                            \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);       \

@@ -11,7 +11,7 @@ hipError_t launch_fused_group_b(const RowPassArgs& a, hipStream_t s, bool prepar
 bool fused_shape(int i, FusedShape* out) {
     static const FusedShape kShapes[] = {
 #define SPFFT_SHAPE(m1, nb, rpw, nt, r1, r2, r3) \
-    {m1, nb, rpw, nt, {r1, r2, r3}, fused_lds_bytes_of(m1, nb, rpw, r1 * r2 * r3)},
+    {m1, nb, rpw, nt, {r1, r2, r3}, fused_lds_bytes_of(m1, nb, rpw, r1, r2, r3)},
         SPFFT_FUSED_LIST(SPFFT_SHAPE)
 #undef SPFFT_SHAPE
     };

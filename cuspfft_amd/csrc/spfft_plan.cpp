@@ -493,6 +493,17 @@ int spfft_fused_lengths(int* out, int cap) {
     return (int)ms.size();
 }
 
+int spfft_direct_lengths(int* out, int cap) {
+    std::vector<int> ms;
+    spfft::FusedShape f;
+    for (int i = 0; spfft::fused_shape(i, &f); i++)
+        if (f.nb == 1) ms.push_back(f.m1 * f.r[0] * f.r[1] * f.r[2]);
+    std::sort(ms.begin(), ms.end());
+    ms.erase(std::unique(ms.begin(), ms.end()), ms.end());
+    for (int i = 0; out && i < (int)ms.size() && i < cap; i++) out[i] = ms[i];
+    return (int)ms.size();
+}
+
 int spfft_factor_3_smooth(int n, int* out, int cap) {
     int c3 = 0, k = 0;
     if (n < 1) return -1;
@@ -510,10 +521,11 @@ namespace {
 // Shape planning only (no device allocation): M, the M1 x M2 split, the
 // convolution kernel and its rows per workgroup.
 // Fused row-kernel shape for M, or -1.  lds_limit: the device's LDS per workgroup.
-int find_fused(int M, int m1_req, size_t lds_limit, spfft::FusedShape* out) {
+int find_fused(int M, int m1_req, size_t lds_limit, spfft::FusedShape* out, bool one_batch = false) {
     spfft::FusedShape f;
     for (int i = 0; spfft::fused_shape(i, &f); i++) {
-        if (f.m1 * f.r[0] * f.r[1] * f.r[2] != M || (m1_req && f.m1 != m1_req) || f.lds_bytes > lds_limit)
+        if (f.m1 * f.r[0] * f.r[1] * f.r[2] != M || (m1_req && f.m1 != m1_req) || f.lds_bytes > lds_limit ||
+            (one_batch && f.nb != 1))
             continue;
         *out = f;
         return i;
@@ -568,6 +580,17 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     } else {
         return fail(SPFFT_ERR_INVALID_ARG, "length_rule must be 3 or 7");
     }
+    // the direct length-cols FFT: a single-batch fused shape of length cols
+    if (o.row_transform < SPFFT_TRANSFORM_AUTO || o.row_transform > SPFFT_TRANSFORM_DIRECT)
+        return fail(SPFFT_ERR_INVALID_ARG, "row_transform must be 0, 1 or 2");
+    spfft::FusedShape fd{};
+    const bool direct_ok = !mfma && o.fft_len == 0 && o.conv_rows == 0 && o.row_kernel != SPFFT_ROW_FOUR_STEP &&
+                           find_fused(cols, o.m1, lds_limit, &fd, true) >= 0;
+    if (o.row_transform == SPFFT_TRANSFORM_DIRECT && !direct_ok)
+        return fail(SPFFT_ERR_UNSUPPORTED, "no direct transform for cols = " + std::to_string(cols) +
+                                               " with these options");
+    const bool direct = direct_ok && o.row_transform != SPFFT_TRANSFORM_BLUESTEIN;
+    if (direct) M = cols;
     p->R = rows;
     p->C = cols;
     p->V = cols / 2 + 1;
@@ -579,9 +602,9 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     p->packed = rows <= 65536 && cols <= 65536;
     if (o.row_kernel < SPFFT_ROW_AUTO || o.row_kernel > SPFFT_ROW_FUSED)
         return fail(SPFFT_ERR_INVALID_ARG, "row_kernel must be 0, 1 or 2");
-    spfft::FusedShape fs{};
-    const bool fused_ok = !mfma && o.conv_rows == 0 && o.row_kernel != SPFFT_ROW_FOUR_STEP &&
-                          find_fused(M, o.m1, lds_limit, &fs) >= 0;
+    spfft::FusedShape fs = fd;
+    const bool fused_ok = direct || (!mfma && o.conv_rows == 0 && o.row_kernel != SPFFT_ROW_FOUR_STEP &&
+                                     find_fused(M, o.m1, lds_limit, &fs) >= 0);
     if (o.row_kernel == SPFFT_ROW_FUSED && !fused_ok)
         return fail(SPFFT_ERR_UNSUPPORTED, "no fused row-kernel shape for M = " + std::to_string(M) +
                                                (mfma ? " in the BF16x3 form" : ""));
@@ -591,6 +614,7 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
         p->M1 = fs.m1;
         p->M2 = fs.r[0] * fs.r[1] * fs.r[2];
         a.fused = 1;
+        a.direct = direct ? 1 : 0;
         a.fused_nb = fs.nb;
         a.fused_rpw = fs.rpw;
         a.fused_nt = fs.nt;
@@ -673,6 +697,7 @@ void fill_info(const spfft_plan_s* p, spfft_plan_info* info) {
     info->workspace_bytes = p->table_bytes;
     info->row_kernel = p->base.fused ? SPFFT_ROW_FUSED : SPFFT_ROW_FOUR_STEP;
     info->colpass_paired = pair_eligible(*p) ? 1 : 0;
+    info->direct = p->base.direct;
 }
 
 }  // namespace
@@ -731,7 +756,8 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
     std::vector<float2> h(cols), lo(1 << hs), hi(a.n_hi), t2(p->M2), tm((size_t)M), bh((size_t)M);
     for (int k = 0; k < cols; k++) {
         const long long k2 = (long long)k * k % (2LL * cols);
-        h[k] = to_f2(unit_root(k2, 2LL * cols, -1));  // exp(-i pi k^2 / C)
+        // exp(-i pi k^2 / C); the direct transform has no chirp (the column pass takes h = 1)
+        h[k] = a.direct ? make_float2(1.f, 0.f) : to_f2(unit_root(k2, 2LL * cols, -1));
     }
     for (int l = 0; l < (1 << hs); l++) lo[l] = to_f2(unit_root(l, rows, -1));
     for (int j = 0; j < a.n_hi; j++) hi[j] = to_f2(unit_root((long long)j << hs, rows, -1));
@@ -739,7 +765,7 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
     for (int k1 = 0; k1 < p->M1; k1++)
         for (int n2 = 0; n2 < p->M2; n2++)
             tm[(size_t)k1 * p->M2 + n2] = to_f2(unit_root((long long)k1 * n2, M, -1));
-    {
+    if (!a.direct) {
         // b[0] = 1, b[k] = b[M-k] = exp(+i pi k^2 / C) for 1 <= k < C
         std::vector<cd> b((size_t)M, cd(0, 0));
         b[0] = cd(1, 0);

@@ -57,6 +57,7 @@ typedef struct spfft_opts {
                           rows) are computed: a sampled run for timing (the autotuner) */
     int row_kernel;    /* SPFFT_ROW_*: how the Bluestein row pass runs                       */
     int colpass_pair;  /* SPFFT_PAIR_*: half-period row pairing in the column pass            */
+    int row_transform; /* SPFFT_TRANSFORM_*: Bluestein, or a direct length-cols FFT            */
 } spfft_opts;
 
 /* Row-pass kernels.  FOUR_STEP: k_fft1 + k_conv + k_ifft1 with the tile buffer T
@@ -72,6 +73,14 @@ enum { SPFFT_ROW_AUTO = 0, SPFFT_ROW_FOUR_STEP = 1, SPFFT_ROW_FUSED = 2 };
  * (u_tile >= R/2 + 1), row_limit is 0 and colpass_rows is not 1 (a lane's output
  * rows are then half its own, half their partners).  OFF: every row on its own. */
 enum { SPFFT_PAIR_AUTO = 0, SPFFT_PAIR_OFF = 1 };
+
+/* Row transform.  BLUESTEIN: the chirp convolution at the smooth length M >= 2 cols - 1
+ * of the reference's variants.  DIRECT: one FFT of length cols itself when cols has a
+ * fused-kernel shape holding the whole row on chip (cols in spfft_direct_lengths();
+ * FP32 form, no fft_len / conv_rows / four-step options): no chirps, a quarter of the
+ * row-pass work.  AUTO: DIRECT when possible, else BLUESTEIN.  The transform computed
+ * is the same; only the rounding differs. */
+enum { SPFFT_TRANSFORM_AUTO = 0, SPFFT_TRANSFORM_BLUESTEIN = 1, SPFFT_TRANSFORM_DIRECT = 2 };
 
 typedef struct spfft_plan_info {
     int rows, cols;
@@ -91,6 +100,7 @@ typedef struct spfft_plan_info {
     size_t workspace_bytes; /* device bytes held by the plan (tables + tile buffer) */
     int row_kernel;         /* SPFFT_ROW_FOUR_STEP or SPFFT_ROW_FUSED                   */
     int colpass_paired;     /* 1 if the column pass pairs rows u and R/2 - u            */
+    int direct;             /* 1: direct length-cols FFT (fft_len = cols), 0: Bluestein  */
 } spfft_plan_info;
 
 typedef struct spfft_plan_s* spfft_plan;
@@ -263,6 +273,11 @@ int spfft_factor_3_smooth(int n, int* out, int cap);
  * increasing order: writes up to cap of them to out and returns how many exist.
  * No counterpart in the reference (its row pass has no fused kernel). */
 int spfft_fused_lengths(int* out, int cap);
+/* Lengths cols with a direct transform (SPFFT_TRANSFORM_DIRECT: a single-batch fused
+ * shape of length cols), increasing; same convention as spfft_fused_lengths.
+ * Replaces the reference's power-of-two-only plain FFT paths for such widths
+ * (README.md:653-673: Bluestein doubles the work of a power-of-two width). */
+int spfft_direct_lengths(int* out, int cap);
 
 /* Seeded synthetic binary matrices (host memory, canonical COO: deduplicated,
  * sorted by (row, col), zero diagonal, symmetric).  Release with spfft_coo_free. */

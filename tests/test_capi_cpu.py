@@ -86,6 +86,28 @@ def test_planner_colpass_pairing():
         sp.describe_plan(2310, 2310, colpass_pair=2)
 
 
+def test_planner_direct_transform():
+    """Widths with a single-batch fused shape take the direct length-cols FFT by default;
+    Bluestein-only options and other widths keep the Bluestein row pass."""
+    import cuspfft_amd as sp
+    d = sp.direct_lengths()
+    assert set(d) <= set(sp.fused_lengths())
+    assert {1024, 2048, 4096, 8192, 16384} <= set(d)
+    for cols in (1024, 4704, 16384, 18432):
+        info = sp.describe_plan(300, cols)
+        assert info["direct"] == 1 and info["fft_len"] == cols and info["row_kernel"] == sp.ROW_FUSED
+        b = sp.describe_plan(300, cols, row_transform=sp.TRANSFORM_BLUESTEIN)
+        assert b["direct"] == 0 and b["fft_len"] >= 2 * cols - 1
+    for kw in (dict(precision=sp.PREC_BF16X3), dict(row_kernel=sp.ROW_FOUR_STEP), dict(fft_len=4096)):
+        assert sp.describe_plan(300, 2048, **kw)["direct"] == 0
+    for cols in (2310, 16706, 20736, 1000):  # not 3-smooth, or no single-batch shape
+        assert sp.describe_plan(300, cols)["direct"] == 0
+        with pytest.raises(sp.SpfftError):
+            sp.describe_plan(300, cols, row_transform=sp.TRANSFORM_DIRECT)
+    with pytest.raises(sp.SpfftError):
+        sp.describe_plan(300, 2048, row_transform=3)
+
+
 def test_planner_fused_row_kernel():
     """The planner picks the fused row kernel for the lengths it has a shape for
     (the configurations' 3- and 7-smooth lengths), the four-step kernels

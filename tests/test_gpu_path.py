@@ -539,6 +539,52 @@ def test_cli_reference_flags(gpu):
     assert bad.returncode == 1 and "--csc-tile must be positive" in bad.stderr
 
 
+# ---- direct length-C transform (smooth widths, no Bluestein) ------------------
+DIRECT_CASES = [(1024, 300, 5000, 128), (1024, 7, 40, 3), (1152, 1001, 9000, 1001), (2048, 64, 900, 64),
+                (4704, 90, 3000, 17), (5184, 2310, 16000, 2310), (16384, 33, 4000, 7),
+                (16384, 16384, 98244, 16384), (17496, 20, 600, 20), (18432, 5, 100, 1)]
+
+
+@pytest.mark.parametrize("cols,rows,nnz,tile", DIRECT_CASES)
+def test_direct_transform(gpu, oracle_mod, cols, rows, nnz, tile):
+    """The direct length-cols FFT against the FP64 oracle and the Bluestein plan of the
+    same matrix; the streamed output of the direct plan is bit-identical to it."""
+    import torch
+    sp = gpu
+    rng = np.random.default_rng(cols + rows)
+    r, c = _random_coo(rng, rows, cols, nnz)
+    print(f"[direct] {rows}x{cols} nnz={len(r)} tile={tile}", flush=True)
+    plan, X = _run(sp, rows, cols, r, c, u_tile=tile)
+    assert plan.info["direct"] == 1 and plan.info["fft_len"] == cols
+    _, Xb = _run(sp, rows, cols, r, c, u_tile=tile, row_transform=sp.TRANSFORM_BLUESTEIN)
+    if rows * cols <= 40_000_000:
+        ref = oracle_mod.sparse_rfft2_f64(rows, cols, r, c, nthreads=8)
+    else:
+        u = np.unique(np.concatenate([[0, 1, rows // 2, rows - 1], rng.choice(rows, 12, replace=False)]))
+        ref = oracle_mod.sparse_rfft2_f64(rows, cols, r, c, u_list=u, nthreads=8)
+        X, Xb = X[u], Xb[u]
+    err, errb = oracle_mod.rel_l2(X, ref), oracle_mod.rel_l2(Xb, ref)
+    print(f"  rel_l2 direct={err:.3e} bluestein={errb:.3e}")
+    assert err <= REL_L2_PLAIN and errb <= REL_L2_PLAIN
+    if rows <= 2310:
+        host = plan.alloc_host_output()
+        host.fill_(complex(float("nan"), float("nan")))
+        plan.execute_to_host(_dev(r), _dev(c), host)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(host[:, : cols // 2 + 1].numpy(), X)
+
+
+def test_direct_single_nonzero_closed_form(gpu):
+    sp = gpu
+    rows, cols, r0, c0 = 1000, 16384, 999, 12345
+    plan, X = _run(sp, rows, cols, np.array([r0], np.int32), np.array([c0], np.int32), u_tile=rows)
+    assert plan.info["direct"] == 1
+    u = np.arange(rows)[:, None]
+    v = np.arange(cols // 2 + 1)[None, :]
+    ref = np.exp(-2j * np.pi * (((u * r0) % rows) / rows + ((v * c0) % cols) / cols))
+    assert np.abs(X - ref).max() < 2e-5
+
+
 # ---- streamed output: device chunks copied to pinned host memory --------------
 STREAM_CASES = [((2310, 2310, 16000), 128, {}), ((2310, 2310, 16000), 2310, {}),
                 ((2310, 2310, 16000), 500, dict(row_kernel=1)),
