@@ -17,16 +17,36 @@ namespace spfft {
 
 enum StageIO { kGlobalToLds = 0, kLdsToLds = 1, kLdsToGlobal = 2 };
 
+// LDS row layout of k_conv_fast<R1, R2>.  Its LDS accesses are the stage-1 writes
+// j R1 + r, the stage-2 reads j + r R1, the pivot writes j R2 + r and the stage-3
+// reads j + r R2: lane strides of R1 or R2 entries, which for even radices land
+// on few banks.  With D = gcd(R1, R2) even, entry i of a row is stored at
+// i + floor(i / D) (row pitch L + L / D): every one of those accesses is still a
+// per-thread base plus a compile-time offset (D divides R1, R2 and each base),
+// and the strides become R1 + R1 / D and R2 + R2 / D.  (With D odd the padded
+// strides conflict more, so those rows stay unpadded.)
+constexpr int conv_gcd(int a, int b) { return b == 0 ? a : conv_gcd(b, a % b); }
+constexpr int conv_pad_div(int r1, int r2) {
+    return r2 > 1 && conv_gcd(r1, r2) % 2 == 0 ? conv_gcd(r1, r2) : 0;
+}
+constexpr int conv_row_pitch(int r1, int r2) {
+    return conv_pad_div(r1, r2) ? r1 * r2 + r1 * r2 / conv_pad_div(r1, r2) : r1 * r2;
+}
+template <int D>
+__device__ __forceinline__ int conv_lds_idx(int i) { return D ? i + i / D : i; }
+
 // One Stockham stage of radix R over G2 rows of length L held in LDS (or
 // streamed from / to T).  NS = product of the radices of the earlier stages.
 // PIVOT: the last forward stage (NS = L / R), the product with B^, and the
 // first inverse stage on the same R values (outputs j R + r), one LDS round
 // trip for both; the inverse then runs its radices in reverse order.
-template <int R, int L, int NS, int IO, bool INV, bool MUL, bool PIVOT = false>
+template <int R, int L, int NS, int IO, bool INV, bool MUL, bool PIVOT = false, int D = 0>
 __device__ __forceinline__ void cstage(const RowPassArgs& a, float2* buf, const float2* s_tw,
                                        const int* s_k1, int row0) {
     constexpr int MB = MaxBfly<R>::value;
     constexpr int nb = L / R;
+    constexpr int LP = D ? L + L / D : L;  // LDS row pitch
+    static_assert(!D || (nb % D == 0 && R % D == 0), "padded rows need D | nb and D | R");
     constexpr int step = L / (NS * R);
     const int total = a.G2 * nb;
     float2 v[MB][R];
@@ -47,9 +67,9 @@ __device__ __forceinline__ void cstage(const RowPassArgs& a, float2* buf, const 
 #pragma unroll
                 for (int r = 0; r < R; r++) v[b][r] = live ? src[j + r * nb] : make_float2(0.f, 0.f);
             } else {
-                const float2* src = buf + g * L;
+                const float2* src = buf + g * LP + conv_lds_idx<D>(j);
 #pragma unroll
-                for (int r = 0; r < R; r++) v[b][r] = src[j + r * nb];
+                for (int r = 0; r < R; r++) v[b][r] = src[conv_lds_idx<D>(r * nb)];
             }
         }
     }
@@ -82,16 +102,19 @@ __device__ __forceinline__ void cstage(const RowPassArgs& a, float2* buf, const 
             base = j * R;
             stride = 1;
         }
-        if (IO == kLdsToGlobal) {
+        if constexpr (IO == kLdsToGlobal) {
             if (s_k1[g] >= 0) {
                 float2* dst = a.T + (size_t)(row0 + g) * L;
 #pragma unroll
                 for (int r = 0; r < R; r++) dst[base + r * stride] = v[b][r];
             }
         } else {
-            float2* dst = buf + g * L;
+            // padded rows: the LDS writes are the stage-1 and pivot ones, stride 1 from
+            // a base j R with D | R
+            static_assert(!D || NS == 1 || PIVOT, "padded LDS writes: stride 1 from a multiple of D");
+            float2* dst = buf + g * LP + conv_lds_idx<D>(base);
 #pragma unroll
-            for (int r = 0; r < R; r++) dst[base + r * stride] = v[b][r];
+            for (int r = 0; r < R; r++) dst[conv_lds_idx<D>(r * stride)] = v[b][r];
         }
     }
     if (IO != kLdsToGlobal) __syncthreads();
@@ -110,18 +133,19 @@ template <int R1, int R2>
 __global__ __launch_bounds__(kThreads) void k_conv_fast(RowPassArgs a) {
     extern __shared__ float2 smem[];
     constexpr int L = R1 * R2;
+    constexpr int D = conv_pad_div(R1, R2);
     float2* s_tw = smem;
     float2* buf = s_tw + L;
-    int* s_k1 = reinterpret_cast<int*>(buf + (size_t)a.G2 * L);
+    int* s_k1 = reinterpret_cast<int*>(buf + (size_t)a.G2 * conv_row_pitch(R1, R2));
     const int row0 = blockIdx.x * a.G2;
     conv_setup(a, s_tw, s_k1, row0);
     if constexpr (R2 == 1) {
         cstage<R1, L, 1, kGlobalToLds, false, true>(a, buf, s_tw, s_k1, row0);
         cstage<R1, L, 1, kLdsToGlobal, true, false>(a, buf, s_tw, s_k1, row0);
     } else {
-        cstage<R1, L, 1, kGlobalToLds, false, false>(a, buf, s_tw, s_k1, row0);
-        cstage<R2, L, R1, kLdsToLds, false, true, true>(a, buf, s_tw, s_k1, row0);
-        cstage<R1, L, R2, kLdsToGlobal, true, false>(a, buf, s_tw, s_k1, row0);
+        cstage<R1, L, 1, kGlobalToLds, false, false, false, D>(a, buf, s_tw, s_k1, row0);
+        cstage<R2, L, R1, kLdsToLds, false, true, true, D>(a, buf, s_tw, s_k1, row0);
+        cstage<R1, L, R2, kLdsToGlobal, true, false, false, D>(a, buf, s_tw, s_k1, row0);
     }
 }
 

@@ -26,7 +26,10 @@ bool launch_conv_e(const RowPassArgs& a, hipStream_t s, hipError_t* err) {
 
 size_t conv_lds_bytes(const RowPassArgs& a) {
     const size_t bufs = a.conv_kind == 0 ? 2 : 1;
-    return ((size_t)a.M2 + bufs * (size_t)a.G2 * a.M2) * sizeof(float2) + (size_t)a.G2 * sizeof(int);
+    // k_conv_fast rows may be padded (conv_row_pitch)
+    const size_t pitch = a.conv_kind == 1 && a.f2.nstages == 2 ? conv_row_pitch(a.f2.radix[0], a.f2.radix[1])
+                                                                : (size_t)a.M2;
+    return ((size_t)a.M2 + bufs * (size_t)a.G2 * pitch) * sizeof(float2) + (size_t)a.G2 * sizeof(int);
 }
 
 bool conv_fast_supported(int r1, int r2) {
