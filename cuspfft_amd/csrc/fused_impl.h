@@ -105,10 +105,37 @@ constexpr bool fused_pad(int m1, int nb, int rpw, int r1, int r2, int r3) {
     return fused_pad_ok(r1, r2, r3) &&
            fused_lds_bytes_pitch(m1, nb, rpw, m2, (size_t)m2 + m2 / 32) <= 160 * 1024;
 }
-constexpr size_t fused_lds_bytes_of(int m1, int nb, int rpw, int r1, int r2, int r3) {
+// r-major twiddle tables.  In a stage with NS R = L (the last forward one, and the
+// last inverse one), k = j runs across the lanes and the twiddle W_L^{r k} read
+// from W_L^i is a stride-r LDS access.  Stored r-major, entry (r - 1) NS + k, the
+// lanes read consecutive entries at a compile-time offset per r.  Table A: the
+// last forward stage (radix ra); table B: the last inverse stage (radix r1, the
+// Bluestein convolution only; shared with A when ra == r1).  Used when they fit.
+constexpr int fused_tp_radix(int r2, int r3) { return r3 > 1 ? r3 : r2 > 1 ? r2 : 0; }
+constexpr int fused_tp_a(int r1, int r2, int r3) {
+    const int ra = fused_tp_radix(r2, r3);
+    return ra ? (ra - 1) * (r1 * r2 * r3 / ra) : 0;
+}
+constexpr int fused_tp_b(int r1, int r2, int r3) {
+    const int ra = fused_tp_radix(r2, r3);
+    return ra && ra != r1 ? (r1 - 1) * r2 * r3 : 0;
+}
+constexpr size_t fused_lds_bytes_base(int m1, int nb, int rpw, int r1, int r2, int r3) {
     const int m2 = r1 * r2 * r3;
     return fused_lds_bytes_pitch(m1, nb, rpw, m2,
                                  fused_pad(m1, nb, rpw, r1, r2, r3) ? (size_t)m2 + m2 / 32 : (size_t)m2);
+}
+constexpr bool fused_tp(int m1, int nb, int rpw, int r1, int r2, int r3) {
+    return fused_tp_a(r1, r2, r3) > 0 &&
+           fused_lds_bytes_base(m1, nb, rpw, r1, r2, r3) +
+                   (size_t)(fused_tp_a(r1, r2, r3) + fused_tp_b(r1, r2, r3)) * sizeof(float2) <=
+               160 * 1024;
+}
+constexpr size_t fused_lds_bytes_of(int m1, int nb, int rpw, int r1, int r2, int r3) {
+    return fused_lds_bytes_base(m1, nb, rpw, r1, r2, r3) +
+           (fused_tp(m1, nb, rpw, r1, r2, r3)
+                ? (size_t)(fused_tp_a(r1, r2, r3) + fused_tp_b(r1, r2, r3)) * sizeof(float2)
+                : 0);
 }
 
 // W_M^e, e = k1 n2 < M, from the two-level LDS table hi[e >> 8] * lo[e & 255].
@@ -139,8 +166,9 @@ __device__ __forceinline__ void fused_twiddle(float2* v, int n2, const float2* s
     }
 }
 
-template <int R, int L, int NS, int MODE, int G, int KB, int NT, int P>
-__device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const float2* s_tw, int kbase) {
+template <int R, int L, int NS, int MODE, int G, int KB, int NT, int P, int TPR = 0>
+__device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const float2* s_tw, int kbase,
+                                            const float2* s_tp = nullptr) {
     constexpr int nb = L / R;
     constexpr int LP = kRowPitch<L, P>;
     constexpr int total = G * nb;
@@ -148,6 +176,7 @@ __device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const
     constexpr int step = L / (NS * R);
     constexpr bool INV = MODE == kInv;
     static_assert(MODE != kPivot || NS == nb, "the pivot stage is the last forward stage");
+    static_assert(!TPR || step == 1, "the r-major table serves the stages with NS R = L");
     float2 v[MB][R];
 #pragma unroll
     for (int b = 0; b < MB; b++) {
@@ -173,7 +202,7 @@ __device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const
             if (NS > 1 && k != 0) {
 #pragma unroll
                 for (int r = 1; r < R; r++) {
-                    const float2 w = s_tw[r * k * step];
+                    const float2 w = TPR ? s_tp[(r - 1) * NS + k] : s_tw[r * k * step];
                     v[b][r] = INV ? cmulc(v[b][r], w) : cmul(v[b][r], w);
                 }
             }
@@ -201,29 +230,36 @@ __device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const
 // The convolution of the batch's rows: forward stages R1 .. Rn, B^, inverse
 // stages Rn .. R1 (radices of 1 are no stage), the last forward and first
 // inverse stage merged (kPivot).
-template <int L, int G, int KB, int NT, int R1, int R2, int R3, int P>
-__device__ __forceinline__ void fused_conv(bio::rsrc_t r_bh, float2* buf, const float2* s_tw, int kbase) {
+// (TP: the r-major tables s_ta / s_tb of the pivot and the last inverse stage)
+template <int L, int G, int KB, int NT, int R1, int R2, int R3, int P, int TP>
+__device__ __forceinline__ void fused_conv(bio::rsrc_t r_bh, float2* buf, const float2* s_tw, int kbase,
+                                           const float2* s_ta, const float2* s_tb) {
     if constexpr (R3 > 1) {
         fused_stage<R1, L, 1, kFwd, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
         fused_stage<R2, L, R1, kFwd, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
-        fused_stage<R3, L, R1 * R2, kPivot, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
+        fused_stage<R3, L, R1 * R2, kPivot, G, KB, NT, P, TP>(r_bh, buf, s_tw, kbase, s_ta);
         fused_stage<R2, L, R3, kInv, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
-        fused_stage<R1, L, R3 * R2, kInv, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
+        fused_stage<R1, L, R3 * R2, kInv, G, KB, NT, P, TP>(r_bh, buf, s_tw, kbase, s_tb);
     } else if constexpr (R2 > 1) {
         fused_stage<R1, L, 1, kFwd, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
-        fused_stage<R2, L, R1, kPivot, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
-        fused_stage<R1, L, R2, kInv, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
+        fused_stage<R2, L, R1, kPivot, G, KB, NT, P, TP>(r_bh, buf, s_tw, kbase, s_ta);
+        fused_stage<R1, L, R2, kInv, G, KB, NT, P, TP>(r_bh, buf, s_tw, kbase, s_tb);
     } else {
         fused_stage<R1, L, 1, kPivot, G, KB, NT, P>(r_bh, buf, s_tw, kbase);
     }
 }
 
 // The forward stages only (the direct transform of a smooth C: no Bluestein).
-template <int L, int G, int KB, int NT, int R1, int R2, int R3, int P>
-__device__ __forceinline__ void fused_forward(bio::rsrc_t r_bh, float2* buf, const float2* s_tw) {
+template <int L, int G, int KB, int NT, int R1, int R2, int R3, int P, int TP>
+__device__ __forceinline__ void fused_forward(bio::rsrc_t r_bh, float2* buf, const float2* s_tw,
+                                              const float2* s_ta) {
     fused_stage<R1, L, 1, kFwd, G, KB, NT, P>(r_bh, buf, s_tw, 0);
-    if constexpr (R2 > 1) fused_stage<R2, L, R1, kFwd, G, KB, NT, P>(r_bh, buf, s_tw, 0);
-    if constexpr (R3 > 1) fused_stage<R3, L, R1 * R2, kFwd, G, KB, NT, P>(r_bh, buf, s_tw, 0);
+    if constexpr (R3 > 1) {
+        fused_stage<R2, L, R1, kFwd, G, KB, NT, P>(r_bh, buf, s_tw, 0);
+        fused_stage<R3, L, R1 * R2, kFwd, G, KB, NT, P, TP>(r_bh, buf, s_tw, 0, s_ta);
+    } else if constexpr (R2 > 1) {
+        fused_stage<R2, L, R1, kFwd, G, KB, NT, P, TP>(r_bh, buf, s_tw, 0, s_ta);
+    }
 }
 
 // DIRECT: the row transform is one length-C FFT (M = C = M1 M2, a shape with the
@@ -249,10 +285,15 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
     constexpr int NZ = DIRECT ? M1 : (M1 + 1) / 2;
     constexpr int P = fused_pad(M1, NB, RPW, R1, R2, R3) ? 1 : 0;  // padded LDS rows
     constexpr int LP = kRowPitch<M2, P>;
+    constexpr int TP = fused_tp(M1, NB, RPW, R1, R2, R3) ? 1 : 0;   // r-major twiddle tables
+    constexpr int RA = fused_tp_radix(R2, R3), NA = TP ? fused_tp_a(R1, R2, R3) : 0;
+    constexpr int NBT = TP ? fused_tp_b(R1, R2, R3) : 0;
     extern __shared__ float2 smem[];
     float2* s_tw = smem;                    // W_M2^i, i < M2
     float2* s_wm = s_tw + M2;               // W_M^e: 256 low + (M + 255) / 256 high entries
-    float2* buf = s_wm + kFusedWmEntries;   // G rows of length M2: [row of workgroup][k1 of batch][n2]
+    float2* s_ta = s_wm + kFusedWmEntries;  // TP: W_M2^{r k} at (r - 1) (M2 / RA) + k
+    float2* s_tb = NBT ? s_ta + NA : s_ta;  // TP: the same for radix R1 (when R1 != RA)
+    float2* buf = s_ta + NA + NBT;          // G rows of length M2: [row of workgroup][k1 of batch][n2]
     const int t = threadIdx.x;
     const int row0 = blockIdx.x * RPW;  // first tile row of this workgroup
     int n2[CPT], g[CPT];
@@ -294,6 +335,11 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
         }
         for (int i = t; i < M2; i += NT) s_tw[i] = a.tw2[i];
         for (int i = t; i < 256 + (a.M + 255) / 256; i += NT) s_wm[i] = a.wm2[i];
+        if constexpr (TP) {
+            constexpr int nsa = NA ? M2 / RA : 1, nsb = NBT ? M2 / R1 : 1;
+            for (int i = t; i < NA; i += NT) s_ta[i] = a.tw2[(i / nsa + 1) * (i % nsa)];
+            for (int i = t; i < NBT; i += NT) s_tb[i] = a.tw2[(i / nsb + 1) * (i % nsb)];
+        }
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
@@ -325,9 +371,9 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
         }
         __syncthreads();
         if constexpr (DIRECT)
-            fused_forward<M2, G, KB, NT, R1, R2, R3, P>(r_bh, buf, s_tw);
+            fused_forward<M2, G, KB, NT, R1, R2, R3, P, TP>(r_bh, buf, s_tw, s_ta);
         else
-            fused_conv<M2, G, KB, NT, R1, R2, R3, P>(r_bh, buf, s_tw, b * KB);
+            fused_conv<M2, G, KB, NT, R1, R2, R3, P, TP>(r_bh, buf, s_tw, b * KB, s_ta, s_tb);
     });
 
     // phase C

@@ -25,7 +25,7 @@ import os
 
 import numpy as np
 
-from . import _check, _vp, describe_plan, load
+from . import TRANSFORM_DIRECT, _check, _vp, describe_plan, load
 
 N_FEATURES = 29
 # the network trained on the autotuner timings of 2000 seeded matrices (train_selector)
@@ -67,10 +67,16 @@ def features(rec: dict, opts: dict, info: dict | None = None) -> np.ndarray:
     return np.asarray(f, np.float32)
 
 
-def trial_opts(t: dict) -> dict:
-    """The options of a timed trial with its Bluestein length pinned to the one it ran."""
+def trial_opts(t: dict, rec: dict | None = None) -> dict:
+    """The options of a timed trial with its Bluestein length pinned to the one it ran;
+    a trial that ran the direct length-cols transform (flagged, or a length below
+    2 cols - 1 in records written before the flag) is planned as that again."""
     o = dict(t["opts"])
-    if t.get("fft_len"):
+    direct = t.get("direct") or (rec is not None and t.get("fft_len") and t["fft_len"] < 2 * rec["cols"] - 1)
+    if direct:
+        o.pop("fft_len", None)
+        o["row_transform"] = TRANSFORM_DIRECT
+    elif t.get("fft_len"):
         o["fft_len"] = t["fft_len"]
     return o
 
@@ -80,7 +86,7 @@ def dataset(records: list[dict]):
     X, y, g = [], [], []
     for i, rec in enumerate(records):
         for t in rec["trials"]:
-            X.append(features(rec, trial_opts(t)))
+            X.append(features(rec, trial_opts(t, rec)))
             y.append(math.log(max(t["ms"], 1e-4)))
             g.append(i)
     return np.asarray(X, np.float32), np.asarray(y, np.float32), np.asarray(g, np.int64)
@@ -224,7 +230,7 @@ def regret(sel: Selector, records: list[dict]) -> dict:
     for rec in records:
         trials = rec["trials"]
         best = min(t["ms"] for t in trials)
-        X = np.stack([features(rec, trial_opts(t)) for t in trials])
+        X = np.stack([features(rec, trial_opts(t, rec)) for t in trials])
         pick = trials[int(np.argmin(sel.predict(X)))]
         ratios.append(pick["ms"] / best)
         dflt = [t for t in trials if t.get("default")]
