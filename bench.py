@@ -6,10 +6,13 @@ COO->CSC build + column sparse-DFT pass + Bluestein row pass (all on the GPU).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3] [--no-cpu]
 
-Prints ONE JSON line.  Roofline: algorithmic bytes per FFT
-B_alg = 8*nnz (COO int32 read) + 8*rows*(cols/2+1) (complex64 output write);
-achieved = B_alg * K / (GPU time of the dominant kernel over the K steps),
-measured with HIP events on the launch stream; peak = 8 TB/s (MI355X HBM3E).
+Prints ONE JSON line.  value = FFTs/s over the K timed transforms (no per-kernel
+events in them).  The plan is the shipped selector's choice among the plain FP32
+candidates (--plan default: the planner's default plan); both are timed.
+Roofline: algorithmic bytes per FFT B_alg = 8*nnz (COO int32 read) +
+8*rows*(cols/2+1) (complex64 output write); achieved = B_alg / (the dominant kernel's
+average launch time, from HIP events on the launch stream over K more transforms
+after the timed ones); peak = 8 TB/s (MI355X HBM3E).
 """
 from __future__ import annotations
 
@@ -29,9 +32,14 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # defaults: 50 untimed transforms (about 0.1 s on c3) let the GPU clocks settle
+    # before the timed ones; fewer measured 3-8% slow on the first plan of a process
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--config", default="c3")
+    ap.add_argument("--plan", default="selector", choices=("selector", "default"),
+                    help="plan of the timed transform when no plan option is given: the shipped "
+                         "selector's choice among the plain FP32 candidates, or the planner's default")
     ap.add_argument("--rule", type=int, default=3)
     ap.add_argument("--tile", type=int, default=0, help="u-tile rows (0 = tuned default)")
     ap.add_argument("--m1", type=int, default=0)
@@ -222,10 +230,22 @@ def main():
     dev = torch.device("cuda:0")
     m = sp.config_matrix(args.config)
     tile = args.tile if args.tile > 0 else m.rows
-    plan = sp.Plan(m.rows, m.cols, length_rule=args.rule, u_tile=tile, m1=args.m1, fft_len=args.fft_len,
-                   precision=args.precision, row_kernel=args.row_kernel,
-                   colpass_rows=args.colpass_rows, colpass_table=args.colpass_table,
-                   colpass_pair=args.colpass_pair, row_transform=args.row_transform)
+    explicit = (args.tile > 0 or args.fft_len or args.m1 or args.precision or args.rule != 3
+                or args.colpass_rows or args.colpass_table or args.row_kernel or args.colpass_pair
+                or args.row_transform)
+    if explicit or args.plan == "default":
+        plan_source = "options" if explicit else "default"
+        opts = dict(length_rule=args.rule, m1=args.m1, fft_len=args.fft_len, precision=args.precision,
+                    row_kernel=args.row_kernel, colpass_rows=args.colpass_rows,
+                    colpass_table=args.colpass_table, colpass_pair=args.colpass_pair,
+                    row_transform=args.row_transform)
+    else:
+        # the shipped network's choice among the autotuner's plain FP32 candidates: the
+        # same transform, the plan predicted fastest
+        from cuspfft_amd import selector
+        plan_source = "selector"
+        opts = selector.choose_plan(m, fp32_only=True)
+    plan = sp.Plan(m.rows, m.cols, u_tile=tile, **opts)
     d_row = torch.as_tensor(m.row_idx).to(dev)
     d_col = torch.as_tensor(m.col_idx).to(dev)
     out = plan.alloc_output()
@@ -235,13 +255,18 @@ def main():
         plan.execute(d_row, d_col, out)
     torch.cuda.synchronize()
 
-    plan.profile(True)
+    # the timed region: K transforms, no per-kernel events
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         plan.execute(d_row, d_col, out)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    # then K more with HIP events around every kernel on the launch stream: the
+    # per-kernel times and the dominant kernel's launch time for the roofline
+    plan.profile(True)
+    for _ in range(args.steps):
+        plan.execute(d_row, d_col, out)
     prof = plan.profile_read(stream)
     plan.profile(False)
     plan.check_input()
@@ -252,6 +277,22 @@ def main():
 
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     value = 1000.0 / ms_per_step
+    default_plan = None
+    if plan_source == "selector":
+        # the planner's default plan on the same input, for comparison
+        dplan = sp.Plan(m.rows, m.cols, u_tile=tile)
+        dout = dplan.alloc_output()
+        for _ in range(args.warmup):
+            dplan.execute(d_row, d_col, dout)
+        torch.cuda.synchronize()
+        td = time.perf_counter()
+        for _ in range(args.steps):
+            dplan.execute(d_row, d_col, dout)
+        torch.cuda.synchronize()
+        dms = (time.perf_counter() - td) * 1000.0 / args.steps
+        default_plan = {"ms_per_step": round(dms, 4), "fft_len": dplan.info["fft_len"], "m1": dplan.info["m1"],
+                        "conv_radix": dplan.info["conv_radix"]}
+        del dplan, dout
     V = m.cols // 2 + 1
     b_alg = 8.0 * m.nnz + 8.0 * m.rows * V
     dom = max(prof, key=lambda k: prof[k][0])
@@ -260,15 +301,14 @@ def main():
     step_gbs = b_alg / (ms_per_step * 1e-3) / 1e9
 
     # HBM bytes per launch of the dominant kernel, from the rocprofv3 FETCH_SIZE /
-    # WRITE_SIZE runs recorded in profiles/traffic.json (same workload and stage)
+    # WRITE_SIZE runs recorded in profiles/traffic.json for this workload and row-pass shape
     traffic = None
     try:
         with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")) as f:
             rec = json.load(f).get(args.config, {}).get(dom)
-        default_plan = (args.tile <= 0 and args.fft_len == 0 and args.m1 == 0 and args.precision == 0
-                        and args.rule == 3 and args.colpass_rows == 0 and args.colpass_table == 0
-                        and args.row_kernel == 0 and args.colpass_pair == 0 and args.row_transform == 0)
-        if rec and default_plan:
+        info0 = plan.info
+        if (rec and rec.get("fft_len") == info0["fft_len"] and rec.get("m1") == info0["m1"]
+                and rec.get("row_kernel") == info0["row_kernel"] and args.tile <= 0):
             traffic = rec["bytes"]
     except (OSError, ValueError):
         traffic = None
@@ -355,6 +395,7 @@ def main():
             "colpass_paired": info["colpass_paired"],
             "row_kernel": {1: "four-step", 2: "fused"}[info["row_kernel"]],
             "row_transform": "direct" if info["direct"] else "bluestein",
+            "plan": plan_source,
         },
         "rel_l2_vs_fp64": rel,
         "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 5),
@@ -364,6 +405,7 @@ def main():
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "avg_launch_ms": round(dom_ms / max(dom_n, 1), 4),
         },
+        "default_plan": default_plan,
         "forms": forms,
         "streamed_to_host": streamed,
         "cpu_baseline": cpu,

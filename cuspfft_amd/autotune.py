@@ -37,8 +37,8 @@ def candidates(rows: int, cols: int, max_m1: int = 3) -> list[dict]:
     """Row-pass structure candidates: for both length rules, the four-step
     kernels with the planner's M1 and up to max_m1-1 alternatives with a fast
     convolution kernel, the BF16x3 MFMA form, and the fused row kernel at the
-    rule's M when it has a shape; plus the fused row kernel at the smallest
-    fused length M >= 2*cols-1."""
+    rule's M when it has a shape; plus the fused row kernel at the fused lengths
+    M in [2*cols-1, 1.12 (2*cols-1)] (at most four; else the smallest above)."""
     out = []
     for rule in (3, 7):
         base = describe_plan(rows, cols, length_rule=rule, row_kernel=ROW_FOUR_STEP)
@@ -61,9 +61,11 @@ def candidates(rows: int, cols: int, max_m1: int = 3) -> list[dict]:
         out.append(dict(length_rule=rule, m1=0, precision=PREC_BF16X3, row_kernel=ROW_FOUR_STEP))
         if describe_plan(rows, cols, length_rule=rule)["row_kernel"] == ROW_FUSED:
             out.append(dict(length_rule=rule, m1=0, precision=0, row_kernel=ROW_FUSED))
-    fl = [M for M in fused_lengths() if M >= 2 * cols - 1]
-    if fl:
-        out.append(dict(length_rule=3, fft_len=fl[0], m1=0, precision=0, row_kernel=ROW_FUSED))
+    # the fused row kernel at the fused lengths up to 12% above 2*cols-1: a longer length
+    # with cheaper (power-of-two) stage radices can be faster (c3: 36864 beats 34992)
+    fl = [M for M in fused_lengths() if 2 * cols - 1 <= M <= 1.12 * (2 * cols - 1)]
+    for M in (fl or [M for M in fused_lengths() if M >= 2 * cols - 1])[:4]:
+        out.append(dict(length_rule=3, fft_len=M, m1=0, precision=0, row_kernel=ROW_FUSED))
     # identical plans (e.g. both rules give the same M) are tried once
     seen, uniq = set(), []
     for o in out:

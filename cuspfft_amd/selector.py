@@ -209,9 +209,10 @@ class Selector:
 _default = None
 
 
-def choose_plan(m, max_m1: int = 2) -> dict:
+def choose_plan(m, max_m1: int = 2, fp32_only: bool = False) -> dict:
     """Plan options for matrix m (COOMatrix) chosen by the shipped network among
-    the autotuner's candidates (cuspfft_amd.autotune.candidates), for Plan(**opts)."""
+    the autotuner's candidates (cuspfft_amd.autotune.candidates), for Plan(**opts).
+    fp32_only: among the plain FP32 plans only (no BF16x3 precision form)."""
     global _default
     from . import autotune
     if _default is None:
@@ -219,6 +220,8 @@ def choose_plan(m, max_m1: int = 2) -> dict:
     rec = autotune.features_of(m)
     cands = [dict(length_rule=3)] + autotune.candidates(m.rows, m.cols, max_m1)
     cands += [dict(c, colpass_rows=r) for c in cands[1:] for r in autotune.COLPASS_ROWS]
+    if fp32_only:
+        cands = [c for c in cands if not c.get("precision")]
     return _default.choose(rec, cands)
 
 
