@@ -676,3 +676,35 @@ def test_batch_execute_and_graph(gpu, oracle_mod, streams):
                                           b.output_of(eager, i)[:, :V].cpu().numpy())
         b.execute(d_r, d_c, off, eager)
         torch.cuda.synchronize()
+
+
+def test_batch_graph_with_direct_widths(gpu, oracle_mod):
+    """A batch mixing direct-transform widths (1024, 1152, 2048) with Bluestein ones,
+    one tile per matrix (paired column passes for even rows): eager against the FP64
+    oracle, and three graph replays bit-identical to eager runs."""
+    import torch
+    sp = gpu
+    mats = [sp.generate("uniform", n, n * 6, seed=70 + i) for i, n in enumerate((1024, 1500, 1152, 2048, 999))]
+    b = sp.Batch([(m.rows, m.cols) for m in mats], opts=[dict(u_tile=m.rows) for m in mats],
+                 nnz_hint=[m.nnz for m in mats], streams=2)
+    assert [inf["direct"] for inf in b.infos] == [1, 0, 1, 1, 0]
+    d_r, d_c, off = sp.Batch.pack(mats)
+    arena = b.alloc_output()
+    b.execute(d_r, d_c, off, arena)
+    torch.cuda.synchronize()
+    b.check_input()
+    eager = arena.clone()
+    for i, m in enumerate(mats):
+        X = b.output_of(arena, i)[:, : m.cols // 2 + 1].cpu().numpy()
+        ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, nthreads=8)
+        assert oracle_mod.rel_l2(X, ref) <= REL_L2_PLAIN, i
+    g = b.capture(d_r, d_c, off, arena)
+    for rep in range(3):
+        arena.zero_()
+        g.launch()
+        torch.cuda.synchronize()
+        for i, m in enumerate(mats):
+            V = m.cols // 2 + 1
+            np.testing.assert_array_equal(b.output_of(arena, i)[:, :V].cpu().numpy(),
+                                          b.output_of(eager, i)[:, :V].cpu().numpy())
+
