@@ -51,6 +51,8 @@ def parse():
                     help="0 auto (rows u and R/2 - u share the column pass when R is even), 1 off")
     ap.add_argument("--row-transform", type=int, default=0,
                     help="0 auto (direct length-cols FFT when cols has a shape), 1 Bluestein, 2 direct")
+    ap.add_argument("--row-persist", type=int, default=0,
+                    help="fused row kernel: 0 persistent workgroups, 1 one workgroup per row group")
     ap.add_argument("--row-kernel", type=int, default=0,
                     help="0 auto (fused row kernel when M has a shape), 1 four-step kernels, 2 fused")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -232,13 +234,13 @@ def main():
     tile = args.tile if args.tile > 0 else m.rows
     explicit = (args.tile > 0 or args.fft_len or args.m1 or args.precision or args.rule != 3
                 or args.colpass_rows or args.colpass_table or args.row_kernel or args.colpass_pair
-                or args.row_transform)
+                or args.row_transform or args.row_persist)
     if explicit or args.plan == "default":
         plan_source = "options" if explicit else "default"
         opts = dict(length_rule=args.rule, m1=args.m1, fft_len=args.fft_len, precision=args.precision,
                     row_kernel=args.row_kernel, colpass_rows=args.colpass_rows,
                     colpass_table=args.colpass_table, colpass_pair=args.colpass_pair,
-                    row_transform=args.row_transform)
+                    row_transform=args.row_transform, row_persist=args.row_persist)
     else:
         # the shipped network's choice among the autotuner's plain FP32 candidates: the
         # same transform, the plan predicted fastest

@@ -29,6 +29,8 @@
 // accesses would keep the whole column live in registers.  RPW > 1 puts RPW
 // short rows in one workgroup (one descriptor for all, explicit bounds).
 #pragma once
+#include <algorithm>
+
 #include "buffer_io.h"
 #include "rowpass_internal.h"
 
@@ -48,6 +50,23 @@ namespace spfft {
 enum { kFwd = 0, kInv = 1, kPivot = 2 };
 // the W_M^e table: 256 low entries and (M + 255) / 256 high ones, M <= 65536
 constexpr int kFusedWmEntries = 256 + 256;
+
+// Persistent launches (resident workgroups looping over the row groups, the tables
+// loaded once) for the direct transform, whose rows are short: on p14 (16384^2) the
+// row kernel went 0.98 -> 0.84 ms.  The Bluestein rows are long enough that the
+// static share of rows to workgroups costs more at the tail than the launches save
+// (c2: 0.378 -> 0.393 ms), so they keep one workgroup per row group.  So do the
+// direct shapes whose loop spills more than 2 VGPRs at the 128-VGPR cap of 1024
+// threads (the compiler's report): (M1, RPW, radices) below.
+constexpr bool fused_persistent(int m1, int nb, int rpw, int r1, int r2, int r3, bool direct) {
+    constexpr int no[11][5] = {{21, 4, 8, 28, 1},  {18, 9, 12, 9, 1},  {18, 5, 16, 12, 1},
+                               {16, 4, 27, 9, 1},  {18, 4, 27, 9, 1},  {18, 4, 16, 16, 1},
+                               {16, 2, 27, 18, 1}, {18, 2, 27, 18, 1}, {18, 2, 8, 8, 8},
+                               {16, 1, 36, 27, 1}, {18, 1, 32, 32, 1}};
+    for (const auto& s : no)
+        if (m1 == s[0] && rpw == s[1] && r1 == s[2] && r2 == s[3] && r3 == s[4]) return false;
+    return direct && nb == 1;
+}
 
 // LDS row layout.  Power-of-two stage radices access a row with lane strides of
 // R entries (8 R bytes), which puts all lanes of a wave on a few LDS banks.  When
@@ -295,7 +314,22 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
     float2* s_tb = NBT ? s_ta + NA : s_ta;  // TP: the same for radix R1 (when R1 != RA)
     float2* buf = s_ta + NA + NBT;          // G rows of length M2: [row of workgroup][k1 of batch][n2]
     const int t = threadIdx.x;
-    const int row0 = blockIdx.x * RPW;  // first tile row of this workgroup
+    const bio::rsrc_t r_bh = bio::make_rsrc(a.bhat, (uint32_t)a.M * 8u);
+    // the twiddle tables, once per workgroup
+    auto fill_tables = [&]() {
+        for (int i = t; i < M2; i += NT) s_tw[i] = a.tw2[i];
+        for (int i = t; i < 256 + (a.M + 255) / 256; i += NT) s_wm[i] = a.wm2[i];
+        if constexpr (TP) {
+            constexpr int nsa = NA ? M2 / RA : 1, nsb = NBT ? M2 / R1 : 1;
+            for (int i = t; i < NA; i += NT) s_ta[i] = a.tw2[(i / nsa + 1) * (i % nsa)];
+            for (int i = t; i < NBT; i += NT) s_tb[i] = a.tw2[(i / nsb + 1) * (i % nsb)];
+        }
+    };
+
+    // One row group (the RPW rows from tile row row0); fill: the first one of the
+    // workgroup, which fills the tables once its loads are out.
+    auto row_group = [&](const int row0, const bool fill) {
+    // the thread's columns n2 (row g of the workgroup's RPW rows)
     int n2[CPT], g[CPT];
     bool live[CPT];
 #pragma unroll
@@ -305,10 +339,10 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
         n2[c] = q - g[c] * M2;
         live[c] = (RPW == 1 ? q < M2 : g[c] < RPW) && row0 + g[c] < a.tile_rows;
     }
-    const bio::rsrc_t r_bh = bio::make_rsrc(a.bhat, (uint32_t)a.M * 8u);
 
-    // phase A: the loads go out first, then the tables are filled (phase A's
-    // twiddles read W_M from LDS after a barrier)
+    // phase A: the loads go out first, then the tables are filled (first row group);
+    // the codelet's twiddles read W_M from LDS after a barrier, which also ends the
+    // previous row group's reads of buf
     float2 v[CPT][M1];
     {
         const int nrows = min(RPW, a.tile_rows - row0);
@@ -333,13 +367,7 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
                 }
             }
         }
-        for (int i = t; i < M2; i += NT) s_tw[i] = a.tw2[i];
-        for (int i = t; i < 256 + (a.M + 255) / 256; i += NT) s_wm[i] = a.wm2[i];
-        if constexpr (TP) {
-            constexpr int nsa = NA ? M2 / RA : 1, nsb = NBT ? M2 / R1 : 1;
-            for (int i = t; i < NA; i += NT) s_ta[i] = a.tw2[(i / nsa + 1) * (i % nsa)];
-            for (int i = t; i < NBT; i += NT) s_tb[i] = a.tw2[(i / nsb + 1) * (i % nsb)];
-        }
+        if (fill) fill_tables();
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
@@ -461,6 +489,16 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
             }
         }
     }
+    };  // row_group
+
+    if constexpr (fused_persistent(M1, NB, RPW, R1, R2, R3, DIRECT)) {
+        // persistent launch: the resident workgroups take the row groups blockIdx.x,
+        // blockIdx.x + gridDim.x, ... and keep the tables in LDS between them
+        const int nblk = (a.tile_rows + RPW - 1) / RPW;
+        for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) row_group(blk * RPW, blk == (int)blockIdx.x);
+    } else {
+        row_group(blockIdx.x * RPW, true);
+    }
 }
 
 // Instantiated shapes: X(M1, NB, RPW, NT, R1, R2, R3), M = M1 * R1 * R2 * R3 (a
@@ -522,7 +560,9 @@ hipError_t launch_fused_group_f(const RowPassArgs& a, hipStream_t s, bool prepar
 hipError_t launch_fused_group_g(const RowPassArgs& a, hipStream_t s, bool prepare_only);
 hipError_t launch_fused_group_h(const RowPassArgs& a, hipStream_t s, bool prepare_only);
 
-// (a.direct: the direct-transform instantiation, single-batch shapes only)
+// (a.direct: the direct-transform instantiation, single-batch shapes only; a.fused_cus > 0:
+// persistent launch, as many workgroups as can be resident on that many CUs, each looping
+// over row groups)
 #define SPFFT_FUSED_LAUNCH_CASE(m1, nb, rpw, nt, r1, r2, r3)                                       \
     if (a.M1 == m1 && a.fused_nb == nb && a.fused_rpw == rpw && a.fused_nt == nt &&                 \
         a.f2.radix[0] == r1 && (a.f2.nstages > 1 ? a.f2.radix[1] : 1) == r2 &&                      \
@@ -536,7 +576,19 @@ hipError_t launch_fused_group_h(const RowPassArgs& a, hipStream_t s, bool prepar
         if (prepare_only)                                                                           \
             return hipFuncSetAttribute(reinterpret_cast<const void*>(k),                            \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);       \
-        hipLaunchKernelGGL(k, dim3((a.tile_rows + rpw - 1) / rpw), dim3(nt), lds, s, a);            \
+        const int nblk = (a.tile_rows + rpw - 1) / rpw;                                             \
+        int grid = nblk;                                                                            \
+        if (a.fused_cus > 0 && fused_persistent(m1, nb, rpw, r1, r2, r3, a.direct != 0)) {         \
+            static int occ[2] = {0, 0}; /* resident workgroups per CU, per form */                  \
+            if (!occ[a.direct ? 1 : 0]) {                                                           \
+                int n = 0;                                                                          \
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, nt, lds) != hipSuccess)     \
+                    n = 1;                                                                          \
+                occ[a.direct ? 1 : 0] = std::max(n, 1);                                             \
+            }                                                                                       \
+            grid = std::min(nblk, a.fused_cus * occ[a.direct ? 1 : 0]);                             \
+        }                                                                                           \
+        hipLaunchKernelGGL(k, dim3(grid), dim3(nt), lds, s, a);                                     \
         return hipGetLastError();                                                                   \
     }
 

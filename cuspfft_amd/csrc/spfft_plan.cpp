@@ -144,6 +144,7 @@ void fft64(std::vector<cd>& x, int sign) {
 }
 
 size_t g_lds_limit = 0;
+int g_cu_count = 0;  // compute units of the device (persistent fused launches)
 constexpr size_t kGfx950Lds = 163840;
 std::mutex g_lds_mutex;
 
@@ -155,6 +156,8 @@ spfft_status ensure_lds_limit(size_t& budget) {
         SPFFT_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
                   "hipDeviceGetAttribute");
         g_lds_limit = (size_t)v;
+        SPFFT_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+        g_cu_count = v;
     }
     budget = std::min<size_t>(g_lds_limit, 65536);
     return SPFFT_OK;
@@ -580,6 +583,7 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     } else {
         return fail(SPFFT_ERR_INVALID_ARG, "length_rule must be 3 or 7");
     }
+    if (o.row_persist != 0 && o.row_persist != 1) return fail(SPFFT_ERR_INVALID_ARG, "row_persist must be 0 or 1");
     // the direct length-cols FFT: a single-batch fused shape of length cols
     if (o.row_transform < SPFFT_TRANSFORM_AUTO || o.row_transform > SPFFT_TRANSFORM_DIRECT)
         return fail(SPFFT_ERR_INVALID_ARG, "row_transform must be 0, 1 or 2");
@@ -748,6 +752,8 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
     st = plan_shape(rows, cols, opts_in, budget, g_lds_limit, p);
     if (st != SPFFT_OK) return st;
     RowPassArgs& a = p->base;
+    // fused row kernel: persistent launch over the device's CUs unless turned off
+    a.fused_cus = a.fused && p->opts.row_persist == 0 ? g_cu_count : 0;
     const int M = p->M;
     const int hs = a.hi_shift;
     const spfft_opts& o = p->opts;

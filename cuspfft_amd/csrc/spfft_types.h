@@ -60,6 +60,8 @@ struct RowPassArgs {
     int fused_nb, fused_rpw;   // k_rowfused: LDS batches per row, rows per workgroup
     int fused_nt;              // k_rowfused: threads per workgroup
     int direct;                // 1: k_rowfused runs the direct length-C FFT (M = C, no Bluestein)
+    int fused_cus;             // k_rowfused: > 0: persistent launch on this many CUs (resident
+                               //   workgroups loop over row groups); 0: one workgroup per group
     float2* X0;                // tile pass-1 output [u_local][ldx]
     float2* T;                 // tile workspace [u_local][k1][n2]
     float2* out;               // output base (row 0), pitch P: row u at out + u P

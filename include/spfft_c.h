@@ -58,6 +58,8 @@ typedef struct spfft_opts {
     int row_kernel;    /* SPFFT_ROW_*: how the Bluestein row pass runs                       */
     int colpass_pair;  /* SPFFT_PAIR_*: half-period row pairing in the column pass            */
     int row_transform; /* SPFFT_TRANSFORM_*: Bluestein, or a direct length-cols FFT            */
+    int row_persist;   /* fused row kernel: 0 persistent (the resident workgroups loop over the
+                          rows and load the twiddle tables once), 1 one workgroup per row group */
 } spfft_opts;
 
 /* Row-pass kernels.  FOUR_STEP: k_fft1 + k_conv + k_ifft1 with the tile buffer T
