@@ -50,7 +50,8 @@ def parse():
     ap.add_argument("--colpass-pair", type=int, default=0,
                     help="0 auto (rows u and R/2 - u share the column pass when R is even), 1 off")
     ap.add_argument("--row-transform", type=int, default=0,
-                    help="0 auto (direct length-cols FFT when cols has a shape), 1 Bluestein, 2 direct")
+                    help="0 auto (direct length-cols FFT, else Rader for cols = 2p, when there is a "
+                         "shape), 1 Bluestein, 2 direct, 3 Rader")
     ap.add_argument("--row-persist", type=int, default=0,
                     help="fused row kernel: 0 persistent workgroups, 1 one workgroup per row group")
     ap.add_argument("--row-kernel", type=int, default=0,
@@ -396,7 +397,7 @@ def main():
             "colpass_full_table": info["colpass_full_table"],
             "colpass_paired": info["colpass_paired"],
             "row_kernel": {1: "four-step", 2: "fused"}[info["row_kernel"]],
-            "row_transform": "direct" if info["direct"] else "bluestein",
+            "row_transform": "direct" if info["direct"] else "rader" if info.get("rader") else "bluestein",
             "plan": plan_source,
         },
         "rel_l2_vs_fp64": rel,

@@ -27,8 +27,9 @@ PREC_BF16X3 = 1
 ROW_AUTO, ROW_FOUR_STEP, ROW_FUSED = 0, 1, 2
 # column-pass half-period row pairing (spfft_opts.colpass_pair)
 PAIR_AUTO, PAIR_OFF = 0, 1
-# row transform (spfft_opts.row_transform): Bluestein, or a direct length-cols FFT
-TRANSFORM_AUTO, TRANSFORM_BLUESTEIN, TRANSFORM_DIRECT = 0, 1, 2
+# row transform (spfft_opts.row_transform): Bluestein, a direct length-cols FFT, or
+# Rader's convolutions for cols = 2p (p an odd prime)
+TRANSFORM_AUTO, TRANSFORM_BLUESTEIN, TRANSFORM_DIRECT, TRANSFORM_RADER = 0, 1, 2, 3
 
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "HIP", 3: "ALLOC", 4: "UNSUPPORTED", 5: "BAD_INPUT"}
 
@@ -57,7 +58,7 @@ class PlanInfo(ctypes.Structure):
                 ("colpass_rows", ctypes.c_int), ("colpass_full_table", ctypes.c_int),
                 ("precision", ctypes.c_int), ("workspace_bytes", ctypes.c_size_t),
                 ("row_kernel", ctypes.c_int), ("colpass_paired", ctypes.c_int),
-                ("direct", ctypes.c_int)]
+                ("direct", ctypes.c_int), ("rader", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -145,6 +146,7 @@ def load():
         "spfft_factor_3_smooth": (_i, [_i, _vp, _i]),
         "spfft_fused_lengths": (_i, [_vp, _i]),
         "spfft_direct_lengths": (_i, [_vp, _i]),
+        "spfft_rader_widths": (_i, [_vp, _i]),
         "spfft_coo_generate": (_i, [ctypes.c_char_p, _i, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
                                     ctypes.POINTER(HostCoo)]),
@@ -192,6 +194,13 @@ def direct_lengths() -> list[int]:
     """Widths cols with a direct length-cols transform (row_transform=TRANSFORM_DIRECT)."""
     buf = (ctypes.c_int * 256)()
     k = load().spfft_direct_lengths(ctypes.cast(buf, _vp), 256)
+    return list(buf[:min(k, 256)])
+
+
+def rader_widths() -> list[int]:
+    """Widths cols = 2p with a Rader row transform (row_transform=TRANSFORM_RADER)."""
+    buf = (ctypes.c_int * 256)()
+    k = load().spfft_rader_widths(ctypes.cast(buf, _vp), 256)
     return list(buf[:min(k, 256)])
 
 

@@ -33,7 +33,7 @@ LIB_SOURCES = ["csc_build.hip", "rowpass.hip", "rowpass_conv_a.hip", "rowpass_co
                "rowpass_fft1_a.hip", "rowpass_fft1_b.hip", "rowpass_fft1_c.hip", "rowpass_mfma.hip",
                "rowpass_fused_a.hip", "rowpass_fused_b.hip", "rowpass_fused_c.hip",
                "rowpass_fused_d.hip", "rowpass_fused_e.hip", "rowpass_fused_f.hip",
-               "rowpass_fused_g.hip", "rowpass_fused_h.hip",
+               "rowpass_fused_g.hip", "rowpass_fused_h.hip", "rowpass_rader.hip",
                "spfft_plan.cpp", "spfft_batch.cpp", "mlp.hip",
                "spfft_gen.cpp", "sparse_fft_shim.cpp"]
 CLI_SOURCES = ["cli_main.cpp"]
@@ -43,7 +43,7 @@ CLI_SOURCES = ["cli_main.cpp"]
 # non-zero / are needed: with signed zeros ignored, x + 0 folds to x.
 NO_SIGNED_ZEROS = {"rowpass_fused_a.hip", "rowpass_fused_b.hip", "rowpass_fused_c.hip",
                    "rowpass_fused_d.hip", "rowpass_fused_e.hip", "rowpass_fused_f.hip",
-                   "rowpass_fused_g.hip", "rowpass_fused_h.hip", "rowpass_fft1_a.hip",
+                   "rowpass_fused_g.hip", "rowpass_fused_h.hip", "rowpass_rader.hip", "rowpass_fft1_a.hip",
                    "rowpass_fft1_b.hip", "rowpass_fft1_c.hip"}
 
 

@@ -1,7 +1,8 @@
 // codelets.h -- compile-time FFT codelets held entirely in registers.
 //
 // RegFft<L>::run(v, inv) transforms v[0..L) in place (forward: exponent -,
-// inverse: exponent +, no scaling) for any 7-smooth L <= 64.  The codelet is
+// inverse: exponent +, no scaling) for any L <= 64 whose prime factors above 7
+// occur once (radices 8, 4, 2, 7, 5, 3, then an odd prime up to 61).  The codelet is
 // the register form of the Stockham autosort stages used everywhere in this
 // library (see spfft_device.h): the radix list of L is fixed at compile time,
 // every index is a compile-time constant, and every twiddle W_N^i is a
@@ -55,7 +56,7 @@ constexpr double sin2pi(long i, long n) {
     return q == 0 ? s : q == 1 ? c : q == 2 ? -s : -c;
 }
 
-// Radix list of a 7-smooth L <= 64 (largest radices first).
+// Radix list of L <= 64 (powers of two first, then 7, 5, 3 and odd primes up to 61).
 struct RadixList {
     int n;
     int r[8];
@@ -72,6 +73,12 @@ constexpr RadixList radix_list(int L) {
     while (L % 7 == 0) { out.r[out.n++] = 7; L /= 7; }
     while (L % 5 == 0) { out.r[out.n++] = 5; L /= 5; }
     while (L % 3 == 0) { out.r[out.n++] = 3; L /= 3; }
+    // larger odd primes (the Rader lengths): one stage each, a direct butterfly
+    for (int p = 11; p <= 61 && L > 1; p += 2) {
+        bool prime = true;
+        for (int d = 3; d * d <= p; d += 2) prime = prime && p % d != 0;
+        while (prime && L % p == 0) { out.r[out.n++] = p; L /= p; }
+    }
     if (L != 1) out.n = -1;
     return out;
 }
@@ -96,6 +103,27 @@ SPFFT_HD float2 twc(float2 v, bool inv) {
     return inv ? make_float2(fmaf(v.x, c, -v.y * s), fmaf(v.x, s, v.y * c))
                : make_float2(fmaf(v.x, c, v.y * s), fmaf(v.y, c, -v.x * s));
 }
+
+// Butterfly of an odd prime radix R > 7 (dft_odd) with compile-time cos / sin
+// tables: (R - 1)^2 real FMAs, no twiddle memory.
+template <int R>
+struct OddRoots {
+    float c[R], s[R];
+    constexpr OddRoots() : c{}, s{} {
+        for (int k = 0; k < R; k++) {
+            c[k] = (float)cx::cos2pi(k, R);
+            s[k] = (float)cx::sin2pi(k, R);
+        }
+    }
+};
+template <int R>
+struct Dft {
+    static_assert(R > 7 && R % 2 == 1, "radix 2, 3, 4, 5, 7 and 8 have their own butterflies");
+    static SPFFT_HD void run(float2* v, bool inv) {
+        constexpr OddRoots<R> t{};
+        dft_odd<R, (R - 1) / 2>(v, inv, t.c, t.s);
+    }
+};
 
 template <int L, int S, int Ns>
 struct RegStages {
@@ -130,7 +158,7 @@ struct RegStages {
 
 template <int L>
 struct RegFft {
-    static_assert(cx::radix_list(L).n >= 0, "RegFft length must be 7-smooth");
+    static_assert(cx::radix_list(L).n >= 0, "RegFft length has no radix list");
     // In-place on v[0..L).
     static SPFFT_HD void run(float2* v, bool inv) {
         if constexpr (L > 1) {

@@ -47,13 +47,16 @@ constexpr int kColWaves = kColThreads / 64;
 // cost (nonzeros + 1) from the largest down, each column to the least-loaded
 // wave.  One wave per tile; lane = column.  Output: the tile's columns grouped
 // by wave, sched[tile][0 .. TC), and the groups' offsets off[tile][0 .. kColWaves].
-// Columns past C are not listed.
+// Columns past C are not listed.  With a column order (colperm), position c of
+// the tile is matrix column colperm[c].
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_colsched(const int* __restrict__ colstart, int C, int TC,
+__global__ __launch_bounds__(64) void k_colsched(const int* __restrict__ colstart,
+                                                 const int* __restrict__ colperm, int C, int TC,
                                                  int* __restrict__ sched, int* __restrict__ off) {
     const int lane = threadIdx.x;
     const int c = blockIdx.x * TC + lane;
-    const int cost = lane < TC && c < C ? colstart[c + 1] - colstart[c] + 1 : 0;
+    const int mc = c < C && colperm ? colperm[c] : c;  // the matrix column at position c
+    const int cost = lane < TC && c < C ? colstart[mc + 1] - colstart[mc] + 1 : 0;
     const int cls = cost > 0 ? 31 - __clz(cost) : -1;
     int load[kColWaves] = {}, cnt[kColWaves] = {};
     int mine_w = 0, mine_k = 0;  // this lane's column: wave and position in its group
@@ -147,14 +150,26 @@ __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
     constexpr int NTILE = PAIR ? 2 : 1;  // row tiles: rows u, and (PAIR) rows H - u
     float* t_re = reinterpret_cast<float*>(FULL ? s_lo + R : s_hi + a.n_hi);  // [NTILE][64U][TS]
     float* t_im = t_re + NTILE * 64 * U * TS;                                  // [NTILE][64U][TS]
-    int* s_cs = reinterpret_cast<int*>(t_im + NTILE * 64 * U * TS);          // colstart[c0 .. c0+TC]
+    int* s_cs = reinterpret_cast<int*>(t_im + NTILE * 64 * U * TS);          // [TC] column starts
+    int* s_ce = s_cs + TC;                                                    // [TC] column ends
     if (FULL) {
         for (int i = threadIdx.x; i < (int)R; i += kColThreads) s_lo[i] = a.wr_full[i];
     } else {
         for (int i = threadIdx.x; i < nlo; i += kColThreads) s_lo[i] = a.wr_lo[i];
         for (int i = threadIdx.x; i < a.n_hi; i += kColThreads) s_hi[i] = a.wr_hi[i];
     }
-    for (int i = threadIdx.x; i <= TC; i += kColThreads) s_cs[i] = a.colstart[min(c0 + i, a.C)];
+    // X0 position c0 + i holds matrix column colperm[c0 + i] (identity when NULL)
+    for (int i = threadIdx.x; i < TC; i += kColThreads) {
+        const int pos = c0 + i;
+        int b = 0, e = 0;
+        if (pos < a.C) {
+            const int mc = a.colperm ? a.colperm[pos] : pos;
+            b = a.colstart[mc];
+            e = a.colstart[mc + 1];
+        }
+        s_cs[i] = b;
+        s_ce[i] = e;
+    }
     __syncthreads();
 
     const uint32_t mask = (uint32_t)nlo - 1u;
@@ -174,7 +189,7 @@ __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
 #pragma unroll
         for (int t = 0; t < U; t++) acc[t] = acc_o[t] = make_float2(0.f, 0.f);
         if (c < a.C) {
-            const int beg = s_cs[cc], end = s_cs[cc + 1];
+            const int beg = s_cs[cc], end = s_ce[cc];
             for (int p0 = beg; p0 < end; p0 += 64) {
                 const int n = min(64, end - p0);
                 // per-nonzero constants, one nonzero per lane
@@ -290,7 +305,7 @@ size_t colpass_lds_bytes(const RowPassArgs& a) {
     const int U = a.colpass_rows > 0 ? a.colpass_rows : 1;
     const int TC = colpass_tile_cols(U);
     // (paired: U/2 rows per lane and their U/2 partner rows: the same tile)
-    const size_t tile = (size_t)2 * 64 * U * (TC + 1) * sizeof(float) + (TC + 2) * sizeof(int);
+    const size_t tile = (size_t)2 * 64 * U * (TC + 1) * sizeof(float) + (size_t)2 * TC * sizeof(int);
     if (a.wr_full) return tile + (size_t)a.R * sizeof(float2);
     return tile + ((size_t)(1 << a.hi_shift) + a.n_hi) * sizeof(float2);
 }
@@ -302,11 +317,12 @@ size_t colsched_elems(int cols, size_t* off_elems) {
     return tiles * 8;
 }
 
-hipError_t launch_colsched(const int* colstart, int cols, int rows_per_lane, int* sched, int* off,
-                           hipStream_t s) {
+hipError_t launch_colsched(const int* colstart, const int* colperm, int cols, int rows_per_lane, int* sched,
+                           int* off, hipStream_t s) {
     if (cols <= 0) return hipSuccess;
     const int TC = colpass_tile_cols(rows_per_lane);
-    hipLaunchKernelGGL(k_colsched, dim3((cols + TC - 1) / TC), dim3(64), 0, s, colstart, cols, TC, sched, off);
+    hipLaunchKernelGGL(k_colsched, dim3((cols + TC - 1) / TC), dim3(64), 0, s, colstart, colperm, cols, TC, sched,
+                       off);
     return hipGetLastError();
 }
 
@@ -399,7 +415,7 @@ hipError_t launch_rowpass_tile(const RowPassArgs& a, bool packed, hipStream_t s,
     if (a.fused) {
         // one kernel for the whole row pass; the fft1 and ifft1 stages time as empty
         if (ev && (e = hipEventRecord(ev[2], s)) != hipSuccess) return e;
-        if ((e = launch_fused(a, s, false)) != hipSuccess) return e;
+        if ((e = a.rader ? launch_rader(a, s, false) : launch_fused(a, s, false)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[3], s)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[4], s)) != hipSuccess) return e;
         return hipSuccess;

@@ -41,8 +41,9 @@ size_t colpass_lds_bytes(const RowPassArgs& a);   // needs a.colpass_rows and a.
 // k_colsched: the per-tile column schedule of k_colpass (once per CSC build);
 // sizes in ints of sched (returned) and of off (*off_elems)
 size_t colsched_elems(int cols, size_t* off_elems);
-hipError_t launch_colsched(const int* colstart, int cols, int rows_per_lane, int* sched, int* off,
-                           hipStream_t s);
+// (colperm: NULL, or the column of each X0 position, as RowPassArgs::colperm)
+hipError_t launch_colsched(const int* colstart, const int* colperm, int cols, int rows_per_lane, int* sched,
+                           int* off, hipStream_t s);
 hipError_t colpass_prepare(size_t lds_limit);      // once per process, before launches
 size_t conv_lds_bytes(const RowPassArgs& a);
 // Fused row kernel (k_rowfused) shapes: M1, LDS batches, rows per workgroup,
@@ -54,6 +55,14 @@ struct FusedShape {
 bool fused_shape(int i, FusedShape* out);
 // prepare_only: raise the kernel's dynamic LDS limit (once per process, before launches)
 hipError_t launch_fused(const RowPassArgs& a, hipStream_t s, bool prepare_only);
+// Rader row kernel (k_rowrader, rowpass_rader.hip) shapes for widths C = 2p, p an odd
+// prime: M1, threads, stage radices r[3] with p - 1 = M1 r[0] r[1] r[2].
+struct RaderShape {
+    int m1, nt, r[3];
+    size_t lds_bytes;
+};
+bool rader_shape(int i, RaderShape* out);
+hipError_t launch_rader(const RowPassArgs& a, hipStream_t s, bool prepare_only);
 // ev: NULL, or 5 events recorded before k_colpass and after each of the 4 kernels
 // (with the fused row kernel: after k_colpass, twice, after k_rowfused, twice).
 hipError_t launch_rowpass_tile(const RowPassArgs& a, bool packed, hipStream_t s, hipEvent_t* ev);

@@ -107,7 +107,8 @@ bool plan_stages(int L, FftStages& f) {
     return true;
 }
 
-// FP64 Stockham FFT (same autosort form as the device kernels), any 7-smooth n.
+// FP64 Stockham FFT (same autosort form as the device kernels), any n: radices
+// 8, 4, 2, 7, 5, 3, then every remaining prime factor as a direct DFT stage.
 void fft64(std::vector<cd>& x, int sign) {
     const int n = (int)x.size();
     if (n <= 1) return;
@@ -116,23 +117,25 @@ void fft64(std::vector<cd>& x, int sign) {
         int m = n;
         for (int p : {8, 4, 2, 7, 5, 3})
             while (m % p == 0) { rad.push_back(p); m /= p; }
-        if (m != 1) return;
+        for (int p = 11; m > 1; p += 2) {
+            if ((long long)p * p > m) p = m;
+            while (m % p == 0) { rad.push_back(p); m /= p; }
+        }
     }
     std::vector<cd> y(n), tw(n);
     for (int i = 0; i < n; i++) tw[i] = unit_root(i, n, sign);
     int Ns = 1;
     for (int R : rad) {
         const int nb = n / R, step = n / (Ns * R);
-        std::vector<cd> F(R * R);
+        std::vector<cd> F((size_t)R * R), v(R), o(R);
         for (int a = 0; a < R; a++)
-            for (int b = 0; b < R; b++) F[a * R + b] = unit_root((long long)a * b, R, sign);
-        cd v[8], o[8];
+            for (int b = 0; b < R; b++) F[(size_t)a * R + b] = unit_root((long long)a * b, R, sign);
         for (int j = 0; j < nb; j++) {
             const int k = j % Ns;
             for (int r = 0; r < R; r++) v[r] = x[j + r * nb] * tw[(long long)r * k * step % n];
             for (int p = 0; p < R; p++) {
                 cd acc = 0;
-                for (int q = 0; q < R; q++) acc += F[p * R + q] * v[q];
+                for (int q = 0; q < R; q++) acc += F[(size_t)p * R + q] * v[q];
                 o[p] = acc;
             }
             const int d = (j / Ns) * Ns * R + k;
@@ -140,6 +143,38 @@ void fft64(std::vector<cd>& x, int sign) {
         }
         x.swap(y);
         Ns *= R;
+    }
+}
+
+bool is_odd_prime(long long n) {
+    if (n < 3 || n % 2 == 0) return false;
+    for (long long d = 3; d * d <= n; d += 2)
+        if (n % d == 0) return false;
+    return true;
+}
+
+long long pow_mod(long long b, long long e, long long m) {
+    long long r = 1 % m;
+    b %= m;
+    for (; e > 0; e >>= 1, b = b * b % m)
+        if (e & 1) r = r * b % m;
+    return r;
+}
+
+// The smallest primitive root g of the prime p: g^((p-1)/q) != 1 for every prime q | p - 1.
+long long primitive_root(long long p) {
+    std::vector<long long> qs;
+    long long n = p - 1;
+    for (long long q = 2; q * q <= n; q++)
+        if (n % q == 0) {
+            qs.push_back(q);
+            while (n % q == 0) n /= q;
+        }
+    if (n > 1) qs.push_back(n);
+    for (long long g = 2;; g++) {
+        bool ok = true;
+        for (long long q : qs) ok = ok && pow_mod(g, (p - 1) / q, p) != 1;
+        if (ok) return g;
     }
 }
 
@@ -332,7 +367,7 @@ spfft_status do_csc_build(spfft_plan_s& p, const int* d_row, const int* d_col, i
     // rows) once the tile has at least four such row blocks, else four
     p.colpass_rows = p.base.colpass_rows;
     if (p.colpass_rows == 0) p.colpass_rows = std::min(p.opts.u_tile, p.R / 2 + 1) >= 2048 ? 8 : 4;
-    SPFFT_HIP(spfft::launch_colsched(a.colstart, p.C, p.colpass_rows, p.col_sched.as<int>(),
+    SPFFT_HIP(spfft::launch_colsched(a.colstart, p.base.colperm, p.C, p.colpass_rows, p.col_sched.as<int>(),
                                      p.col_sched_off.as<int>(), s),
               "column schedule launch");
     if (ev) SPFFT_HIP(hipEventRecord(ev[1], s), "hipEventRecord");
@@ -507,6 +542,19 @@ int spfft_direct_lengths(int* out, int cap) {
     return (int)ms.size();
 }
 
+int spfft_rader_widths(int* out, int cap) {
+    std::vector<int> ws;
+    spfft::RaderShape r;
+    for (int i = 0; spfft::rader_shape(i, &r); i++) {
+        const int p = r.m1 * r.r[0] * r.r[1] * r.r[2] + 1;
+        if (is_odd_prime(p)) ws.push_back(2 * p);
+    }
+    std::sort(ws.begin(), ws.end());
+    ws.erase(std::unique(ws.begin(), ws.end()), ws.end());
+    for (int i = 0; out && i < (int)ws.size() && i < cap; i++) out[i] = ws[i];
+    return (int)ws.size();
+}
+
 int spfft_factor_3_smooth(int n, int* out, int cap) {
     int c3 = 0, k = 0;
     if (n < 1) return -1;
@@ -531,6 +579,18 @@ int find_fused(int M, int m1_req, size_t lds_limit, spfft::FusedShape* out, bool
             (one_batch && f.nb != 1))
             continue;
         *out = f;
+        return i;
+    }
+    return -1;
+}
+
+// Rader shape for the convolution length L = p - 1 (as a fused shape: one batch, the
+// two row groups of k_rowrader), or -1.
+int find_rader(int L, int m1_req, size_t lds_limit, spfft::FusedShape* out) {
+    spfft::RaderShape r;
+    for (int i = 0; spfft::rader_shape(i, &r); i++) {
+        if (r.m1 * r.r[0] * r.r[1] * r.r[2] != L || (m1_req && r.m1 != m1_req) || r.lds_bytes > lds_limit) continue;
+        *out = spfft::FusedShape{r.m1, 1, 2, r.nt, {r.r[0], r.r[1], r.r[2]}, r.lds_bytes};
         return i;
     }
     return -1;
@@ -585,16 +645,29 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     }
     if (o.row_persist != 0 && o.row_persist != 1) return fail(SPFFT_ERR_INVALID_ARG, "row_persist must be 0 or 1");
     // the direct length-cols FFT: a single-batch fused shape of length cols
-    if (o.row_transform < SPFFT_TRANSFORM_AUTO || o.row_transform > SPFFT_TRANSFORM_DIRECT)
-        return fail(SPFFT_ERR_INVALID_ARG, "row_transform must be 0, 1 or 2");
+    if (o.row_transform < SPFFT_TRANSFORM_AUTO || o.row_transform > SPFFT_TRANSFORM_RADER)
+        return fail(SPFFT_ERR_INVALID_ARG, "row_transform must be 0, 1, 2 or 3");
     spfft::FusedShape fd{};
-    const bool direct_ok = !mfma && o.fft_len == 0 && o.conv_rows == 0 && o.row_kernel != SPFFT_ROW_FOUR_STEP &&
-                           find_fused(cols, o.m1, lds_limit, &fd, true) >= 0;
+    const bool chip_ok = !mfma && o.fft_len == 0 && o.conv_rows == 0 && o.row_kernel != SPFFT_ROW_FOUR_STEP;
+    const bool direct_ok = chip_ok && find_fused(cols, o.m1, lds_limit, &fd, true) >= 0;
     if (o.row_transform == SPFFT_TRANSFORM_DIRECT && !direct_ok)
         return fail(SPFFT_ERR_UNSUPPORTED, "no direct transform for cols = " + std::to_string(cols) +
                                                " with these options");
-    const bool direct = direct_ok && o.row_transform != SPFFT_TRANSFORM_BLUESTEIN;
-    if (direct) M = cols;
+    const bool direct =
+        direct_ok && (o.row_transform == SPFFT_TRANSFORM_AUTO || o.row_transform == SPFFT_TRANSFORM_DIRECT);
+    // Rader: cols = 2p, p an odd prime, with a shape for the convolution length p - 1
+    const bool rader_ok = chip_ok && cols % 2 == 0 && is_odd_prime(cols / 2) &&
+                          find_rader(cols / 2 - 1, o.m1, lds_limit, &fd) >= 0;
+    if (o.row_transform == SPFFT_TRANSFORM_RADER && !rader_ok)
+        return fail(SPFFT_ERR_UNSUPPORTED, "no Rader transform for cols = " + std::to_string(cols) +
+                                               " with these options");
+    const bool rader =
+        !direct && rader_ok && (o.row_transform == SPFFT_TRANSFORM_AUTO || o.row_transform == SPFFT_TRANSFORM_RADER);
+    if (direct) {
+        find_fused(cols, o.m1, lds_limit, &fd, true);
+        M = cols;
+    }
+    if (rader) M = cols / 2 - 1;
     p->R = rows;
     p->C = cols;
     p->V = cols / 2 + 1;
@@ -607,8 +680,9 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     if (o.row_kernel < SPFFT_ROW_AUTO || o.row_kernel > SPFFT_ROW_FUSED)
         return fail(SPFFT_ERR_INVALID_ARG, "row_kernel must be 0, 1 or 2");
     spfft::FusedShape fs = fd;
-    const bool fused_ok = direct || (!mfma && o.conv_rows == 0 && o.row_kernel != SPFFT_ROW_FOUR_STEP &&
-                                     find_fused(M, o.m1, lds_limit, &fs) >= 0);
+    const bool fused_ok = direct || rader ||
+                          (!mfma && o.conv_rows == 0 && o.row_kernel != SPFFT_ROW_FOUR_STEP &&
+                           find_fused(M, o.m1, lds_limit, &fs) >= 0);
     if (o.row_kernel == SPFFT_ROW_FUSED && !fused_ok)
         return fail(SPFFT_ERR_UNSUPPORTED, "no fused row-kernel shape for M = " + std::to_string(M) +
                                                (mfma ? " in the BF16x3 form" : ""));
@@ -619,6 +693,8 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
         p->M2 = fs.r[0] * fs.r[1] * fs.r[2];
         a.fused = 1;
         a.direct = direct ? 1 : 0;
+        a.rader = rader ? 1 : 0;
+        a.rader_p = rader ? cols / 2 : 0;
         a.fused_nb = fs.nb;
         a.fused_rpw = fs.rpw;
         a.fused_nt = fs.nt;
@@ -702,6 +778,7 @@ void fill_info(const spfft_plan_s* p, spfft_plan_info* info) {
     info->row_kernel = p->base.fused ? SPFFT_ROW_FUSED : SPFFT_ROW_FOUR_STEP;
     info->colpass_paired = pair_eligible(*p) ? 1 : 0;
     info->direct = p->base.direct;
+    info->rader = p->base.rader;
 }
 
 }  // namespace
@@ -762,8 +839,8 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
     std::vector<float2> h(cols), lo(1 << hs), hi(a.n_hi), t2(p->M2), tm((size_t)M), bh((size_t)M);
     for (int k = 0; k < cols; k++) {
         const long long k2 = (long long)k * k % (2LL * cols);
-        // exp(-i pi k^2 / C); the direct transform has no chirp (the column pass takes h = 1)
-        h[k] = a.direct ? make_float2(1.f, 0.f) : to_f2(unit_root(k2, 2LL * cols, -1));
+        // exp(-i pi k^2 / C); the direct and Rader transforms have no chirp (h = 1)
+        h[k] = a.direct || a.rader ? make_float2(1.f, 0.f) : to_f2(unit_root(k2, 2LL * cols, -1));
     }
     for (int l = 0; l < (1 << hs); l++) lo[l] = to_f2(unit_root(l, rows, -1));
     for (int j = 0; j < a.n_hi; j++) hi[j] = to_f2(unit_root((long long)j << hs, rows, -1));
@@ -771,7 +848,39 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
     for (int k1 = 0; k1 < p->M1; k1++)
         for (int n2 = 0; n2 < p->M2; n2++)
             tm[(size_t)k1 * p->M2 + n2] = to_f2(unit_root((long long)k1 * n2, M, -1));
-    if (!a.direct) {
+    if (a.rader) {
+        // g a primitive root of p, L = p - 1 = M: kt[m] = g^-m mod p; b[q] = W_p^{g^-q};
+        // X0 position s L + q holds column 2 g^q + s, positions 2L, 2L + 1 columns 0, 1;
+        // W_C two-level table for the combination of the halves
+        const long long pr = a.rader_p, g = primitive_root(pr), ginv = pow_mod(g, pr - 2, pr);
+        std::vector<int> kt((size_t)M), perm((size_t)cols);
+        std::vector<cd> b((size_t)M);
+        long long gm = 1, gq = 1;
+        for (int m = 0; m < M; m++) {
+            kt[m] = (int)gm;
+            b[m] = unit_root(gm, pr, -1);
+            perm[m] = (int)(2 * gq);
+            perm[(size_t)M + m] = (int)(2 * gq + 1);
+            gm = gm * ginv % pr;
+            gq = gq * g % pr;
+        }
+        perm[2 * (size_t)M] = 0;
+        perm[2 * (size_t)M + 1] = 1;
+        fft64(b, -1);
+        const double inv_m = 1.0 / (double)M;
+        for (int k1 = 0; k1 < p->M1; k1++)
+            for (int k2 = 0; k2 < p->M2; k2++)
+                bh[(size_t)k1 * p->M2 + k2] = to_f2(b[(size_t)k1 + (size_t)p->M1 * k2] * inv_m);
+        std::vector<float2> wc(256 + (pr + 255) / 256);
+        for (int e = 0; e < 256; e++) wc[e] = to_f2(unit_root(e, cols, -1));
+        for (int i = 0; i < (int)(pr + 255) / 256; i++) wc[256 + i] = to_f2(unit_root(256LL * i, cols, -1));
+        SPFFT_HIP(upload(p->rader_k, kt), "upload Rader indices");
+        SPFFT_HIP(upload(p->colperm, perm), "upload Rader column order");
+        SPFFT_HIP(upload(p->wc2, wc), "upload twiddles");
+        a.rader_k = p->rader_k.as<int>();
+        a.colperm = p->colperm.as<int>();
+        a.wc2 = p->wc2.as<float2>();
+    } else if (!a.direct) {
         // b[0] = 1, b[k] = b[M-k] = exp(+i pi k^2 / C) for 1 <= k < C
         std::vector<cd> b((size_t)M, cd(0, 0));
         b[0] = cd(1, 0);
@@ -850,9 +959,11 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
     a.mfma_dft = a.precision == 1 ? p->mfma_dft.as<float>() : nullptr;
     a.T = p->T.as<float2>();
     a.X0 = p->X0.as<float2>();
-    if (a.fused) SPFFT_HIP(spfft::launch_fused(a, nullptr, true), "fused row kernel LDS limit");
+    if (a.fused)
+        SPFFT_HIP(a.rader ? spfft::launch_rader(a, nullptr, true) : spfft::launch_fused(a, nullptr, true),
+                  "fused row kernel LDS limit");
     p->table_bytes = p->h.bytes + p->wr_lo.bytes + p->wr_hi.bytes + p->wr_full.bytes + p->tw2.bytes + p->twM.bytes +
-                     p->wm2.bytes +
+                     p->wm2.bytes + p->rader_k.bytes + p->colperm.bytes + p->wc2.bytes +
                      p->bhat.bytes + p->T.bytes + p->X0.bytes;
     *out = guard.release();
     return SPFFT_OK;

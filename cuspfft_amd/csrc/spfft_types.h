@@ -62,6 +62,12 @@ struct RowPassArgs {
     int direct;                // 1: k_rowfused runs the direct length-C FFT (M = C, no Bluestein)
     int fused_cus;             // k_rowfused: > 0: persistent launch on this many CUs (resident
                                //   workgroups loop over row groups); 0: one workgroup per group
+    int rader;                 // 1: k_rowrader (C = 2p, p an odd prime, M = p - 1; fused = 1)
+    int rader_p;               // k_rowrader: p
+    const int* rader_k;        // k_rowrader: [m] = g^-m mod p, m < p - 1 (g a primitive root)
+    const float2* wc2;         // k_rowrader: W_C^e = wc2[256 + (e >> 8)] * wc2[e & 255], e < p
+    const int* colperm;        // column pass: X0 column (position) i holds matrix column
+                               //   colperm[i] (NULL: the identity; k_rowrader's order)
     float2* X0;                // tile pass-1 output [u_local][ldx]
     float2* T;                 // tile workspace [u_local][k1][n2]
     float2* out;               // output base (row 0), pitch P: row u at out + u P

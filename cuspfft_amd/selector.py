@@ -211,10 +211,16 @@ _default = None
 
 def choose_plan(m, max_m1: int = 2, fp32_only: bool = False) -> dict:
     """Plan options for matrix m (COOMatrix) chosen by the shipped network among
-    the autotuner's candidates (cuspfft_amd.autotune.candidates), for Plan(**opts).
+    the autotuner's candidates (cuspfft_amd.autotune.candidates), for Plan(**opts);
+    the planner's plan for widths with a Rader transform.
     fp32_only: among the plain FP32 plans only (no BF16x3 precision form)."""
     global _default
     from . import autotune
+    from . import describe_plan
+    if describe_plan(m.rows, m.cols)["rader"]:
+        # Rader's convolutions (half the Bluestein row-pass work) are not among the
+        # plans the network was trained on: the planner's plan
+        return dict(length_rule=3)
     if _default is None:
         _default = Selector.load(DEFAULT_MODEL)
     rec = autotune.features_of(m)

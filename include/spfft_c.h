@@ -57,7 +57,7 @@ typedef struct spfft_opts {
                           rows) are computed: a sampled run for timing (the autotuner) */
     int row_kernel;    /* SPFFT_ROW_*: how the Bluestein row pass runs                       */
     int colpass_pair;  /* SPFFT_PAIR_*: half-period row pairing in the column pass            */
-    int row_transform; /* SPFFT_TRANSFORM_*: Bluestein, or a direct length-cols FFT            */
+    int row_transform; /* SPFFT_TRANSFORM_*: Bluestein, a direct length-cols FFT, or Rader     */
     int row_persist;   /* fused row kernel: 0 persistent (the resident workgroups loop over the
                           rows and load the twiddle tables once), 1 one workgroup per row group */
 } spfft_opts;
@@ -80,9 +80,14 @@ enum { SPFFT_PAIR_AUTO = 0, SPFFT_PAIR_OFF = 1 };
  * of the reference's variants.  DIRECT: one FFT of length cols itself when cols has a
  * fused-kernel shape holding the whole row on chip (cols in spfft_direct_lengths();
  * FP32 form, no fft_len / conv_rows / four-step options): no chirps, a quarter of the
- * row-pass work.  AUTO: DIRECT when possible, else BLUESTEIN.  The transform computed
- * is the same; only the rounding differs. */
-enum { SPFFT_TRANSFORM_AUTO = 0, SPFFT_TRANSFORM_BLUESTEIN = 1, SPFFT_TRANSFORM_DIRECT = 2 };
+ * row-pass work.  RADER: for cols = 2p, p an odd prime with a Rader shape for p - 1
+ * (cols in spfft_rader_widths(); same restrictions): the two length-p halves of the
+ * row DFT (even and odd columns) as cyclic convolutions of length p - 1 (fft_len =
+ * p - 1), about half the row-pass work of BLUESTEIN.  AUTO: DIRECT, else RADER, when
+ * possible, else BLUESTEIN.  The transform computed is the same; only the rounding
+ * differs. */
+enum { SPFFT_TRANSFORM_AUTO = 0, SPFFT_TRANSFORM_BLUESTEIN = 1, SPFFT_TRANSFORM_DIRECT = 2,
+       SPFFT_TRANSFORM_RADER = 3 };
 
 typedef struct spfft_plan_info {
     int rows, cols;
@@ -103,6 +108,7 @@ typedef struct spfft_plan_info {
     int row_kernel;         /* SPFFT_ROW_FOUR_STEP or SPFFT_ROW_FUSED                   */
     int colpass_paired;     /* 1 if the column pass pairs rows u and R/2 - u            */
     int direct;             /* 1: direct length-cols FFT (fft_len = cols), 0: Bluestein  */
+    int rader;              /* 1: Rader convolutions of length fft_len = cols/2 - 1      */
 } spfft_plan_info;
 
 typedef struct spfft_plan_s* spfft_plan;
@@ -280,6 +286,10 @@ int spfft_fused_lengths(int* out, int cap);
  * Replaces the reference's power-of-two-only plain FFT paths for such widths
  * (README.md:653-673: Bluestein doubles the work of a power-of-two width). */
 int spfft_direct_lengths(int* out, int cap);
+/* Widths cols = 2p with a Rader row transform (SPFFT_TRANSFORM_RADER), increasing; same
+ * convention.  No counterpart in the reference, whose every non-power-of-two width
+ * goes through Bluestein (src/sparse_fft_bluestein.cu:2327-2370). */
+int spfft_rader_widths(int* out, int cap);
 
 /* Seeded synthetic binary matrices (host memory, canonical COO: deduplicated,
  * sorted by (row, col), zero diagonal, symmetric).  Release with spfft_coo_free. */

@@ -63,8 +63,8 @@ def test_planner_covers_many_shapes():
 def test_planner_config_choices():
     import cuspfft_amd as sp
     for cols, m3, m7 in [(2310, 5184, 4704), (8219, 17496, 16464), (16706, 34992, 33600)]:
-        a = sp.describe_plan(cols, cols, length_rule=3)
-        b = sp.describe_plan(cols, cols, length_rule=7)
+        a = sp.describe_plan(cols, cols, length_rule=3, row_transform=sp.TRANSFORM_BLUESTEIN)
+        b = sp.describe_plan(cols, cols, length_rule=7, row_transform=sp.TRANSFORM_BLUESTEIN)
         assert a["fft_len"] == m3 and b["fft_len"] == m7
         assert a["conv_fast"] == 1 and b["conv_fast"] == 1
 
@@ -105,7 +105,36 @@ def test_planner_direct_transform():
         with pytest.raises(sp.SpfftError):
             sp.describe_plan(300, cols, row_transform=sp.TRANSFORM_DIRECT)
     with pytest.raises(sp.SpfftError):
-        sp.describe_plan(300, 2048, row_transform=3)
+        sp.describe_plan(300, 2048, row_transform=4)
+
+
+def test_planner_rader_transform():
+    """Widths 2p (p an odd prime with a shape for p - 1) take Rader's convolutions of
+    length p - 1 by default; Bluestein when asked, or when an option rules the on-chip
+    kernels out; other widths reject TRANSFORM_RADER."""
+    import cuspfft_amd as sp
+    widths = sp.rader_widths()
+    assert widths == sorted(widths) and 16706 in widths
+    for cols in widths:
+        p = cols // 2
+        assert cols % 2 == 0 and all(p % d for d in range(2, int(p ** 0.5) + 1))
+        for rows in (7, 64, cols):
+            info = sp.describe_plan(rows, cols)
+            assert info["rader"] == 1 and info["direct"] == 0 and info["fft_len"] == p - 1
+            assert info["m1"] * info["m2"] == p - 1 and info["row_kernel"] == sp.ROW_FUSED
+            assert sp.describe_plan(rows, cols, row_transform=sp.TRANSFORM_RADER)["rader"] == 1
+            b = sp.describe_plan(rows, cols, row_transform=sp.TRANSFORM_BLUESTEIN)
+            assert b["rader"] == 0 and b["fft_len"] >= 2 * cols - 1
+        for kw in (dict(precision=sp.PREC_BF16X3), dict(row_kernel=sp.ROW_FOUR_STEP),
+                   dict(fft_len=sp.next_3_smooth(2 * cols - 1)), dict(conv_rows=4)):
+            assert sp.describe_plan(64, cols, **kw)["rader"] == 0
+            with pytest.raises(sp.SpfftError):
+                sp.describe_plan(64, cols, row_transform=sp.TRANSFORM_RADER, **kw)
+    # prime halves without a shape, odd widths, smooth widths
+    for cols in (2 * 8363, 16707, 2310, 2048, 6):
+        assert sp.describe_plan(64, cols)["rader"] == 0
+        with pytest.raises(sp.SpfftError):
+            sp.describe_plan(64, cols, row_transform=sp.TRANSFORM_RADER)
 
 
 def test_planner_fused_row_kernel():
@@ -116,7 +145,7 @@ def test_planner_fused_row_kernel():
     for cols, M in [(2310, 5184), (2310, 4704), (8219, 17496), (8219, 16464), (16706, 34992),
                     (16706, 33600)]:
         rule = 3 if sp.next_3_smooth(2 * cols - 1) == M else 7
-        info = sp.describe_plan(cols, cols, length_rule=rule)
+        info = sp.describe_plan(cols, cols, length_rule=rule, row_transform=sp.TRANSFORM_BLUESTEIN)
         assert info["fft_len"] == M and info["row_kernel"] == sp.ROW_FUSED
         assert info["m1"] * info["m2"] == M and int(np.prod(info["conv_radix"])) == info["m2"]
         four = sp.describe_plan(cols, cols, length_rule=rule, row_kernel=sp.ROW_FOUR_STEP)

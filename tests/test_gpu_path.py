@@ -585,6 +585,51 @@ def test_direct_single_nonzero_closed_form(gpu):
     assert np.abs(X - ref).max() < 2e-5
 
 
+# ---- Rader row transform: widths 2p, p an odd prime with a shape for p - 1 ------
+RADER_CASES = [(1858, 50, 2000, 50), (1858, 7, 40, 3), (1858, 1858, 12000, 1858), (4178, 100, 5000, 33),
+               (4178, 4177, 20000, 4177), (16706, 64, 3000, 64), (16706, 5, 300, 2)]
+
+
+@pytest.mark.parametrize("cols,rows,nnz,tile", RADER_CASES)
+def test_rader_transform(gpu, oracle_mod, cols, rows, nnz, tile):
+    """Rader's convolutions against the FP64 oracle and the Bluestein plan of the same
+    matrix (odd and even rows, several tiles, the paired column pass); the streamed
+    output of the Rader plan is bit-identical to it."""
+    import torch
+    sp = gpu
+    assert cols in sp.rader_widths()
+    rng = np.random.default_rng(cols + rows)
+    r, c = _random_coo(rng, rows, cols, nnz)
+    print(f"[rader] {rows}x{cols} nnz={len(r)} tile={tile}", flush=True)
+    plan, X = _run(sp, rows, cols, r, c, u_tile=tile)
+    assert plan.info["rader"] == 1 and plan.info["fft_len"] == cols // 2 - 1
+    _, Xb = _run(sp, rows, cols, r, c, u_tile=tile, row_transform=sp.TRANSFORM_BLUESTEIN)
+    ref = oracle_mod.sparse_rfft2_f64(rows, cols, r, c, nthreads=8)
+    err, errb = oracle_mod.rel_l2(X, ref), oracle_mod.rel_l2(Xb, ref)
+    print(f"  rel_l2 rader={err:.3e} bluestein={errb:.3e}")
+    assert err <= REL_L2_PLAIN and errb <= REL_L2_PLAIN
+    if rows <= 2310:
+        host = plan.alloc_host_output()
+        host.fill_(complex(float("nan"), float("nan")))
+        plan.execute_to_host(_dev(r), _dev(c), host)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(host[:, : cols // 2 + 1].numpy(), X)
+
+
+@pytest.mark.parametrize("c0", [0, 1, 2, 3, 8352, 16705])
+def test_rader_single_nonzero_closed_form(gpu, c0):
+    """One nonzero per column class: columns 0 and 1 (x_s[0] of the two halves), the
+    first columns of each Rader run and the last column."""
+    sp = gpu
+    rows, cols, r0 = 1000, 16706, 999
+    plan, X = _run(sp, rows, cols, np.array([r0], np.int32), np.array([c0], np.int32), u_tile=rows)
+    assert plan.info["rader"] == 1
+    u = np.arange(rows)[:, None]
+    v = np.arange(cols // 2 + 1)[None, :]
+    ref = np.exp(-2j * np.pi * (((u * r0) % rows) / rows + ((v * c0) % cols) / cols))
+    assert np.abs(X - ref).max() < 2e-5
+
+
 # ---- streamed output: device chunks copied to pinned host memory --------------
 STREAM_CASES = [((2310, 2310, 16000), 128, {}), ((2310, 2310, 16000), 2310, {}),
                 ((2310, 2310, 16000), 500, dict(row_kernel=1)),
