@@ -308,11 +308,13 @@ def main():
     traffic = None
     try:
         with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")) as f:
-            rec = json.load(f).get(args.config, {}).get(dom)
+            recs = json.load(f).get(args.config, {}).get(dom) or []
         info0 = plan.info
-        if (rec and rec.get("fft_len") == info0["fft_len"] and rec.get("m1") == info0["m1"]
-                and rec.get("row_kernel") == info0["row_kernel"] and args.tile <= 0):
-            traffic = rec["bytes"]
+        for rec in recs if isinstance(recs, list) else [recs]:
+            if (rec.get("fft_len") == info0["fft_len"] and rec.get("m1") == info0["m1"]
+                    and rec.get("row_kernel") == info0["row_kernel"] and args.tile <= 0):
+                traffic = rec["bytes"]
+                break
     except (OSError, ValueError):
         traffic = None
 
