@@ -76,10 +76,30 @@ constexpr bool fused_persistent(int m1, int nb, int rpw, int r1, int r2, int r3,
 // per-thread base plus a compile-time offset, and x + y keeps that form in the
 // padded layout iff (x mod 32) + (y mod 32) < 32 for all its bases x and offsets
 // y -- checked at compile time for every stage of the shape (fused_pad_ok).
+// P = 2 pads by one entry every 8 (entry i at i + floor(i / 8)), for radix-8 stages of
+// rows whose length is a multiple of 8 but not of 32 (the Rader kernel); accesses that
+// do not keep the base + offset form in that layout compute their index per element.
 template <int P>
-__host__ __device__ constexpr int lds_idx(int i) { return P ? i + (i >> 5) : i; }
+__host__ __device__ constexpr int lds_idx(int i) { return P == 2 ? i + (i >> 3) : P ? i + (i >> 5) : i; }
 template <int L, int P>
-constexpr int kRowPitch = P ? L + L / 32 : L;
+constexpr int kRowPitch = P == 2 ? L + L / 8 : P ? L + L / 32 : L;
+
+// For padding period m: does x + y keep the per-thread-base + compile-time-offset
+// form ((x mod m) + (y mod m) < m for every base x and offset y) for the reads
+// (write = false) or the writes of one stage?  Bases and offsets as in pad_stage_ok.
+constexpr bool stage_split_ok(int L, int R, int NS, bool pivot, int m, bool write) {
+    const int nb = L / R;
+    int base = 0, off = 0;
+    for (int j = 0; j < nb; j++) {
+        const int b = !write ? j : pivot ? j * R : (j / NS) * NS * R + j % NS;
+        base = b % m > base ? b % m : base;
+    }
+    for (int r = 0; r < R; r++) {
+        const int o = !write ? r * nb : pivot ? r : r * NS;
+        off = o % m > off ? o % m : off;
+    }
+    return base + off < m;
+}
 
 // (x mod 32) + (y mod 32) < 32 for the bases x and offsets y of one stage:
 // reads j + r (L / R); writes ((j / NS) NS R + j % NS) + r NS, or (pivot: the
@@ -196,6 +216,9 @@ __device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const
     constexpr bool INV = MODE == kInv;
     static_assert(MODE != kPivot || NS == nb, "the pivot stage is the last forward stage");
     static_assert(!TPR || step == 1, "the r-major table serves the stages with NS R = L");
+    // (P = 1 shapes satisfy both at every stage: fused_pad_ok)
+    constexpr bool RSPLIT = P != 2 || stage_split_ok(L, R, NS, MODE == kPivot, 8, false);
+    constexpr bool WSPLIT = P != 2 || stage_split_ok(L, R, NS, MODE == kPivot, 8, true);
     float2 v[MB][R];
 #pragma unroll
     for (int b = 0; b < MB; b++) {
@@ -203,9 +226,9 @@ __device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const
         if (t < total) {
             const int g = t / nb;
             const int j = t - g * nb;
-            const float2* src = buf + g * LP + lds_idx<P>(j);
+            const float2* src = buf + g * LP + (RSPLIT ? lds_idx<P>(j) : 0);
 #pragma unroll
-            for (int r = 0; r < R; r++) v[b][r] = src[lds_idx<P>(r * nb)];
+            for (int r = 0; r < R; r++) v[b][r] = src[RSPLIT ? lds_idx<P>(r * nb) : lds_idx<P>(j + r * nb)];
         }
     }
     __syncthreads();
@@ -238,9 +261,10 @@ __device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const
                 base = j * R;
                 stride = 1;
             }
-            float2* dst = buf + g * LP + lds_idx<P>(base);
+            float2* dst = buf + g * LP + (WSPLIT ? lds_idx<P>(base) : 0);
 #pragma unroll
-            for (int r = 0; r < R; r++) dst[lds_idx<P>(r * stride)] = v[b][r];
+            for (int r = 0; r < R; r++)
+                dst[WSPLIT ? lds_idx<P>(r * stride) : lds_idx<P>(base + r * stride)] = v[b][r];
         }
     }
     __syncthreads();

@@ -30,16 +30,26 @@ namespace {
 // LDS entries (float2) of the row buffer: the two row groups of the convolution,
 // or afterwards E_0 and E_1 in natural order (2p = 2L + 2 entries)
 constexpr int rader_buf(int m1, int m2, int lp) { return 2 * m1 * lp > 2 * m1 * m2 + 2 ? 2 * m1 * lp : 2 * m1 * m2 + 2; }
-constexpr size_t rader_lds_bytes(int m1, int nt, int r1, int r2, int r3, bool pad, bool tp) {
-    const int m2 = r1 * r2 * r3;
-    const int lp = pad ? m2 + m2 / 32 : m2;
+// two-level tables of W_n^e, e < n: 256 low entries and ceil(n / 256) high ones
+constexpr int rader_w2(int n) { return 256 + (n + 255) / 256; }
+constexpr size_t rader_lds_bytes(int m1, int nt, int r1, int r2, int r3, int pad, bool tp) {
+    const int m2 = r1 * r2 * r3, L = m1 * m2;
+    const int lp = pad == 2 ? m2 + m2 / 8 : pad ? m2 + m2 / 32 : m2;
     const int tabs = tp ? fused_tp_a(r1, r2, r3) + fused_tp_b(r1, r2, r3) : 0;
-    // W_M2, W_L, the r-major tables, the row buffer, W_C, the partial sums
-    return (size_t)(m2 + kFusedWmEntries + tabs + rader_buf(m1, m2, lp) + kFusedWmEntries + 2 * (nt / 64)) *
+    // W_M2, W_L, the r-major tables, the row buffer, W_C (C = 2p = 2L + 2), the partial sums
+    return (size_t)(m2 + rader_w2(L) + tabs + rader_buf(m1, m2, lp) + rader_w2(L + 1) + 2 * (nt / 64)) *
            sizeof(float2);
 }
-constexpr bool rader_pad(int m1, int nt, int r1, int r2, int r3) {
-    return fused_pad_ok(r1, r2, r3) && rader_lds_bytes(m1, nt, r1, r2, r3, true, false) <= 160 * 1024;
+// Row padding: 1 (entry i at i + i/32) where every stage keeps its compile-time
+// offsets, else 2 (i + i/8: radix-8 stages write at a stride of 9 entries instead of
+// 8, some accesses compute their index per element) for rows of a multiple of 8
+// entries with a radix-8 stage, else none; only if the padded rows fit.
+constexpr int rader_pad(int m1, int nt, int r1, int r2, int r3) {
+    const int m2 = r1 * r2 * r3;
+    if (fused_pad_ok(r1, r2, r3) && rader_lds_bytes(m1, nt, r1, r2, r3, 1, false) <= 160 * 1024) return 1;
+    if (m2 % 8 == 0 && (r1 == 8 || r2 == 8 || r3 == 8) && rader_lds_bytes(m1, nt, r1, r2, r3, 2, false) <= 160 * 1024)
+        return 2;
+    return 0;
 }
 constexpr bool rader_tp(int m1, int nt, int r1, int r2, int r3) {
     return fused_tp_a(r1, r2, r3) > 0 &&
@@ -58,7 +68,7 @@ __global__ __launch_bounds__(NT) void k_rowrader(RowPassArgs a) {
     constexpr int NW = NT / 64;
     static_assert(NT % 64 == 0 && 2 * M2 <= NT, "one column n2 of one row group per thread");
     static_assert(R1 > 1 && (R2 > 1 || R3 == 1), "stage radices first");
-    constexpr int P = rader_pad(M1, NT, R1, R2, R3) ? 1 : 0;
+    constexpr int P = rader_pad(M1, NT, R1, R2, R3);
     constexpr int LP = kRowPitch<M2, P>;
     constexpr int TP = rader_tp(M1, NT, R1, R2, R3) ? 1 : 0;
     constexpr int RA = fused_tp_radix(R2, R3), NA = TP ? fused_tp_a(R1, R2, R3) : 0;
@@ -67,11 +77,11 @@ __global__ __launch_bounds__(NT) void k_rowrader(RowPassArgs a) {
     extern __shared__ float2 smem[];
     float2* s_tw = smem;                    // W_M2^i, i < M2
     float2* s_wm = s_tw + M2;               // W_L^e: 256 low + (L + 255) / 256 high entries
-    float2* s_ta = s_wm + kFusedWmEntries;  // TP: r-major tables of the pivot stage ...
+    float2* s_ta = s_wm + rader_w2(L);      // TP: r-major tables of the pivot stage ...
     float2* s_tb = NBT ? s_ta + NA : s_ta;  // ... and of the last inverse stage
     float2* buf = s_ta + NA + NBT;          // [s][k1][n2] rows, then E_0 | E_1
     float2* s_wc = buf + NBUF;              // W_C^e: 256 low + (p + 255) / 256 high entries
-    float2* s_red = s_wc + kFusedWmEntries; // [wave][s]: partial sums of a_s
+    float2* s_red = s_wc + rader_w2(L + 1); // [wave][s]: partial sums of a_s
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const int s = t / M2, n2 = t - s * M2;  // row group and column (t < 2 M2)
@@ -179,7 +189,9 @@ __global__ __launch_bounds__(NT) void k_rowrader(RowPassArgs a) {
 //   8352 = 36 * (29 * 8)   (width 16706 = 2 * 8353; 512 threads, so 256 VGPRs)
 //   2088 = 29 * (8 * 9)    (width 4178 = 2 * 2089)
 //    928 = 29 * (4 * 8)    (width 1858 = 2 * 929)
-// For 8352, measured on c3 (row kernel per launch, one MI355X): 36 * (29 * 8) 1.176 ms,
+// For 8352, measured on c3 (row kernel per launch, one MI355X): 36 * (29 * 8) 1.176 ms
+// (1.150 with rows padded by one entry every 8, which takes the radix-8 pivot's writes
+// off a stride of 8 entries: bank-conflict cycles 9.3e7 -> 7.8e7 per launch),
 // 36 * (8 * 29) 1.223, 29 * (32 * 9) 1.306, 29 * (18 * 16) 1.307, 29 * (16 * 18) 1.319,
 // 29 * (16 * 18) at 768 threads 1.313.  The 576-thread shapes are held to 168 VGPRs
 // and lose on LDS bank conflicts (the radix-16 first stage writes at a stride of 16
