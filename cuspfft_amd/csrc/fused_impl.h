@@ -535,6 +535,9 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
 //   33600 = 35 * 960   (7-smooth length of 2 * 16706 - 1)
 //   36864 = 36 * 1024, 35840 = 35 * 1024 (power-of-two convolution lengths)
 //   17496 = 24 * 729, 16464 = 21 * 784, 5184 = 16 * 324 (3 rows), 4704 = 21 * 224 (4 rows)
+//   (Direct shapes for c1's width 2310 = 35 * (6 * 11) = 22 * (7 * 15), 4 rows, with the
+//   radix-11 butterfly, measured no faster than Bluestein at 5184 on c1: row kernel 0.043
+//   and 0.037 ms against 0.040; c1's 1156 short rows are latency-bound.  Not kept.)
 #define SPFFT_FUSED_GROUP_A(X)                                                              \
     X(36, 2, 1, 512, 9, 12, 9) X(35, 5, 1, 512, 8, 12, 10) X(36, 3, 1, 512, 16, 8, 8)     \
     X(35, 5, 1, 512, 16, 8, 8)

@@ -524,7 +524,10 @@ int spfft_next_7_smooth(int n) {
 int spfft_fused_lengths(int* out, int cap) {
     std::vector<int> ms;
     spfft::FusedShape f;
-    for (int i = 0; spfft::fused_shape(i, &f); i++) ms.push_back(f.m1 * f.r[0] * f.r[1] * f.r[2]);
+    for (int i = 0; spfft::fused_shape(i, &f); i++) {
+        const int m = f.m1 * f.r[0] * f.r[1] * f.r[2];
+        if (is_7_smooth(m)) ms.push_back(m);  // Bluestein lengths (direct-only shapes: spfft_direct_lengths)
+    }
     std::sort(ms.begin(), ms.end());
     ms.erase(std::unique(ms.begin(), ms.end()), ms.end());
     for (int i = 0; out && i < (int)ms.size() && i < cap; i++) out[i] = ms[i];

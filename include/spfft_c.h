@@ -277,7 +277,7 @@ int spfft_next_7_smooth(int n);
 /* Writes the factor_3_smooth radix order into out (cap entries); returns the
  * count, or -1 if n is not 3-smooth. */
 int spfft_factor_3_smooth(int n, int* out, int cap);
-/* Bluestein lengths M that have a fused row-kernel shape (SPFFT_ROW_FUSED), in
+/* Bluestein lengths M (7-smooth) that have a fused row-kernel shape (SPFFT_ROW_FUSED), in
  * increasing order: writes up to cap of them to out and returns how many exist.
  * No counterpart in the reference (its row pass has no fused kernel). */
 int spfft_fused_lengths(int* out, int cap);
