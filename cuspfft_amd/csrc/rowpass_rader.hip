@@ -16,7 +16,7 @@
 // inverse), then scatters E_s to natural order in LDS and writes
 //   row u:      X[u][k] = E_0[k] + W_C^k E_1[k]      (k < p),  X[u][p] = E_0[0] - E_1[0]
 //   row R - u:  X[R-u][p - k] = conj(E_0[k] - W_C^k E_1[k]),   X[R-u][0] = conj(X[u][0])
-// For c3's width 16706 = 2 * 8353 the convolutions have length 8352 = 36 * 232
+// For c3's width 16706 = 2 * 8353 the convolutions have length 8352 = 29 * 288
 // (two of them per row) against one of length 34992 for Bluestein.
 //
 // Replaces, for these widths, the Bluestein row pass of the reference
@@ -186,17 +186,20 @@ __global__ __launch_bounds__(NT) void k_rowrader(RowPassArgs a) {
 }  // namespace
 
 // Instantiated shapes X(M1, NT, R1, R2, R3), L = p - 1 = M1 R1 R2 R3:
-//   8352 = 36 * (29 * 8)   (width 16706 = 2 * 8353; 512 threads, so 256 VGPRs)
+//   8352 = 29 * (36 * 8)   (width 16706 = 2 * 8353; 704 threads, rows padded by one
+//                           entry every 8)
 //   2088 = 29 * (8 * 9)    (width 4178 = 2 * 2089)
 //    928 = 29 * (4 * 8)    (width 1858 = 2 * 929)
-// For 8352, measured on c3 (row kernel per launch, one MI355X): 36 * (29 * 8) 1.176 ms
-// (1.150 with rows padded by one entry every 8, which takes the radix-8 pivot's writes
-// off a stride of 8 entries: bank-conflict cycles 9.3e7 -> 7.8e7 per launch),
-// 36 * (8 * 29) 1.223, 29 * (32 * 9) 1.306, 29 * (18 * 16) 1.307, 29 * (16 * 18) 1.319,
-// 29 * (16 * 18) at 768 threads 1.313.  The 576-thread shapes are held to 168 VGPRs
-// and lose on LDS bank conflicts (the radix-16 first stage writes at a stride of 16
-// entries); radix-29 LDS stages at 576 threads, or with M1 = 32 or 48, spill.
-#define SPFFT_RADER_LIST(X) X(36, 512, 29, 8, 1) X(29, 192, 8, 9, 1) X(29, 64, 4, 8, 1)
+// For 8352, measured on c3 (row kernel per launch, one MI355X), with the padding where
+// it applies: 29 * (36 * 8) at 704 threads 1.090 ms, at 768 1.093, at 640 1.106, at
+// 576 1.118; 36 * (29 * 8) at 512 threads 1.150 (1.176 unpadded: the radix-8 pivot
+// writes at a stride of 8 entries; bank-conflict cycles 9.3e7 -> 7.8e7 per launch), and
+// its radix-29 stages run 576 butterflies on 512 threads, a second round for one wave;
+// 36 * (8 * 29) 1.223; 29 * (32 * 9) 1.306; 29 * (18 * 16) 1.307; 29 * (16 * 18) 1.319
+// (its radix-16 first stage writes at a stride of 16 entries: 1.9e8 conflict cycles).
+// Radix-29 LDS stages at 576+ threads (168 VGPRs), radix 12 / 24 stages, or M1 = 32 or
+// 48 spill 50-175 VGPRs.
+#define SPFFT_RADER_LIST(X) X(29, 704, 36, 8, 1) X(29, 192, 8, 9, 1) X(29, 64, 4, 8, 1)
 
 bool rader_shape(int i, RaderShape* out) {
     static const RaderShape kShapes[] = {
