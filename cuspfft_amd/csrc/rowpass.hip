@@ -236,15 +236,18 @@ __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
                 run(0, n_even, acc);
                 if (PAIR) run(n_even, n, acc_o);
             }
-            const float2 hc = a.h[c];
+            // the direct and Rader transforms have no chirp (h = 1): no product
+            const bool chirp = !(a.direct || a.rader);
+            const float2 hc = chirp ? a.h[c] : make_float2(1.f, 0.f);
 #pragma unroll
             for (int t = 0; t < U; t++) {
                 if (PAIR) {
                     const float2 e = acc[t], o = acc_o[t];
                     acc[t] = make_float2(e.x + o.x, e.y + o.y);
-                    acc_o[t] = cmul(make_float2(e.x - o.x, o.y - e.y), hc);
+                    acc_o[t] = make_float2(e.x - o.x, o.y - e.y);
+                    if (chirp) acc_o[t] = cmul(acc_o[t], hc);
                 }
-                acc[t] = cmul(acc[t], hc);
+                if (chirp) acc[t] = cmul(acc[t], hc);
             }
         }
 #pragma unroll
