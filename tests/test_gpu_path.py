@@ -587,7 +587,8 @@ def test_direct_single_nonzero_closed_form(gpu):
 
 # ---- Rader row transform: widths 2p, p an odd prime with a shape for p - 1 ------
 RADER_CASES = [(1858, 50, 2000, 50), (1858, 7, 40, 3), (1858, 1858, 12000, 1858), (4178, 100, 5000, 33),
-               (4178, 4177, 20000, 4177), (16706, 64, 3000, 64), (16706, 5, 300, 2)]
+               (4178, 4177, 20000, 4177), (16706, 64, 3000, 64), (16706, 5, 300, 2), (1858, 40, 0, 40),
+               (1858, 1, 5, 1), (1858, 2, 9, 2)]
 
 
 @pytest.mark.parametrize("cols,rows,nnz,tile", RADER_CASES)
@@ -673,6 +674,45 @@ def test_streaming_entry_points(gpu, oracle_mod):
         assert oracle_mod.rel_l2(res.h_output[:, :V].numpy(), ref) <= REL_L2_PLAIN
     with pytest.raises(RuntimeError):
         sp.sparse_fft_csc_stockham_streaming(m, 0)
+
+
+def test_batch_graph_with_rader_widths(gpu, oracle_mod):
+    """A batch mixing Rader widths (1858, 4178, 16706), a direct width and a Bluestein one,
+    one tile per matrix: eager against the FP64 oracle (sampled rows for the 16706 one), and
+    two graph replays bit-identical to the eager run."""
+    import torch
+    sp = gpu
+    sizes = (1858, 1500, 4178, 1024, 16706)
+    mats = [sp.generate("uniform", n, n * (6 if n < 10000 else 2), seed=90 + i) for i, n in enumerate(sizes)]
+    b = sp.Batch([(m.rows, m.cols) for m in mats], opts=[dict(u_tile=m.rows) for m in mats],
+                 nnz_hint=[m.nnz for m in mats], streams=2)
+    assert [inf["rader"] for inf in b.infos] == [1, 0, 1, 0, 1]
+    assert [inf["direct"] for inf in b.infos] == [0, 0, 0, 1, 0]
+    print("[batch] rader widths", flush=True)
+    d_r, d_c, off = sp.Batch.pack(mats)
+    arena = b.alloc_output()
+    b.execute(d_r, d_c, off, arena)
+    torch.cuda.synchronize()
+    b.check_input()
+    eager = arena.clone()
+    for i, m in enumerate(mats):
+        X = b.output_of(arena, i)[:, : m.cols // 2 + 1].cpu().numpy()
+        if m.rows <= 5000:
+            ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, nthreads=8)
+        else:
+            u = np.array([0, 1, 2, m.rows // 4, m.rows // 2, m.rows // 2 + 1, m.rows - 2, m.rows - 1])
+            ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, u_list=u, nthreads=8)
+            X = X[u]
+        assert oracle_mod.rel_l2(X, ref) <= REL_L2_PLAIN, i
+    g = b.capture(d_r, d_c, off, arena)
+    for rep in range(2):
+        arena.zero_()
+        g.launch()
+        torch.cuda.synchronize()
+        for i, m in enumerate(mats):
+            V = m.cols // 2 + 1
+            np.testing.assert_array_equal(b.output_of(arena, i)[:, :V].cpu().numpy(),
+                                          b.output_of(eager, i)[:, :V].cpu().numpy())
 
 
 def _batch_mats(n, seed):
