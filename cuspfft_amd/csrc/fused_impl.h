@@ -30,6 +30,7 @@
 // short rows in one workgroup (one descriptor for all, explicit bounds).
 #pragma once
 #include <algorithm>
+#include <atomic>
 
 #include "buffer_io.h"
 #include "rowpass_internal.h"
@@ -606,14 +607,17 @@ hipError_t launch_fused_group_h(const RowPassArgs& a, hipStream_t s, bool prepar
         const int nblk = (a.tile_rows + rpw - 1) / rpw;                                             \
         int grid = nblk;                                                                            \
         if (a.fused_cus > 0 && fused_persistent(m1, nb, rpw, r1, r2, r3, a.direct != 0)) {         \
-            static int occ[2] = {0, 0}; /* resident workgroups per CU, per form */                  \
-            if (!occ[a.direct ? 1 : 0]) {                                                           \
+            /* resident workgroups per CU, per form (any host thread may fill it: the same value) */ \
+            static std::atomic<int> occ[2] = {{0}, {0}};                                            \
+            int o = occ[a.direct ? 1 : 0].load(std::memory_order_relaxed);                          \
+            if (!o) {                                                                               \
                 int n = 0;                                                                          \
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, nt, lds) != hipSuccess)     \
                     n = 1;                                                                          \
-                occ[a.direct ? 1 : 0] = std::max(n, 1);                                             \
+                o = std::max(n, 1);                                                                 \
+                occ[a.direct ? 1 : 0].store(o, std::memory_order_relaxed);                          \
             }                                                                                       \
-            grid = std::min(nblk, a.fused_cus * occ[a.direct ? 1 : 0]);                             \
+            grid = std::min(nblk, a.fused_cus * o);                                                 \
         }                                                                                           \
         hipLaunchKernelGGL(k, dim3(grid), dim3(nt), lds, s, a);                                     \
         return hipGetLastError();                                                                   \
