@@ -617,6 +617,28 @@ def test_rader_transform(gpu, oracle_mod, cols, rows, nnz, tile):
         np.testing.assert_array_equal(host[:, : cols // 2 + 1].numpy(), X)
 
 
+def test_rader_row_limit_and_tiles(gpu):
+    """A Rader plan computing only rows u < 37 (and their mirror rows) in tiles of 16 rows
+    matches the full one-tile run bit for bit on those rows (both with the unpaired column
+    pass, whose rows u < 64 each take their own phase lookup)."""
+    import torch
+    sp = gpu
+    rows, cols = 300, 4178
+    r, c = _random_coo(np.random.default_rng(11), rows, cols, 6000)
+    full, X = _run(sp, rows, cols, r, c, u_tile=rows, colpass_pair=sp.PAIR_OFF)
+    plan = sp.Plan(rows, cols, u_tile=16, row_limit=37)
+    assert plan.info["rader"] == 1 and full.info["rader"] == 1
+    out = plan.alloc_output()
+    out.fill_(complex(float("nan"), float("nan")))
+    plan.execute(_dev(r), _dev(c), out)
+    torch.cuda.synchronize()
+    V = cols // 2 + 1
+    got = out[:, :V].cpu().numpy()
+    u = np.arange(37)
+    np.testing.assert_array_equal(got[u], X[u])
+    np.testing.assert_array_equal(got[rows - u[1:]], X[rows - u[1:]])
+
+
 @pytest.mark.parametrize("c0", [0, 1, 2, 3, 8352, 16705])
 def test_rader_single_nonzero_closed_form(gpu, c0):
     """One nonzero per column class: columns 0 and 1 (x_s[0] of the two halves), the
