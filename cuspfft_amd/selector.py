@@ -25,7 +25,7 @@ import os
 
 import numpy as np
 
-from . import TRANSFORM_DIRECT, _check, _vp, describe_plan, load
+from . import TRANSFORM_DIRECT, TRANSFORM_RADER, _check, _vp, describe_plan, load
 
 N_FEATURES = 29
 # the network trained on the autotuner timings of 2000 seeded matrices (train_selector)
@@ -69,15 +69,19 @@ def features(rec: dict, opts: dict, info: dict | None = None) -> np.ndarray:
 
 def trial_opts(t: dict, rec: dict | None = None) -> dict:
     """The options of a timed trial with its Bluestein length pinned to the one it ran;
-    a trial that ran the direct length-cols transform (flagged, or a length below
-    2 cols - 1 in records written before the flag) is planned as that again."""
+    a trial that ran the direct length-cols transform or Rader's convolutions (flagged,
+    or, in records written before the flags, a length of cols or cols/2 - 1) is planned
+    as that again."""
     o = dict(t["opts"])
-    direct = t.get("direct") or (rec is not None and t.get("fft_len") and t["fft_len"] < 2 * rec["cols"] - 1)
-    if direct:
+    n = t.get("fft_len")
+    cols = rec["cols"] if rec is not None else None
+    direct = t.get("direct") or (cols is not None and n == cols and n < 2 * cols - 1)
+    rader = t.get("rader") or (cols is not None and cols % 2 == 0 and n == cols // 2 - 1)
+    if direct or rader:
         o.pop("fft_len", None)
-        o["row_transform"] = TRANSFORM_DIRECT
-    elif t.get("fft_len"):
-        o["fft_len"] = t["fft_len"]
+        o["row_transform"] = TRANSFORM_DIRECT if direct else TRANSFORM_RADER
+    elif n:
+        o["fft_len"] = n
     return o
 
 
@@ -216,7 +220,6 @@ def choose_plan(m, max_m1: int = 2, fp32_only: bool = False) -> dict:
     fp32_only: among the plain FP32 plans only (no BF16x3 precision form)."""
     global _default
     from . import autotune
-    from . import describe_plan
     if describe_plan(m.rows, m.cols)["rader"]:
         # Rader's convolutions (half the Bluestein row-pass work) are not among the
         # plans the network was trained on: the planner's plan

@@ -50,6 +50,26 @@ def test_features_shape_and_determinism():
     np.testing.assert_array_equal(f, np.stack([selector.features(rec, c) for c in cands]))
 
 
+def test_trial_opts_replans_each_transform():
+    """A timed trial is planned again as the transform it ran: Bluestein at its length,
+    the direct transform, or Rader's convolutions (flagged, or recognised by their length
+    in records written before the flags); choose_plan gives Rader widths the planner's plan."""
+    import cuspfft_amd as sp
+    from cuspfft_amd import selector
+    rec = {"cols": 16706}
+    blue = selector.trial_opts({"opts": dict(length_rule=3), "fft_len": 34992}, rec)
+    assert blue["fft_len"] == 34992 and "row_transform" not in blue
+    for t in ({"opts": dict(length_rule=3), "fft_len": 8352, "rader": 1},
+              {"opts": dict(length_rule=3), "fft_len": 8352}):
+        o = selector.trial_opts(t, rec)
+        assert o["row_transform"] == sp.TRANSFORM_RADER and "fft_len" not in o
+        assert sp.describe_plan(16706, 16706, **o)["rader"] == 1
+    d = selector.trial_opts({"opts": dict(length_rule=3), "fft_len": 16384}, {"cols": 16384})
+    assert d["row_transform"] == sp.TRANSFORM_DIRECT
+    m = sp.generate("uniform", 16706, 2000, seed=3)
+    assert sp.describe_plan(m.rows, m.cols, **selector.choose_plan(m))["rader"] == 1
+
+
 def test_training_set_is_seeded():
     from cuspfft_amd import autotune
     a, ta = autotune.training_matrices(6, nnz_cap=20000)
