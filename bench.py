@@ -2,17 +2,22 @@
 
 One step = one full binary-sparse 2-D R2C FFT of the workload matrix, from a
 device-resident int32 COO to the device-resident complex64 half spectrum:
-COO->CSC build + column sparse-DFT pass + Bluestein row pass (all on the GPU).
+COO->CSC build + column sparse-DFT pass + row pass (all on the GPU).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3] [--no-cpu]
 
 Prints ONE JSON line.  value = FFTs/s over the K timed transforms (no per-kernel
-events in them).  The plan is the shipped selector's choice among the plain FP32
-candidates (--plan default: the planner's default plan); both are timed.
-Roofline: algorithmic bytes per FFT B_alg = 8*nnz (COO int32 read) +
+events in them) of the plan under test: by default the planner's plan (for c3 the
+Rader row transform; --plan selector: the shipped selector's choice).  Beside it,
+`forms` times the other forms of the metric on the same input, each with its
+rel-L2 error against the FP64 oracle: the plain FP32 Bluestein plan at the reference's
+length (the reference's algorithm) and the BF16x3 MFMA precision form, and their
+ratios.  Roofline: algorithmic bytes per FFT B_alg = 8*nnz (COO int32 read) +
 8*rows*(cols/2+1) (complex64 output write); achieved = B_alg / (the dominant kernel's
 average launch time, from HIP events on the launch stream over K more transforms
-after the timed ones); peak = 8 TB/s (MI355X HBM3E).
+after the timed ones); peak = 8 TB/s (MI355X HBM3E).  cpu_baseline: scipy.fft.rfft2
+(pocketfft) in float32 on the dense matrix, 1 thread and all host threads (FFTW, the
+reference's CPU path, is not installed).
 """
 from __future__ import annotations
 
@@ -37,11 +42,13 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--config", default="c3")
-    ap.add_argument("--plan", default="selector", choices=("selector", "default"),
-                    help="plan of the timed transform when no plan option is given: the shipped "
-                         "selector's choice among the plain FP32 candidates, or the planner's default")
+    ap.add_argument("--plan", default="default", choices=("selector", "default"),
+                    help="plan of the timed transform when no plan option is given: the planner's "
+                         "default, or the shipped selector's choice among the plain FP32 candidates")
     ap.add_argument("--rule", type=int, default=3)
-    ap.add_argument("--tile", type=int, default=0, help="u-tile rows (0 = tuned default)")
+    ap.add_argument("--tile", type=int, default=0, help="u-tile rows (0 = one tile of every computed row)")
+    ap.add_argument("--t-rows", type=int, default=0,
+                    help="four-step kernels: rows of the tile buffer T per sub-tile (0 = automatic)")
     ap.add_argument("--m1", type=int, default=0)
     ap.add_argument("--fft-len", type=int, default=0, help="explicit Bluestein length M (0: the rule's)")
     ap.add_argument("--precision", type=int, default=0, help="0 plain FP32, 1 BF16x3 MFMA")
@@ -57,13 +64,12 @@ def parse():
     ap.add_argument("--row-kernel", type=int, default=0,
                     help="0 auto (fused row kernel when M has a shape), 1 four-step kernels, 2 fused")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-rows", type=int, default=0, help="rows sampled by the CPU baseline")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--batch", type=int, default=512, help="c4: number of matrices")
     ap.add_argument("--tune-db", default="", help="c4: timing DB (JSON lines) to read or append")
     ap.add_argument("--streams", type=int, default=4, help="c4: worker streams")
-    ap.add_argument("--forms", action="store_true",
-                    help="also time the graph-replayed plain and BF16x3 MFMA forms")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the other forms (plain Bluestein, BF16x3 MFMA) beside the timed plan")
     ap.add_argument("--to-host", type=int, default=0, metavar="TILE",
                     help="also time the output streamed to pinned host memory with u-tiles of TILE rows")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -82,24 +88,6 @@ def copy_calibration(mib: int):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "launches": 3}), flush=True)
-
-
-def time_graph(sp, m, d_row, d_col, steps: int, warmup: int, **plan_kw):
-    """FFTs/s of one plan captured as a hipGraph and replayed `steps` times."""
-    import torch
-    plan = sp.Plan(m.rows, m.cols, u_tile=m.rows, **plan_kw)
-    out = plan.alloc_output()
-    g = sp.capture_plan(plan, d_row, d_col, out)
-    for _ in range(warmup):
-        g.launch()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        g.launch()
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / steps
-    plan.check_input()
-    return 1.0 / dt, dt * 1e3, plan, out
 
 
 def run_batch_bench(args):
@@ -195,22 +183,59 @@ def run_batch_bench(args):
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(m, rule: int, rows_sample: int):
-    """The oracle's FP32 restatement of the reference algorithm on a bounded sample of
-    output rows, one host thread; scaled to FFTs/s."""
-    import numpy as np
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    import oracle
-    oracle.build()
-    rows = np.arange(min(rows_sample, m.rows), dtype=np.int32)
+
+def cpu_baseline(m):
+    """The reference's CPU path is FFTW's single-precision dense R2C 2-D transform with
+    only fftwf_execute timed (src/cpu_reference.cpp:8-64).  FFTW is not installed, so the
+    labelled substitute is scipy.fft.rfft2 (pocketfft) on the dense float32 matrix: the
+    dense scatter is untimed, one transform is timed with 1 worker (the reference's
+    single thread) and one with every host thread."""
+    import numpy as np
+    import scipy
+    import scipy.fft
+    dense = np.zeros((m.rows, m.cols), np.float32)
+    dense[m.row_idx, m.col_idx] = 1.0
+    # every host thread this process may use: the CPU affinity set, capped by
+    # OMP_NUM_THREADS where the machine sets it to this job's share
+    nproc = len(os.sched_getaffinity(0)) or 1
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        nproc = max(1, min(nproc, int(os.environ["OMP_NUM_THREADS"])))
+    out = {}
+    for workers in (1, nproc):
+        t0 = time.perf_counter()
+        X = scipy.fft.rfft2(dense, workers=workers)
+        out[workers] = time.perf_counter() - t0
+        del X
+    del dense
+    kind = "scipy-pocketfft-f32 (FFTW absent)"
+    return {"value": round(1.0 / out[1], 5), "unit": "FFTs/s", "cores": 1, "kind": kind,
+            "sample": f"one full {m.rows}x{m.cols} float32 rfft2, transform only "
+                      f"(scipy {scipy.__version__}, workers=1)",
+            "all_cores": {"value": round(1.0 / out[nproc], 5), "cores": nproc, "workers": nproc},
+            "host": {"threads_used": nproc, "os_cpu_count": os.cpu_count(), "cpu_model": _cpu_model()}}
+
+
+def time_eager(plan, d_row, d_col, out, steps: int, warmup: int) -> float:
+    """Seconds per transform over `steps` eager executions after `warmup` untimed ones."""
+    import torch
+    for _ in range(warmup):
+        plan.execute(d_row, d_col, out)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    oracle.sparse_rfft2_f32(m.rows, m.cols, m.row_idx, m.col_idx, smooth=rule, u_list=rows,
-                            nthreads=1)
-    dt = time.perf_counter() - t0
-    per_fft = dt * m.rows / len(rows)
-    return {"value": 1.0 / per_fft, "unit": "FFTs/s", "cores": 1, "kind": "port",
-            "sample": f"{len(rows)} of {m.rows} output rows in {dt:.1f} s, scaled to one full FFT "
-                      f"(oracle FP32 restatement of the reference algorithm, 1 thread)"}
+    for _ in range(steps):
+        plan.execute(d_row, d_col, out)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
 
 
 def main():
@@ -232,23 +257,28 @@ def main():
     sp.load()
     dev = torch.device("cuda:0")
     m = sp.config_matrix(args.config)
-    tile = args.tile if args.tile > 0 else m.rows
     explicit = (args.tile > 0 or args.fft_len or args.m1 or args.precision or args.rule != 3
                 or args.colpass_rows or args.colpass_table or args.row_kernel or args.colpass_pair
-                or args.row_transform or args.row_persist)
+                or args.row_transform or args.row_persist or args.t_rows)
+    opts = dict(length_rule=args.rule, m1=args.m1, fft_len=args.fft_len, precision=args.precision,
+                row_kernel=args.row_kernel, colpass_rows=args.colpass_rows,
+                colpass_table=args.colpass_table, colpass_pair=args.colpass_pair,
+                row_transform=args.row_transform, row_persist=args.row_persist, t_rows=args.t_rows)
     if explicit or args.plan == "default":
-        plan_source = "options" if explicit else "default"
-        opts = dict(length_rule=args.rule, m1=args.m1, fft_len=args.fft_len, precision=args.precision,
-                    row_kernel=args.row_kernel, colpass_rows=args.colpass_rows,
-                    colpass_table=args.colpass_table, colpass_pair=args.colpass_pair,
-                    row_transform=args.row_transform, row_persist=args.row_persist)
+        plan_source = "options" if explicit else "planner"
     else:
         # the shipped network's choice among the autotuner's plain FP32 candidates: the
         # same transform, the plan predicted fastest
         from cuspfft_amd import selector
-        plan_source = "selector"
         opts = selector.choose_plan(m, fp32_only=True)
-    plan = sp.Plan(m.rows, m.cols, u_tile=tile, **opts)
+        plan_source = "selector"
+    plan = sp.Plan(m.rows, m.cols, u_tile=args.tile, **opts)
+    info = plan.info
+    transform = "direct" if info["direct"] else "rader" if info["rader"] else "bluestein"
+    if plan_source == "planner":
+        plan_source = f"planner ({transform}; selector not used)"
+    elif plan_source == "selector" and info["rader"]:
+        plan_source = "planner (rader; selector bypassed: its network has no Rader plans)"
     d_row = torch.as_tensor(m.row_idx).to(dev)
     d_col = torch.as_tensor(m.col_idx).to(dev)
     out = plan.alloc_output()
@@ -265,6 +295,8 @@ def main():
         plan.execute(d_row, d_col, out)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    print(f"[bench] {args.config} {transform} M={info['fft_len']}: {(t1 - t0) * 1e3 / args.steps:.4f} ms",
+          file=sys.stderr, flush=True)
     # then K more with HIP events around every kernel on the launch stream: the
     # per-kernel times and the dominant kernel's launch time for the roofline
     plan.profile(True)
@@ -274,28 +306,16 @@ def main():
     plan.profile(False)
     plan.check_input()
     if plan.info["row_kernel"] == sp.ROW_FUSED:
-        # one kernel (k_rowfused) does fft1 + conv + ifft1; its time is reported under conv
+        # one kernel (k_rowfused / k_rowrader) does fft1 + conv + ifft1: reported as row_fused
         prof = {("row_fused" if k == "conv" else k): v for k, v in prof.items()
+                if k not in ("fft1", "ifft1_finalize")}
+    elif 0 < info["t_rows"] < info["u_tile"]:
+        # sub-tiles of T: the three four-step kernels are timed together
+        prof = {("row_four_step" if k == "conv" else k): v for k, v in prof.items()
                 if k not in ("fft1", "ifft1_finalize")}
 
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     value = 1000.0 / ms_per_step
-    default_plan = None
-    if plan_source == "selector":
-        # the planner's default plan on the same input, for comparison
-        dplan = sp.Plan(m.rows, m.cols, u_tile=tile)
-        dout = dplan.alloc_output()
-        for _ in range(args.warmup):
-            dplan.execute(d_row, d_col, dout)
-        torch.cuda.synchronize()
-        td = time.perf_counter()
-        for _ in range(args.steps):
-            dplan.execute(d_row, d_col, dout)
-        torch.cuda.synchronize()
-        dms = (time.perf_counter() - td) * 1000.0 / args.steps
-        default_plan = {"ms_per_step": round(dms, 4), "fft_len": dplan.info["fft_len"], "m1": dplan.info["m1"],
-                        "conv_radix": dplan.info["conv_radix"]}
-        del dplan, dout
     V = m.cols // 2 + 1
     b_alg = 8.0 * m.nnz + 8.0 * m.rows * V
     dom = max(prof, key=lambda k: prof[k][0])
@@ -303,22 +323,26 @@ def main():
     achieved = b_alg * args.steps / (dom_ms * 1e-3) / 1e9
     step_gbs = b_alg / (ms_per_step * 1e-3) / 1e9
 
-    # HBM bytes per launch of the dominant kernel, from the rocprofv3 FETCH_SIZE /
-    # WRITE_SIZE runs recorded in profiles/traffic.json for this workload and row-pass shape
-    traffic = None
+    # HBM bytes per launch of the dominant kernel: not measured in this run -- the
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE record of the same kernel and plan shape on this
+    # workload, kept in profiles/traffic.json (null when there is none)
+    traffic, traffic_source = None, None
     try:
-        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")) as f:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             recs = json.load(f).get(args.config, {}).get(dom) or []
-        info0 = plan.info
         for rec in recs if isinstance(recs, list) else [recs]:
-            if (rec.get("fft_len") == info0["fft_len"] and rec.get("m1") == info0["m1"]
-                    and rec.get("row_kernel") == info0["row_kernel"] and args.tile <= 0):
+            if (rec.get("fft_len") == info["fft_len"] and rec.get("m1") == info["m1"]
+                    and rec.get("row_kernel") == info["row_kernel"]
+                    and rec.get("precision", 0) == info["precision"] and args.tile <= 0):
                 traffic = rec["bytes"]
+                traffic_source = (f"rocprofv3 FETCH_SIZE + WRITE_SIZE of {rec['kernel']} on this "
+                                  f"workload and plan shape, from profiles/traffic.json ({rec['source']}); "
+                                  "not measured in this run")
                 break
     except (OSError, ValueError):
         traffic = None
 
-    rel = None
+    rel, rows, ref = None, None, None
     if not args.no_check:
         import oracle
         oracle.build()
@@ -329,22 +353,43 @@ def main():
         got = out[torch.as_tensor(rows).to(dev), :V].cpu().numpy()
         rel = oracle.rel_l2(got, ref)
 
+    def plan_desc(i):
+        return {"fft_len": i["fft_len"], "m1": i["m1"], "m2": i["m2"],
+                "row_kernel": {1: "four-step", 2: "fused"}[i["row_kernel"]],
+                "transform": "direct" if i["direct"] else "rader" if i["rader"] else "bluestein",
+                "precision": "bf16x3-mfma" if i["precision"] else "fp32"}
+
     forms = None
-    if args.forms:
-        forms = {}
-        for name, kw in (("plain_fp32_graph", dict(length_rule=args.rule)),
-                         ("bf16x3_mfma_graph", dict(length_rule=args.rule, precision=1))):
-            fps, ms, gplan, gout = time_graph(sp, m, d_row, d_col, args.steps, args.warmup, **kw)
+    if not args.no_forms:
+        # the forms of the metric on the same input, timed like `value` (eager, K steps
+        # after W warm-up ones): the timed plan; the plain FP32 Bluestein plan at the
+        # reference's length (the reference's algorithm, hand-written Stockham stages);
+        # the BF16x3 MFMA precision form (its planner's plan)
+        forms = {"timed_plan": dict(plan_desc(info), ms=round(ms_per_step, 4), fft_per_s=round(value, 3),
+                                    rel_l2_vs_fp64=rel)}
+        for name, kw in (("plain_fp32_bluestein", dict(length_rule=args.rule, row_transform=1)),
+                         ("bf16x3_mfma", dict(length_rule=args.rule, precision=1))):
+            try:
+                fplan = sp.Plan(m.rows, m.cols, **kw)
+            except sp.SpfftError as e:
+                forms[name] = {"error": str(e)}
+                continue
+            fout = fplan.alloc_output()
+            dt = time_eager(fplan, d_row, d_col, fout, args.steps, args.warmup)
+            fplan.check_input()
             err = None
-            if not args.no_check:
-                g = gout[torch.as_tensor(rows).to(dev), :V].cpu().numpy()
-                err = oracle.rel_l2(g, ref)
-            forms[name] = {"fft_per_s": round(fps, 3), "ms": round(ms, 4),
-                           "fft_len": gplan.info["fft_len"], "m1": gplan.info["m1"],
-                           "rel_l2_vs_fp64": err}
-            del gplan, gout
-        forms["mfma_over_plain"] = round(forms["bf16x3_mfma_graph"]["fft_per_s"]
-                                         / forms["plain_fp32_graph"]["fft_per_s"], 3)
+            if ref is not None:
+                err = oracle.rel_l2(fout[torch.as_tensor(rows).to(dev), :V].cpu().numpy(), ref)
+            forms[name] = dict(plan_desc(fplan.info), ms=round(dt * 1e3, 4), fft_per_s=round(1.0 / dt, 3),
+                               rel_l2_vs_fp64=err)
+            print(f"[bench] form {name}: {dt * 1e3:.4f} ms rel_l2={err}", file=sys.stderr, flush=True)
+            del fplan, fout
+        plain = forms["timed_plan"] if not info["precision"] else forms["plain_fp32_bluestein"]
+        if "fft_per_s" in forms.get("bf16x3_mfma", {}):
+            mf = forms["bf16x3_mfma"]["fft_per_s"]
+            forms["mfma_over_plain"] = round(mf / plain["fft_per_s"], 3)
+            if "fft_per_s" in forms["plain_fp32_bluestein"]:
+                forms["mfma_over_plain_bluestein"] = round(mf / forms["plain_fp32_bluestein"]["fft_per_s"], 3)
 
     streamed = None
     if args.to_host:
@@ -360,7 +405,7 @@ def main():
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / args.steps
         err = None
-        if not args.no_check:
+        if ref is not None:
             err = oracle.rel_l2(host[torch.as_tensor(rows), :V].numpy(), ref)
         streamed = {"fft_per_s": round(1.0 / dt, 3), "ms": round(dt * 1e3, 4), "u_tile": args.to_host,
                     "d2h_gb_per_s": round(m.rows * V * 8 / dt / 1e9, 2),
@@ -370,12 +415,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        # about 10 s of single-thread work on the host
-        n = args.cpu_rows if args.cpu_rows > 0 else {"c1": 2310, "c2": 4000, "c3": 3000,
-                                                     "c5": 800, "p14": 3000}.get(args.config, 2000)
-        cpu = cpu_baseline(m, args.rule, n)
+        cpu = cpu_baseline(m)
 
-    info = plan.info
     line = {
         "metric": "2-D sparse FFTs/s + %HBM roofline, MFMA-split vs plain FP32 at rel-L2\u22641e-5",
         "value": round(value, 3),
@@ -387,19 +428,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "bf16x3" if info["precision"] else "fp32",
         "data": "synthetic (seeded generator, inputs resident in HBM)",
         "config": {
             "workload": f"{args.config}: {sp.CONFIG_TEXT[args.config]}",
             "rows": m.rows, "cols": m.cols, "nnz": m.nnz,
-            "form": "bf16x3-mfma" if args.precision == 1 else "plain-fp32",
+            "form": "bf16x3-mfma" if info["precision"] else "plain-fp32",
             "length_rule": args.rule, "fft_len": info["fft_len"], "m1": info["m1"],
-            "m2": info["m2"], "u_tile": info["u_tile"], "conv_rows": info["conv_rows"],
+            "m2": info["m2"], "u_tile": info["u_tile"], "t_rows": info["t_rows"],
+            "conv_rows": info["conv_rows"],
             "conv_radix": info["conv_radix"], "colpass_rows": info["colpass_rows"],
             "colpass_full_table": info["colpass_full_table"],
             "colpass_paired": info["colpass_paired"],
             "row_kernel": {1: "four-step", 2: "fused"}[info["row_kernel"]],
-            "row_transform": "direct" if info["direct"] else "rader" if info.get("rader") else "bluestein",
+            "row_transform": transform,
             "plan": plan_source,
         },
         "rel_l2_vs_fp64": rel,
@@ -408,9 +450,9 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+            "traffic_source": traffic_source, "b_alg": int(b_alg),
             "avg_launch_ms": round(dom_ms / max(dom_n, 1), 4),
         },
-        "default_plan": default_plan,
         "forms": forms,
         "streamed_to_host": streamed,
         "cpu_baseline": cpu,

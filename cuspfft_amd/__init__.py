@@ -41,16 +41,18 @@ class SpfftError(RuntimeError):
 
 
 class Opts(ctypes.Structure):
-    _fields_ = [("length_rule", ctypes.c_int), ("u_tile", ctypes.c_int),
+    _fields_ = [("struct_size", ctypes.c_int), ("length_rule", ctypes.c_int), ("u_tile", ctypes.c_int),
                 ("precision", ctypes.c_int), ("fft_len", ctypes.c_int), ("m1", ctypes.c_int),
                 ("conv_rows", ctypes.c_int), ("colpass_rows", ctypes.c_int),
                 ("colpass_table", ctypes.c_int), ("row_limit", ctypes.c_int),
                 ("row_kernel", ctypes.c_int), ("colpass_pair", ctypes.c_int),
-                ("row_transform", ctypes.c_int), ("row_persist", ctypes.c_int)]
+                ("row_transform", ctypes.c_int), ("row_persist", ctypes.c_int),
+                ("t_rows", ctypes.c_int)]
 
 
 class PlanInfo(ctypes.Structure):
-    _fields_ = [("rows", ctypes.c_int), ("cols", ctypes.c_int), ("out_cols", ctypes.c_int),
+    _fields_ = [("struct_size", ctypes.c_int),
+                ("rows", ctypes.c_int), ("cols", ctypes.c_int), ("out_cols", ctypes.c_int),
                 ("pitch", ctypes.c_int), ("fft_len", ctypes.c_int), ("m1", ctypes.c_int),
                 ("m2", ctypes.c_int), ("packed", ctypes.c_int), ("u_tile", ctypes.c_int),
                 ("conv_rows", ctypes.c_int), ("conv_fast", ctypes.c_int),
@@ -58,13 +60,23 @@ class PlanInfo(ctypes.Structure):
                 ("colpass_rows", ctypes.c_int), ("colpass_full_table", ctypes.c_int),
                 ("precision", ctypes.c_int), ("workspace_bytes", ctypes.c_size_t),
                 ("row_kernel", ctypes.c_int), ("colpass_paired", ctypes.c_int),
-                ("direct", ctypes.c_int), ("rader", ctypes.c_int)]
+                ("direct", ctypes.c_int), ("rader", ctypes.c_int), ("row_transform", ctypes.c_int),
+                ("t_rows", ctypes.c_int)]
+
+    def __init__(self):
+        super().__init__()
+        self.struct_size = ctypes.sizeof(PlanInfo)
 
 
 class Result(ctypes.Structure):
-    _fields_ = [("d_output", ctypes.c_void_p), ("output_cols", ctypes.c_int),
-                ("ms", ctypes.c_float), ("ms_e2e", ctypes.c_float),
-                ("mem_bytes", ctypes.c_size_t)]
+    _fields_ = [("struct_size", ctypes.c_int), ("d_output", ctypes.c_void_p),
+                ("output_cols", ctypes.c_int), ("ms", ctypes.c_float), ("ms_e2e", ctypes.c_float),
+                ("mem_bytes", ctypes.c_size_t), ("fft_len", ctypes.c_int),
+                ("row_transform", ctypes.c_int)]
+
+    def __init__(self):
+        super().__init__()
+        self.struct_size = ctypes.sizeof(Result)
 
 
 class CscView(ctypes.Structure):
@@ -109,6 +121,7 @@ def load():
     L = ctypes.CDLL(LIB_PATH)
     sig = {
         "spfft_opts_default": (None, [ctypes.POINTER(Opts)]),
+        "spfft_reference_opts": (None, [_i, _i, ctypes.POINTER(Opts)]),
         "spfft_last_error": (ctypes.c_char_p, []),
         "spfft_version": (ctypes.c_char_p, []),
         "spfft_plan_create": (_i, [_i, _i, ctypes.POINTER(Opts), ctypes.POINTER(_vp)]),
@@ -287,9 +300,9 @@ def _stream_handle(stream=None):
     return ctypes.c_void_p(s.cuda_stream)
 
 
-def _opts(length_rule=SMOOTH_3, u_tile=128, fft_len=0, m1=0, conv_rows=0, precision=PREC_FP32,
+def _opts(length_rule=SMOOTH_3, u_tile=0, fft_len=0, m1=0, conv_rows=0, precision=PREC_FP32,
           colpass_rows=0, colpass_table=0, row_limit=0, row_kernel=ROW_AUTO, colpass_pair=PAIR_AUTO,
-          row_transform=TRANSFORM_AUTO, row_persist=0):
+          row_transform=TRANSFORM_AUTO, row_persist=0, t_rows=0):
     o = Opts()
     load().spfft_opts_default(ctypes.byref(o))
     o.length_rule, o.u_tile, o.precision = length_rule, u_tile, precision
@@ -299,11 +312,12 @@ def _opts(length_rule=SMOOTH_3, u_tile=128, fft_len=0, m1=0, conv_rows=0, precis
     o.colpass_pair = colpass_pair
     o.row_transform = row_transform
     o.row_persist = row_persist
+    o.t_rows = t_rows
     return o
 
 
 def _info_dict(inf: PlanInfo) -> dict:
-    d = {k: getattr(inf, k) for k, _ in PlanInfo._fields_}
+    d = {k: getattr(inf, k) for k, _ in PlanInfo._fields_ if k != "struct_size"}
     d["conv_radix"] = list(inf.conv_radix)[: inf.conv_stages]
     return d
 
@@ -320,14 +334,15 @@ class Plan:
     """A planned transform for one (rows, cols) shape.  Mirrors the set-up half
     of the reference drivers (src/sparse_fft_bluestein.cu:2256-2320)."""
 
-    def __init__(self, rows: int, cols: int, length_rule: int = SMOOTH_3, u_tile: int = 128,
+    def __init__(self, rows: int, cols: int, length_rule: int = SMOOTH_3, u_tile: int = 0,
                  fft_len: int = 0, m1: int = 0, conv_rows: int = 0, precision: int = PREC_FP32,
                  colpass_rows: int = 0, colpass_table: int = 0, row_limit: int = 0,
                  row_kernel: int = ROW_AUTO, colpass_pair: int = PAIR_AUTO,
-                 row_transform: int = TRANSFORM_AUTO, row_persist: int = 0):
+                 row_transform: int = TRANSFORM_AUTO, row_persist: int = 0, t_rows: int = 0):
         L = load()
         o = _opts(length_rule, u_tile, fft_len, m1, conv_rows, precision, colpass_rows,
-                  colpass_table, row_limit, row_kernel, colpass_pair, row_transform, row_persist)
+                  colpass_table, row_limit, row_kernel, colpass_pair, row_transform, row_persist,
+                  t_rows)
         self._h = _vp()
         _check(L.spfft_plan_create(int(rows), int(cols), ctypes.byref(o), ctypes.byref(self._h)))
         inf = PlanInfo()
@@ -493,6 +508,8 @@ class Batch:
         self.offsets = np.zeros(n, np.int64)
         self.pitches = np.zeros(n, np.int32)
         infos = (PlanInfo * n)()
+        for k in range(n):
+            infos[k].struct_size = ctypes.sizeof(PlanInfo)
         _check(L.spfft_batch_layout(self._h, ctypes.byref(total), self.offsets.ctypes.data,
                                     self.pitches.ctypes.data, ctypes.cast(infos, _vp)))
         self.total = int(total.value)
@@ -581,12 +598,18 @@ class SparseFFTResult:
     h_output: object = None
 
 
+def reference_opts(length_rule: int, u_tile: int) -> dict:
+    """Plan options of the reference-shaped entry points: the reference's algorithm,
+    the Bluestein row pass at the rule's smooth length, u_tile rows per tile."""
+    return dict(length_rule=length_rule, u_tile=u_tile, row_transform=TRANSFORM_BLUESTEIN)
+
+
 def _run_reference_shaped(coo: COOMatrix, u_tile: int, length_rule: int) -> SparseFFTResult:
     import torch
     if u_tile <= 0:
         raise RuntimeError("CSC tile size must be positive")
     free0, _ = torch.cuda.mem_get_info()
-    plan = Plan(coo.rows, coo.cols, length_rule=length_rule, u_tile=u_tile)
+    plan = Plan(coo.rows, coo.cols, **reference_opts(length_rule, u_tile))
     d_row = torch.as_tensor(np.ascontiguousarray(coo.row_idx, np.int32)).cuda()
     d_col = torch.as_tensor(np.ascontiguousarray(coo.col_idx, np.int32)).cuda()
     out = plan.alloc_output()
@@ -622,7 +645,7 @@ def sparse_fft_csc_bluestein_mixed_radix_graph(coo: COOMatrix, u_tile: int = 128
     if u_tile <= 0:
         raise RuntimeError("CSC tile size must be positive")
     free0, _ = torch.cuda.mem_get_info()
-    plan = Plan(coo.rows, coo.cols, length_rule=SMOOTH_3, u_tile=u_tile)
+    plan = Plan(coo.rows, coo.cols, **reference_opts(SMOOTH_3, u_tile))
     d_row = torch.as_tensor(np.ascontiguousarray(coo.row_idx, np.int32)).cuda()
     d_col = torch.as_tensor(np.ascontiguousarray(coo.col_idx, np.int32)).cuda()
     out = plan.alloc_output()
@@ -650,8 +673,7 @@ def sparse_fft_csc_stockham_streaming(coo: COOMatrix, u_tile: int = 128,
     if u_tile <= 0:
         raise RuntimeError("CSC tile size must be positive")
     free0, _ = torch.cuda.mem_get_info()
-    plan = Plan(coo.rows, coo.cols, length_rule=SMOOTH_7 if fft_backend == 3 else SMOOTH_3,
-                u_tile=u_tile)
+    plan = Plan(coo.rows, coo.cols, **reference_opts(SMOOTH_7 if fft_backend == 3 else SMOOTH_3, u_tile))
     d_row = torch.as_tensor(np.ascontiguousarray(coo.row_idx, np.int32)).cuda()
     d_col = torch.as_tensor(np.ascontiguousarray(coo.col_idx, np.int32)).cuda()
     host = plan.alloc_host_output()

@@ -12,11 +12,17 @@
 //     --csc-tile N        output rows per tile (default 1024, as the reference)
 //     --repeat N          1 warm-up + N timed runs; median, min, max
 //     --sparse-only       accepted for compatibility (no dense baseline is built)
+//     --csc-auto          this build's native plan (direct or Rader row transform
+//                         where the width has one, else Bluestein)
 //     --cpu-reference     FFTW3 CPU reference; reported as unavailable when this
 //                         build has no FFTW (the reference compiles it out then)
+//     --check-ref FILE    after each variant, the reference's check line (max_abs,
+//                         max_rel, rms_abs) against a reference half spectrum
+//                         ("SPFFTREF", int32 rows, int32 cols, complex128 values)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -50,41 +56,124 @@ COOMatrix load(const std::string& spec) {
     return c;
 }
 
+// Timing contract of the reference driver (src/main.cu:45-70): with n > 1 one
+// untimed warm-up call comes first, then n timed calls; the reported time is the
+// upper median sorted[n / 2] with the extremes beside it.  Every output but the
+// last is released as soon as it is replaced.
+struct Timing {
+    float median = 0.f, lo = 0.f, hi = 0.f;
+};
+
 template <typename Fn>
-SparseFFTResult run_repeated(int n_iters, Fn&& fn, float* mn, float* mx) {
-    SparseFFTResult result{};
-    std::vector<float> times;
-    if (n_iters > 1) {
-        result = fn();
-        if (result.d_output) (void)hipFree(result.d_output);
-        result.d_output = nullptr;
+SparseFFTResult timed_calls(int n, Fn&& call, Timing* t) {
+    auto release = [](SparseFFTResult& r) {
+        if (r.d_output) (void)hipFree(r.d_output);
+        r.d_output = nullptr;
+    };
+    SparseFFTResult last{};
+    if (n > 1) {
+        last = call();  // warm-up
+        release(last);
     }
-    for (int i = 0; i < n_iters; i++) {
-        if (i > 0 && result.d_output) (void)hipFree(result.d_output);
-        result = fn();
-        times.push_back(result.ms);
+    std::vector<float> ms;
+    ms.reserve(n);
+    while ((int)ms.size() < n) {
+        release(last);
+        last = call();
+        ms.push_back(last.ms);
     }
-    if (n_iters > 1) {
-        std::sort(times.begin(), times.end());
-        result.ms = times[n_iters / 2];
-        *mn = times.front();
-        *mx = times.back();
-    }
-    return result;
+    std::sort(ms.begin(), ms.end());
+    t->median = ms[ms.size() / 2];
+    t->lo = ms.front();
+    t->hi = ms.back();
+    return last;
 }
 
-void print_bench(const char* label, float ms, size_t mem, int n, float mn, float mx) {
-    if (n > 1)
-        printf("%-26s  %10.3f  %12.2f  [min %.3f, max %.3f]  n=%d\n", label, ms, mem / 1e6, mn, mx, n);
-    else
-        printf("%-26s  %10.3f  %12.2f\n", label, ms, mem / 1e6);
+// One table row: label, time (ms), device memory (MB); with n > 1 also the range.
+void print_row(const char* label, const Timing& t, size_t mem, int n) {
+    printf("%-26s  %10.3f  %12.2f", label, t.median, mem / 1e6);
+    if (n > 1) printf("  [min %.3f, max %.3f]  n=%d", t.lo, t.hi, n);
+    printf("\n");
+}
+
+// A reference half spectrum written by the test tooling: the 8 bytes "SPFFTREF",
+// int32 rows, int32 cols, then rows x (cols/2 + 1) complex128 values, row-major.
+struct RefSpectrum {
+    int rows = 0, cols = 0;
+    std::vector<double> re_im;  // interleaved
+};
+
+RefSpectrum load_reference(const std::string& path) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) throw std::runtime_error("cannot open reference spectrum " + path);
+    char magic[8];
+    int dims[2];
+    RefSpectrum r;
+    const bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "SPFFTREF", 8) == 0 &&
+                    std::fread(dims, sizeof(int), 2, f) == 2 && dims[0] > 0 && dims[1] > 0;
+    if (ok) {
+        r.rows = dims[0];
+        r.cols = dims[1];
+        r.re_im.resize((size_t)r.rows * (r.cols / 2 + 1) * 2);
+    }
+    const bool full = ok && std::fread(r.re_im.data(), sizeof(double), r.re_im.size(), f) == r.re_im.size();
+    std::fclose(f);
+    if (!full) throw std::runtime_error("malformed reference spectrum " + path);
+    return r;
+}
+
+// The reference driver's error triple (src/main.cu:168-215): over every entry of the
+// half spectrum, max |x - ref|, max |x - ref| / max(1, |ref|), and the root mean square
+// of |x - ref|; printed in the same line format.
+void print_check(const char* label, const RefSpectrum& ref, const SparseFFTResult& r) {
+    const int V = ref.cols / 2 + 1;
+    std::vector<hipFloatComplex> got((size_t)ref.rows * V);
+    if (r.d_output) {
+        if (hipMemcpy2D(got.data(), (size_t)V * sizeof(hipFloatComplex), r.d_output,
+                        (size_t)r.output_cols * sizeof(hipFloatComplex), (size_t)V * sizeof(hipFloatComplex),
+                        ref.rows, hipMemcpyDeviceToHost) != hipSuccess)
+            throw std::runtime_error("hipMemcpy2D of the output failed");
+    } else {
+        for (int u = 0; u < ref.rows; u++)
+            std::copy_n(r.h_output.data() + (size_t)u * r.output_cols, V, got.data() + (size_t)u * V);
+    }
+    double worst_abs = 0.0, worst_rel = 0.0, sq = 0.0;
+    for (size_t i = 0; i < got.size(); i++) {
+        const double xr = ref.re_im[2 * i], xi = ref.re_im[2 * i + 1];
+        const double e = std::hypot((double)got[i].x - xr, (double)got[i].y - xi);
+        worst_abs = std::max(worst_abs, e);
+        worst_rel = std::max(worst_rel, e / std::max(1.0, std::hypot(xr, xi)));
+        sq += e * e;
+    }
+    printf("  check %-19s  max_abs %.3e  max_rel %.3e  rms_abs %.3e\n", label, worst_abs, worst_rel,
+           std::sqrt(sq / (double)got.size()));
+}
+
+// The native plan of this build (row_transform AUTO: the direct or Rader transform
+// where the width has one, else Bluestein), through the C ABI.
+SparseFFTResult run_auto(const COOMatrix& coo, int tile) {
+    spfft_opts o;
+    spfft_opts_default(&o);
+    o.u_tile = tile;
+    spfft_result r;
+    r.struct_size = sizeof(r);
+    if (spfft_run_host_coo(coo.rows, coo.cols, coo.nnz, coo.row_idx.data(), coo.col_idx.data(), &o, &r) !=
+        SPFFT_OK)
+        throw std::runtime_error(spfft_last_error());
+    SparseFFTResult out{};
+    out.d_output = static_cast<hipFloatComplex*>(r.d_output);
+    out.output_cols = r.output_cols;
+    out.ms = r.ms;
+    out.mem_bytes = r.mem_bytes;
+    return out;
 }
 
 void usage(const char* p) {
     fprintf(stderr,
             "Usage: %s <matrix.mtx | gen:c1|c2|c3|c5|p14> [--csc-mixed-radix] [--csc-mixed-radix-graph]\n"
             "          [--csc-cufft-smooth] [--csc-stockham-stream] [--csc-stockham-cufft-stream]\n"
-            "          [--csc-tile N] [--repeat N] [--sparse-only] [--cpu-reference]\n",
+            "          [--csc-auto] [--csc-tile N] [--repeat N] [--sparse-only] [--cpu-reference]\n"
+            "          [--check-ref FILE]\n",
             p);
 }
 
@@ -94,8 +183,9 @@ int main(int argc, char** argv) {
     if (argc < 2) { usage(argv[0]); return 1; }
     if (!std::strcmp(argv[1], "--help") || !std::strcmp(argv[1], "-h")) { usage(argv[0]); return 0; }
     bool mixed = false, mixed_graph = false, smooth = false, cpu_ref = false;
-    bool stream3 = false, stream7 = false;
+    bool stream3 = false, stream7 = false, native = false;
     int tile = 1024, repeat = 1;
+    std::string ref_path;
     try {
         for (int i = 2; i < argc; i++) {
             const std::string f(argv[i]);
@@ -105,7 +195,12 @@ int main(int argc, char** argv) {
             else if (f == "--csc-cufft-smooth") smooth = true;
             else if (f == "--csc-stockham-stream") stream3 = true;
             else if (f == "--csc-stockham-cufft-stream") stream7 = true;
+            else if (f == "--csc-auto") native = true;
             else if (f == "--cpu-reference") cpu_ref = true;
+            else if (f == "--check-ref") {
+                if (++i >= argc) throw std::runtime_error("--check-ref requires a file name");
+                ref_path = argv[i];
+            }
             else if (f == "--csc-tile") {
                 if (++i >= argc) throw std::runtime_error("--csc-tile requires an integer value");
                 tile = std::stoi(argv[i]);
@@ -127,29 +222,40 @@ int main(int argc, char** argv) {
         if (cpu_ref)
             printf("%-26s  %10s  %12s  [%s]\n", "CPU reference (FFTW3)", "FAIL", "FAIL",
                    "FFTW3 not available in this build");
+        RefSpectrum ref;
+        if (!ref_path.empty()) {
+            ref = load_reference(ref_path);
+            if (ref.rows != coo.rows || ref.cols != coo.cols)
+                throw std::runtime_error("reference spectrum shape does not match the matrix");
+        }
         struct V { bool on; const char* label; int kind; };
         const V vs[] = {{mixed, "Sparse CSC (mixed-radix)", 0},
                         {mixed_graph, "Sparse CSC (mr-graph)", 1},
                         {smooth, "Sparse CSC (cufft smooth)", 2},
                         {stream3, "Sparse CSC (Stockham stream)", 3},
-                        {stream7, "Sparse CSC (cufft stream)", 4}};
+                        {stream7, "Sparse CSC (cufft stream)", 4},
+                        {native, "Sparse CSC (auto)", 5}};
         for (const V& v : vs) {
             if (!v.on) continue;
             try {
-                float mn = 0, mx = 0;
-                SparseFFTResult r = run_repeated(
+                Timing t;
+                SparseFFTResult r = timed_calls(
                     repeat,
                     [&] {
-                        if (v.kind == 4) return sparse_fft_csc_bluestein_cufft_streaming(coo, tile);
-                        if (v.kind == 3) return sparse_fft_csc_stockham_streaming(coo, tile, 0);
-                        if (v.kind == 2) return sparse_fft_csc_bluestein_cufft_smooth(coo, tile);
-                        if (v.kind == 1) return sparse_fft_csc_bluestein_mixed_radix_graph(coo, tile);
-                        return sparse_fft_csc_bluestein_mixed_radix(coo, tile);
+                        switch (v.kind) {
+                            case 1: return sparse_fft_csc_bluestein_mixed_radix_graph(coo, tile);
+                            case 2: return sparse_fft_csc_bluestein_cufft_smooth(coo, tile);
+                            case 3: return sparse_fft_csc_stockham_streaming(coo, tile, 0);
+                            case 4: return sparse_fft_csc_bluestein_cufft_streaming(coo, tile);
+                            case 5: return run_auto(coo, tile);
+                            default: return sparse_fft_csc_bluestein_mixed_radix(coo, tile);
+                        }
                     },
-                    &mn, &mx);
-                print_bench(v.label, r.ms, r.mem_bytes, repeat, mn, mx);
-                printf("  rate %-20s  %.2f FFTs/s (tile loop%s)\n", v.label, 1000.0 / r.ms,
-                       v.kind >= 3 ? " with device-to-host copies" : "");
+                    &t);
+                print_row(v.label, t, r.mem_bytes, repeat);
+                printf("  rate %-20s  %.2f FFTs/s (tile loop%s)\n", v.label, 1000.0 / t.median,
+                       v.kind == 3 || v.kind == 4 ? " with device-to-host copies" : "");
+                if (!ref_path.empty()) print_check(v.label, ref, r);
                 (void)hipFree(r.d_output);
             } catch (const std::exception& e) {
                 printf("%-26s  %10s  %12s  [%s]\n", v.label, "OOM", "OOM", e.what());

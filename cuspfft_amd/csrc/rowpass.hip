@@ -423,12 +423,26 @@ hipError_t launch_rowpass_tile(const RowPassArgs& a, bool packed, hipStream_t s,
         if (ev && (e = hipEventRecord(ev[4], s)) != hipSuccess) return e;
         return hipSuccess;
     }
-    const dim3 g1((a.M2 + 63) / 64, (a.tile_rows + 3) / 4);
-    if ((e = launch_fft1(a, g1, s, false)) != hipSuccess) return e;
-    if (ev && (e = hipEventRecord(ev[2], s)) != hipSuccess) return e;
-    if ((e = launch_conv(a, s)) != hipSuccess) return e;
-    if (ev && (e = hipEventRecord(ev[3], s)) != hipSuccess) return e;
-    if ((e = launch_fft1(a, g1, s, true)) != hipSuccess) return e;
+    // the four-step kernels, sub_rows rows of the tile at a time: T (sub_rows x M) is
+    // written by k_fft1, rewritten by k_conv and read by k_ifft1 while it still sits in
+    // the last-level cache.  With several sub-tiles the events time the three kernels
+    // together (reported as the convolution stage).
+    const int step = a.sub_rows > 0 ? a.sub_rows : a.tile_rows;
+    const bool split = step < a.tile_rows;
+    if (split && ev && (e = hipEventRecord(ev[2], s)) != hipSuccess) return e;
+    for (int r0 = 0; r0 < a.tile_rows; r0 += step) {
+        RowPassArgs b = a;
+        b.u_base = a.u_base + r0;
+        b.tile_rows = min(step, a.tile_rows - r0);
+        b.X0 = a.X0 + (size_t)r0 * a.ldx;
+        const dim3 g1((b.M2 + 63) / 64, (b.tile_rows + 3) / 4);
+        if ((e = launch_fft1(b, g1, s, false)) != hipSuccess) return e;
+        if (!split && ev && (e = hipEventRecord(ev[2], s)) != hipSuccess) return e;
+        if ((e = launch_conv(b, s)) != hipSuccess) return e;
+        if (!split && ev && (e = hipEventRecord(ev[3], s)) != hipSuccess) return e;
+        if ((e = launch_fft1(b, g1, s, true)) != hipSuccess) return e;
+    }
+    if (split && ev && (e = hipEventRecord(ev[3], s)) != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev[4], s)) != hipSuccess) return e;
     return hipSuccess;
 }

@@ -18,6 +18,7 @@ SparseFFTResult run(const COOMatrix& coo, int u_tile, Variant v) {
     if (coo.nnz < 0 || (size_t)coo.nnz > coo.row_idx.size() || (size_t)coo.nnz > coo.col_idx.size())
         throw std::runtime_error("COOMatrix nnz does not match its index arrays");
     spfft_result r;
+    r.struct_size = sizeof(r);
     const int* hr = coo.row_idx.data();
     const int* hc = coo.col_idx.data();
     spfft_status st;
@@ -69,10 +70,9 @@ SparseFFTResult sparse_fft_csc_stockham_streaming(const COOMatrix& coo, int u_ti
         ~Pinned() { (void)hipHostFree(p); }
     } guard{pinned};
     spfft_opts o;
-    spfft_opts_default(&o);
-    o.length_rule = fft_backend == 3 ? SPFFT_SMOOTH_7 : SPFFT_SMOOTH_3;
-    o.u_tile = u_tile;
+    spfft_reference_opts(fft_backend == 3 ? SPFFT_SMOOTH_7 : SPFFT_SMOOTH_3, u_tile, &o);
     spfft_result r;
+    r.struct_size = sizeof(r);
     if (spfft_run_host_coo_to_host(coo.rows, coo.cols, coo.nnz, coo.row_idx.data(), coo.col_idx.data(), &o,
                                    pinned, pitch, &r) != SPFFT_OK)
         throw std::runtime_error(spfft_last_error());

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -147,9 +148,9 @@ spfft_status spfft_batch_create(int count, const int* rows, const int* cols, con
     std::vector<double> work(count);
     for (int i = 0; i < count; i++) {
         spfft_opts o;
-        if (opts) o = opts[i]; else spfft_opts_default(&o);
-        if (!opts) o.u_tile = std::max(rows[i], 1);  // one tile per matrix
-        if (o.u_tile <= 0) return fail(SPFFT_ERR_INVALID_ARG, "CSC tile size must be positive");
+        spfft_status st0 = read_opts(opts ? &opts[i] : nullptr, &o);
+        if (st0 != SPFFT_OK) return fail(st0, "matrix " + std::to_string(i) + ": " + spfft_last_error());
+        if (o.u_tile < 0) return fail(SPFFT_ERR_INVALID_ARG, "CSC tile size must be positive");
         spfft_plan_s* p = nullptr;
         spfft_status st = plan_create_ex(rows[i], cols[i], &o, false, &p);
         if (st != SPFFT_OK) return st;
@@ -202,6 +203,11 @@ spfft_status spfft_batch_destroy(spfft_batch b) {
 spfft_status spfft_batch_layout(spfft_batch b, int64_t* total_elems, int64_t* offsets, int* pitches,
                                 spfft_plan_info* infos) {
     if (!b) return fail(SPFFT_ERR_INVALID_ARG, "NULL batch");
+    // infos: count entries, each with struct_size set by the caller
+    for (int i = 0; infos && i < b->count; i++) {
+        spfft_status st = check_info(&infos[i]);
+        if (st != SPFFT_OK) return st;
+    }
     if (total_elems) *total_elems = b->out_total;
     for (int i = 0; i < b->count; i++) {
         if (offsets) offsets[i] = b->out_off[i];
@@ -211,11 +217,27 @@ spfft_status spfft_batch_layout(spfft_batch b, int64_t* total_elems, int64_t* of
     return SPFFT_OK;
 }
 
+// NULL checks and the per-matrix COO ranges: 0 <= coo_off[i + 1] - coo_off[i] <= INT_MAX,
+// checked in 64 bits before any use
+static spfft_status check_batch_io(spfft_batch b, const int64_t* coo_off, const void* d_out) {
+    if (!b) return fail(SPFFT_ERR_INVALID_ARG, "NULL batch");
+    if (!coo_off) return fail(SPFFT_ERR_INVALID_ARG, "NULL coo_off");
+    if (!d_out) return fail(SPFFT_ERR_INVALID_ARG, "NULL output arena");
+    for (int i = 0; i < b->count; i++) {
+        const int64_t n = coo_off[i + 1] - coo_off[i];
+        if (coo_off[i] < 0 || n < 0 || n > (int64_t)INT_MAX)
+            return fail(SPFFT_ERR_INVALID_ARG, "coo_off of matrix " + std::to_string(i) +
+                                                   " is not a non-decreasing range of at most INT_MAX pairs");
+    }
+    return SPFFT_OK;
+}
+
 spfft_status spfft_batch_execute(spfft_batch b, const int* d_row, const int* d_col, const int64_t* coo_off,
                                  void* d_out, void* stream) {
-    if (!b) return fail(SPFFT_ERR_INVALID_ARG, "NULL batch");
+    spfft_status st = check_batch_io(b, coo_off, d_out);
+    if (st != SPFFT_OK) return st;
     for (int i = 0; i < b->count; i++) {
-        spfft_status st = ensure_csc_capacity(*b->plans[i], (int)(coo_off[i + 1] - coo_off[i]));
+        st = ensure_csc_capacity(*b->plans[i], (int)(coo_off[i + 1] - coo_off[i]));
         if (st != SPFFT_OK) return st;
     }
     return run_batch(*b, d_row, d_col, coo_off, d_out, (hipStream_t)stream);
@@ -223,8 +245,10 @@ spfft_status spfft_batch_execute(spfft_batch b, const int* d_row, const int* d_c
 
 spfft_status spfft_batch_capture(spfft_batch b, const int* d_row, const int* d_col, const int64_t* coo_off,
                                  void* d_out, spfft_graph* out) {
-    if (!b || !out || !coo_off) return fail(SPFFT_ERR_INVALID_ARG, "NULL argument");
+    if (!out) return fail(SPFFT_ERR_INVALID_ARG, "NULL argument");
     *out = nullptr;
+    spfft_status st0 = check_batch_io(b, coo_off, d_out);
+    if (st0 != SPFFT_OK) return st0;
     for (int i = 0; i < b->count; i++) {
         spfft_status st = ensure_csc_capacity(*b->plans[i], (int)(coo_off[i + 1] - coo_off[i]));
         if (st != SPFFT_OK) return st;
@@ -262,8 +286,8 @@ spfft_status spfft_batch_check_input(spfft_batch b, void* stream) {
 // ms = GPU time of the single graph launch (the reference's timed region).
 spfft_status spfft_csc_bluestein_mixed_radix_graph(int rows, int cols, int nnz, const int* h_row,
                                                    const int* h_col, int u_tile, spfft_result* res) {
-    if (!res) return fail(SPFFT_ERR_INVALID_ARG, "NULL result");
-    std::memset(res, 0, sizeof(*res));
+    spfft_status st = check_result(res);
+    if (st != SPFFT_OK) return st;
     if (u_tile <= 0) return fail(SPFFT_ERR_INVALID_ARG, "CSC tile size must be positive");
     if (rows <= 0 || cols <= 0 || nnz < 0) return fail(SPFFT_ERR_INVALID_ARG, "bad matrix shape");
     if (nnz > 0 && (!h_row || !h_col)) return fail(SPFFT_ERR_INVALID_ARG, "NULL COO pointer");
@@ -273,10 +297,9 @@ spfft_status spfft_csc_bluestein_mixed_radix_graph(int rows, int cols, int nnz, 
     size_t free0 = 0, total = 0;
     SPFFT_HIP(hipMemGetInfo(&free0, &total), "hipMemGetInfo");
     spfft_opts o;
-    spfft_opts_default(&o);
-    o.u_tile = u_tile;
+    spfft_reference_opts(SPFFT_SMOOTH_3, u_tile, &o);
     spfft_plan p = nullptr;
-    spfft_status st = spfft_plan_create(rows, cols, &o, &p);
+    st = spfft_plan_create(rows, cols, &o, &p);
     if (st != SPFFT_OK) return st;
     std::unique_ptr<spfft_plan_s> pg(p);
     DevBuf drow, dcol, dout;
@@ -316,6 +339,8 @@ spfft_status spfft_csc_bluestein_mixed_radix_graph(int rows, int cols, int nnz, 
     res->ms = ms;
     res->ms_e2e = ms;
     res->mem_bytes = free1 < free0 ? free0 - free1 : 0;
+    res->fft_len = p->M;
+    res->row_transform = SPFFT_TRANSFORM_BLUESTEIN;
     return SPFFT_OK;
 }
 

@@ -42,6 +42,43 @@ spfft_status hip_fail(hipError_t e, const char* what) {
                 std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
 }
 
+// The caller's options (NULL: the defaults), rejected when the caller's struct has
+// another size than this library's.
+spfft_status read_opts(const spfft_opts* in, spfft_opts* out) {
+    if (!in) {
+        spfft_opts_default(out);
+        return SPFFT_OK;
+    }
+    if (in->struct_size != (int)sizeof(spfft_opts))
+        return fail(SPFFT_ERR_INVALID_ARG, "spfft_opts.struct_size is " + std::to_string(in->struct_size) +
+                                               ", this library's spfft_opts has " +
+                                               std::to_string(sizeof(spfft_opts)) +
+                                               " bytes (initialise it with spfft_opts_default)");
+    *out = *in;
+    return SPFFT_OK;
+}
+
+spfft_status check_info(const spfft_plan_info* info) {
+    if (!info) return fail(SPFFT_ERR_INVALID_ARG, "NULL info");
+    if (info->struct_size != (int)sizeof(spfft_plan_info))
+        return fail(SPFFT_ERR_INVALID_ARG, "spfft_plan_info.struct_size is " + std::to_string(info->struct_size) +
+                                               ", this library's spfft_plan_info has " +
+                                               std::to_string(sizeof(spfft_plan_info)) + " bytes");
+    return SPFFT_OK;
+}
+
+// Checks the caller's result struct and clears it (struct_size kept).
+spfft_status check_result(spfft_result* res) {
+    if (!res) return fail(SPFFT_ERR_INVALID_ARG, "NULL result");
+    if (res->struct_size != (int)sizeof(spfft_result))
+        return fail(SPFFT_ERR_INVALID_ARG, "spfft_result.struct_size is " + std::to_string(res->struct_size) +
+                                               ", this library's spfft_result has " +
+                                               std::to_string(sizeof(spfft_result)) + " bytes");
+    std::memset(res, 0, sizeof(*res));
+    res->struct_size = (int)sizeof(spfft_result);
+    return SPFFT_OK;
+}
+
 }  // namespace spfft_internal
 
 using namespace spfft_internal;
@@ -415,6 +452,7 @@ static spfft_status rowpass_setup(spfft_plan_s& p, int64_t pitch, float2* T, flo
 static spfft_status rowpass_tile(spfft_plan_s& p, RowPassArgs& a, int u0, int n, hipStream_t s) {
     a.u_base = u0;
     a.tile_rows = n;
+    a.sub_rows = p.base.fused ? 0 : p.t_rows;
     hipEvent_t* ev = nullptr;
     if (p.profiling) {
         const size_t at = p.ev_used;
@@ -506,9 +544,18 @@ const char* spfft_version(void) { return "spfft-mi355x 0.1 (gfx950)"; }
 void spfft_opts_default(spfft_opts* o) {
     if (!o) return;
     std::memset(o, 0, sizeof(*o));
+    o->struct_size = (int)sizeof(spfft_opts);
     o->length_rule = SPFFT_SMOOTH_3;
-    o->u_tile = 128;
+    o->u_tile = 0;  // automatic
     o->precision = SPFFT_PREC_FP32;
+}
+
+void spfft_reference_opts(int rule, int u_tile, spfft_opts* o) {
+    if (!o) return;
+    spfft_opts_default(o);
+    o->length_rule = rule;
+    o->u_tile = u_tile;
+    o->row_transform = SPFFT_TRANSFORM_BLUESTEIN;
 }
 
 int spfft_next_3_smooth(int n) {
@@ -602,9 +649,10 @@ int find_rader(int L, int m1_req, size_t lds_limit, spfft::FusedShape* out) {
 spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t budget, size_t lds_limit,
                         spfft_plan_s* p) {
     spfft_opts o;
-    if (opts_in) o = *opts_in; else spfft_opts_default(&o);
+    spfft_status st0 = read_opts(opts_in, &o);
+    if (st0 != SPFFT_OK) return st0;
     if (rows <= 0 || cols <= 0) return fail(SPFFT_ERR_INVALID_ARG, "rows and cols must be positive");
-    if (o.u_tile <= 0) return fail(SPFFT_ERR_INVALID_ARG, "CSC tile size must be positive");
+    if (o.u_tile < 0) return fail(SPFFT_ERR_INVALID_ARG, "CSC tile size must be positive");
     if (o.precision != SPFFT_PREC_FP32 && o.precision != SPFFT_PREC_BF16X3)
         return fail(SPFFT_ERR_UNSUPPORTED, "unknown precision form");
     const bool mfma = o.precision == SPFFT_PREC_BF16X3;
@@ -677,8 +725,6 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     p->P = (p->V + 3) / 4 * 4;
     p->M = M;
     p->opts = o;
-    // one tile is at most 65535 workgroup rows of the length-M1 kernels' grid
-    p->opts.u_tile = std::min(o.u_tile, 65535);
     p->packed = rows <= 65536 && cols <= 65536;
     if (o.row_kernel < SPFFT_ROW_AUTO || o.row_kernel > SPFFT_ROW_FUSED)
         return fail(SPFFT_ERR_INVALID_ARG, "row_kernel must be 0, 1 or 2");
@@ -691,6 +737,7 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
                                                (mfma ? " in the BF16x3 form" : ""));
     RowPassArgs& a = p->base;
     spfft_status st = SPFFT_OK;
+    p->opts.u_tile = o.u_tile;
     if (fused_ok) {
         p->M1 = fs.m1;
         p->M2 = fs.r[0] * fs.r[1] * fs.r[2];
@@ -726,6 +773,23 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
         }
     }
     if (st != SPFFT_OK) return st;
+    // u_tile 0: one tile of every computed row
+    if (p->opts.u_tile == 0) p->opts.u_tile = rows / 2 + 1;
+    // one tile is at most 65535 workgroup rows of the length-M1 kernels' grid
+    p->opts.u_tile = std::min(p->opts.u_tile, 65535);
+    // four-step kernels: T holds t_rows rows at a time (0: as many as fit in
+    // SPFFT_FOUR_STEP_TILE_BYTES, a multiple of 4 -- the rows of a k_fft1 workgroup)
+    if (o.t_rows < 0) return fail(SPFFT_ERR_INVALID_ARG, "t_rows must be >= 0");
+    p->t_rows = 0;
+    if (!a.fused) {
+        const int tile = std::min(p->opts.u_tile, rows / 2 + 1);
+        long t = o.t_rows;
+        if (t == 0) {
+            t = (long)(SPFFT_FOUR_STEP_TILE_BYTES / ((size_t)M * sizeof(float2)));
+            if (t >= 4) t = t / 4 * 4;
+        }
+        p->t_rows = (int)std::max(1L, std::min<long>(t, tile));
+    }
     a.R = rows;
     a.C = cols;
     a.V = p->V;
@@ -761,6 +825,8 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
 }
 
 void fill_info(const spfft_plan_s* p, spfft_plan_info* info) {
+    std::memset(info, 0, sizeof(*info));
+    info->struct_size = (int)sizeof(spfft_plan_info);
     info->rows = p->R;
     info->cols = p->C;
     info->out_cols = p->V;
@@ -782,6 +848,10 @@ void fill_info(const spfft_plan_s* p, spfft_plan_info* info) {
     info->colpass_paired = pair_eligible(*p) ? 1 : 0;
     info->direct = p->base.direct;
     info->rader = p->base.rader;
+    info->row_transform = p->base.direct ? SPFFT_TRANSFORM_DIRECT
+                          : p->base.rader ? SPFFT_TRANSFORM_RADER
+                                          : SPFFT_TRANSFORM_BLUESTEIN;
+    info->t_rows = p->t_rows;
 }
 
 }  // namespace
@@ -789,7 +859,8 @@ void fill_info(const spfft_plan_s* p, spfft_plan_info* info) {
 extern "C" {
 
 spfft_status spfft_plan_describe(int rows, int cols, const spfft_opts* opts, spfft_plan_info* info) {
-    if (!info) return fail(SPFFT_ERR_INVALID_ARG, "NULL info");
+    spfft_status st0 = check_info(info);
+    if (st0 != SPFFT_OK) return st0;
     spfft_plan_s p;
     // the planner's view of a gfx950 device: 160 KiB of LDS per workgroup
     spfft_status st = plan_shape(rows, cols, opts, 65536, kGfx950Lds, &p);
@@ -810,7 +881,7 @@ namespace spfft_internal {
 
 void workspace_bytes(const spfft_plan_s& p, size_t* t_bytes, size_t* x0_bytes) {
     const size_t tile_rows = (size_t)std::min(p.opts.u_tile, p.R / 2 + 1);
-    *t_bytes = p.base.fused ? 0 : tile_rows * (size_t)p.M * sizeof(float2);
+    *t_bytes = p.base.fused ? 0 : (size_t)p.t_rows * (size_t)p.M * sizeof(float2);
     *x0_bytes = tile_rows * (size_t)p.base.ldx * sizeof(float2);
 }
 
@@ -936,7 +1007,7 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
     const int tile_rows = std::min(o.u_tile, rows / 2 + 1);
     if (own_workspace) {
         if (!a.fused)
-            SPFFT_HIP(p->T.alloc((size_t)tile_rows * M * sizeof(float2)), "hipMalloc(tile buffer)");
+            SPFFT_HIP(p->T.alloc((size_t)p->t_rows * M * sizeof(float2)), "hipMalloc(tile buffer)");
         SPFFT_HIP(p->X0.alloc((size_t)tile_rows * a.ldx * sizeof(float2)), "hipMalloc(pass-1 buffer)");
     }
     SPFFT_HIP(p->counts.alloc((size_t)(cols + 1) * 4), "hipMalloc(counts)");
@@ -982,7 +1053,9 @@ spfft_status spfft_plan_destroy(spfft_plan p) {
 }
 
 spfft_status spfft_plan_get_info(spfft_plan p, spfft_plan_info* info) {
-    if (!p || !info) return fail(SPFFT_ERR_INVALID_ARG, "NULL argument");
+    if (!p) return fail(SPFFT_ERR_INVALID_ARG, "NULL argument");
+    spfft_status st0 = check_info(info);
+    if (st0 != SPFFT_OK) return st0;
     fill_info(p, info);
     return SPFFT_OK;
 }
@@ -1087,11 +1160,11 @@ spfft_status spfft_free_output(void* d) {
 // frees it) or streamed to host memory h_out at pitch h_pitch.
 static spfft_status run_host_coo(int rows, int cols, int nnz, const int* h_row, const int* h_col,
                                  const spfft_opts* opts, void* h_out, int64_t h_pitch, spfft_result* res) {
-    if (!res) return fail(SPFFT_ERR_INVALID_ARG, "NULL result");
-    std::memset(res, 0, sizeof(*res));
+    spfft_status st = check_result(res);
+    if (st != SPFFT_OK) return st;
     spfft_opts o;
-    if (opts) o = *opts; else spfft_opts_default(&o);
-    if (o.u_tile <= 0) return fail(SPFFT_ERR_INVALID_ARG, "CSC tile size must be positive");
+    if ((st = read_opts(opts, &o)) != SPFFT_OK) return st;
+    if (o.u_tile < 0) return fail(SPFFT_ERR_INVALID_ARG, "CSC tile size must be positive");
     if (rows <= 0 || cols <= 0 || nnz < 0) return fail(SPFFT_ERR_INVALID_ARG, "bad matrix shape");
     if (nnz > 0 && (!h_row || !h_col)) return fail(SPFFT_ERR_INVALID_ARG, "NULL COO pointer");
     if (!check_host_coo(rows, cols, nnz, h_row, h_col))
@@ -1099,7 +1172,7 @@ static spfft_status run_host_coo(int rows, int cols, int nnz, const int* h_row, 
     size_t free0 = 0, total = 0;
     SPFFT_HIP(hipMemGetInfo(&free0, &total), "hipMemGetInfo");
     spfft_plan p = nullptr;
-    spfft_status st = spfft_plan_create(rows, cols, &o, &p);
+    st = spfft_plan_create(rows, cols, &o, &p);
     if (st != SPFFT_OK) return st;
     std::unique_ptr<spfft_plan_s> pg(p);
     DevBuf drow, dcol, dout;
@@ -1142,6 +1215,10 @@ static spfft_status run_host_coo(int rows, int cols, int nnz, const int* h_row, 
     res->ms = ms_loop;
     res->ms_e2e = ms_all;
     res->mem_bytes = free1 < free0 ? free0 - free1 : 0;
+    res->fft_len = p->M;
+    res->row_transform = p->base.direct ? SPFFT_TRANSFORM_DIRECT
+                         : p->base.rader ? SPFFT_TRANSFORM_RADER
+                                         : SPFFT_TRANSFORM_BLUESTEIN;
     return SPFFT_OK;
 }
 
@@ -1154,7 +1231,7 @@ spfft_status spfft_run_host_coo_to_host(int rows, int cols, int nnz, const int* 
                                         const spfft_opts* opts, void* h_out, int64_t pitch,
                                         spfft_result* res) {
     if (!h_out) {
-        if (res) std::memset(res, 0, sizeof(*res));
+        if (check_result(res) == SPFFT_OK) {}
         return fail(SPFFT_ERR_INVALID_ARG, "NULL host output pointer");
     }
     return run_host_coo(rows, cols, nnz, h_row, h_col, opts, h_out, pitch, res);
@@ -1162,19 +1239,17 @@ spfft_status spfft_run_host_coo_to_host(int rows, int cols, int nnz, const int* 
 
 spfft_status spfft_csc_bluestein_mixed_radix(int rows, int cols, int nnz, const int* h_row,
                                              const int* h_col, int u_tile, spfft_result* res) {
+    if (u_tile <= 0) return fail(SPFFT_ERR_INVALID_ARG, "CSC tile size must be positive");
     spfft_opts o;
-    spfft_opts_default(&o);
-    o.length_rule = SPFFT_SMOOTH_3;
-    o.u_tile = u_tile;
+    spfft_reference_opts(SPFFT_SMOOTH_3, u_tile, &o);
     return spfft_run_host_coo(rows, cols, nnz, h_row, h_col, &o, res);
 }
 
 spfft_status spfft_csc_bluestein_cufft_smooth(int rows, int cols, int nnz, const int* h_row,
                                               const int* h_col, int u_tile, spfft_result* res) {
+    if (u_tile <= 0) return fail(SPFFT_ERR_INVALID_ARG, "CSC tile size must be positive");
     spfft_opts o;
-    spfft_opts_default(&o);
-    o.length_rule = SPFFT_SMOOTH_7;
-    o.u_tile = u_tile;
+    spfft_reference_opts(SPFFT_SMOOTH_7, u_tile, &o);
     return spfft_run_host_coo(rows, cols, nnz, h_row, h_col, &o, res);
 }
 

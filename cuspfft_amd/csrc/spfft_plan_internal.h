@@ -14,6 +14,9 @@ namespace spfft_internal {
 
 spfft_status fail(spfft_status st, const std::string& msg);
 spfft_status hip_fail(hipError_t e, const char* what);
+spfft_status read_opts(const spfft_opts* in, spfft_opts* out);
+spfft_status check_info(const spfft_plan_info* info);
+spfft_status check_result(spfft_result* res);
 
 #define SPFFT_HIP(call, what)                                                      \
     do {                                                                           \
@@ -60,6 +63,7 @@ using spfft_internal::DevBuf;
 struct spfft_plan_s {
     int R = 0, C = 0, V = 0, P = 0, M = 0, M1 = 0, M2 = 0;
     spfft_opts opts{};
+    int t_rows = 0;  // four-step kernels: rows of the tile buffer T per sub-tile
     bool packed = false;
     bool colpass_full = false;
     spfft::RowPassArgs base{};  // kernel argument template (pointers to the tables filled in)

@@ -30,6 +30,7 @@ struct RowPassArgs {
     int R, C, V, P;            // rows, cols, C/2+1, output pitch (elements)
     int M, M1, M2;
     int u_base, tile_rows;     // first output row of this tile, rows in tile
+    int sub_rows;              // four-step kernels: rows of T per sub-tile (0: the whole tile)
     int ldx;                   // row stride of X0 (elements)
     int G2;                    // k_conv: (u, k1) rows per workgroup
     int conv_kind;             // 1: k_conv_fast<f2.radix[0], f2.radix[1]>, 0: k_conv_generic

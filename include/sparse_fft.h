@@ -15,6 +15,10 @@ struct SparseFFTResult {
     std::vector<hipFloatComplex> h_output;  // streamed variants: rows * output_cols, host memory
 };
 
+// Every entry point here runs the reference's algorithm, the Bluestein row pass at
+// its smooth length (spfft_reference_opts); the native C ABI (spfft_c.h) also
+// offers the direct and Rader row transforms of the same half spectrum.
+//
 // Bluestein length next_3_smooth(2*cols - 1).  Replaces
 // sparse_fft_csc_bluestein_mixed_radix (reference src/sparse_fft_bluestein.cu:2256-2392).
 SparseFFTResult sparse_fft_csc_bluestein_mixed_radix(const COOMatrix& coo, int u_tile = 128);

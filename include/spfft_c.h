@@ -44,9 +44,17 @@ enum { SPFFT_SMOOTH_3 = 3, SPFFT_SMOOTH_7 = 7 };
  * with a register-codelet convolution kernel for m2. */
 enum { SPFFT_PREC_FP32 = 0, SPFFT_PREC_BF16X3 = 1 };
 
+/* The structs that grow with the library -- spfft_opts, spfft_plan_info and
+ * spfft_result -- start with struct_size = sizeof(struct) as the caller's header
+ * declares it: spfft_opts_default sets it for spfft_opts, and a caller sets it in
+ * spfft_plan_info / spfft_result before asking for one.  A library built against
+ * another layout of the struct rejects the call with SPFFT_ERR_INVALID_ARG instead
+ * of reading or writing past the end of the caller's struct. */
 typedef struct spfft_opts {
+    int struct_size; /* sizeof(spfft_opts)                                    */
     int length_rule; /* SPFFT_SMOOTH_3 or SPFFT_SMOOTH_7                      */
-    int u_tile;      /* output rows per tile (reference --csc-tile), > 0       */
+    int u_tile;      /* output rows per tile (reference --csc-tile); 0: one tile of all
+                        rows/2 + 1 computed rows                                   */
     int precision;   /* SPFFT_PREC_*                                           */
     int fft_len;     /* 0: from length_rule; else an explicit 7-smooth M >= 2*cols-1 */
     int m1;          /* 0: automatic; else the first factor of M = m1 * m2 (a 7-smooth m1 <= 64) */
@@ -60,6 +68,11 @@ typedef struct spfft_opts {
     int row_transform; /* SPFFT_TRANSFORM_*: Bluestein, a direct length-cols FFT, or Rader     */
     int row_persist;   /* fused row kernel: 0 persistent (the resident workgroups loop over the
                           rows and load the twiddle tables once), 1 one workgroup per row group */
+    int t_rows;        /* four-step kernels: rows of the tile buffer T (tile x M complex64); the
+                          column pass fills a u_tile of rows once, then k_fft1 / k_conv / k_ifft1
+                          run over it t_rows at a time.  0: the rows whose T fits in
+                          SPFFT_FOUR_STEP_TILE_BYTES, so T stays in the 256 MB last-level cache
+                          between the three kernels instead of round-tripping through HBM */
 } spfft_opts;
 
 /* Row-pass kernels.  FOUR_STEP: k_fft1 + k_conv + k_ifft1 with the tile buffer T
@@ -90,6 +103,7 @@ enum { SPFFT_TRANSFORM_AUTO = 0, SPFFT_TRANSFORM_BLUESTEIN = 1, SPFFT_TRANSFORM_
        SPFFT_TRANSFORM_RADER = 3 };
 
 typedef struct spfft_plan_info {
+    int struct_size;   /* sizeof(spfft_plan_info), set by the caller      */
     int rows, cols;
     int out_cols;      /* cols/2 + 1                                   */
     int pitch;         /* P, elements                                  */
@@ -109,7 +123,12 @@ typedef struct spfft_plan_info {
     int colpass_paired;     /* 1 if the column pass pairs rows u and R/2 - u            */
     int direct;             /* 1: direct length-cols FFT (fft_len = cols), 0: Bluestein  */
     int rader;              /* 1: Rader convolutions of length fft_len = cols/2 - 1      */
+    int row_transform;      /* SPFFT_TRANSFORM_BLUESTEIN, _DIRECT or _RADER (resolved)  */
+    int t_rows;             /* four-step kernels: rows of T per sub-tile (0: fused)      */
 } spfft_plan_info;
+
+/* Tile-buffer budget of the automatic u_tile of four-step plans (bytes of T). */
+#define SPFFT_FOUR_STEP_TILE_BYTES (96u << 20)
 
 typedef struct spfft_plan_s* spfft_plan;
 
@@ -119,11 +138,14 @@ typedef struct spfft_plan_s* spfft_plan;
  * timed region, src/sparse_fft_bluestein.cu:2322-2377); ms_e2e = GPU time of
  * the CSC build plus the tile loop. */
 typedef struct spfft_result {
+    int struct_size; /* sizeof(spfft_result), set by the caller */
     void* d_output;
     int output_cols; /* pitch P */
     float ms;
     float ms_e2e;
     size_t mem_bytes; /* device free-memory delta over the call */
+    int fft_len;       /* the plan's M (Bluestein length, or cols / cols/2 - 1) */
+    int row_transform; /* SPFFT_TRANSFORM_BLUESTEIN, _DIRECT or _RADER        */
 } spfft_result;
 
 /* Device view of the compact CSC built by the last execute / csc_build call
@@ -196,8 +218,16 @@ spfft_status spfft_profile_read(spfft_plan plan, void* stream, spfft_profile* ou
  *                                     (include/sparse_fft.h:57, src/sparse_fft_bluestein.cu:2256-2392)
  *   spfft_csc_bluestein_cufft_smooth  replaces sparse_fft_csc_bluestein_cufft_smooth
  *                                     (include/sparse_fft.h:70, src/sparse_fft_bluestein.cu:2557-2719)
+ * Both always run the reference's algorithm: the Bluestein row pass at
+ * M = next_3_smooth(2*cols-1) (mixed radix) or next_7_smooth(2*cols-1) (cuFFT
+ * smooth), with u_tile rows per tile (spfft_reference_opts).  The direct and
+ * Rader row transforms, which compute the same half spectrum with less work, are
+ * reached through spfft_opts (row_transform AUTO) and spfft_run_host_coo.
  * u_tile <= 0 is rejected with SPFFT_ERR_INVALID_ARG ("CSC tile size must be
  * positive", as the reference throws at :2264-2265). */
+/* The options of the reference-shaped entry points: length_rule = rule (3 or 7),
+ * u_tile, row_transform = SPFFT_TRANSFORM_BLUESTEIN, the rest defaults. */
+void spfft_reference_opts(int rule, int u_tile, spfft_opts* opts);
 spfft_status spfft_csc_bluestein_mixed_radix(int rows, int cols, int nnz, const int* h_row,
                                              const int* h_col, int u_tile, spfft_result* result);
 spfft_status spfft_csc_bluestein_cufft_smooth(int rows, int cols, int nnz, const int* h_row,

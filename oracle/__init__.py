@@ -140,3 +140,15 @@ def rel_l2(a, b) -> float:
     b = np.asarray(b, np.complex128)
     den = np.linalg.norm(b.ravel())
     return float(np.linalg.norm((a - b).ravel()) / (den if den > 0 else 1.0))
+
+
+def write_reference(path: str, rows: int, cols: int, row, col, nthreads: int = 0) -> np.ndarray:
+    """Writes the FP64 half spectrum of a binary COO matrix as a reference file for the
+    benchmark driver's --check-ref: the 8 bytes "SPFFTREF", int32 rows, int32 cols, then
+    rows x (cols//2 + 1) complex128 values, row-major.  Returns the spectrum."""
+    X = sparse_rfft2_f64(rows, cols, row, col, nthreads=nthreads)
+    with open(path, "wb") as f:
+        f.write(b"SPFFTREF")
+        f.write(np.asarray([rows, cols], np.int32).tobytes())
+        f.write(np.ascontiguousarray(X, np.complex128).tobytes())
+    return X

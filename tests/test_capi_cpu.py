@@ -192,7 +192,7 @@ def test_fused_length_ladder():
 def test_planner_errors():
     import cuspfft_amd as sp
     with pytest.raises(sp.SpfftError) as e:
-        sp.describe_plan(10, 10, u_tile=0)
+        sp.describe_plan(10, 10, u_tile=-1)
     assert "CSC tile size must be positive" in str(e.value)
     with pytest.raises(sp.SpfftError):
         sp.describe_plan(10, 10, fft_len=11)  # not 7-smooth
@@ -258,3 +258,92 @@ def test_codelets_on_host(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout
     assert out.stdout.strip().endswith("OK")
+
+
+def _describe_raw(rows, cols, opts, info):
+    import ctypes
+    import cuspfft_amd as sp
+    return sp.load().spfft_plan_describe(rows, cols, ctypes.byref(opts), ctypes.byref(info))
+
+
+def test_struct_size_mismatch_is_rejected():
+    """A caller built against another layout of spfft_opts / spfft_plan_info / spfft_result
+    (struct_size differs from the library's sizeof) gets SPFFT_ERR_INVALID_ARG, never a
+    read or write past the end of its struct."""
+    import ctypes
+    import cuspfft_amd as sp
+    L = sp.load()
+    o = sp.Opts()
+    L.spfft_opts_default(ctypes.byref(o))
+    assert o.struct_size == ctypes.sizeof(sp.Opts)
+    info = sp.PlanInfo()
+    assert _describe_raw(100, 100, o, info) == 0
+    assert info.struct_size == ctypes.sizeof(sp.PlanInfo) and info.fft_len >= 199
+
+    # an older, shorter spfft_opts: the fields up to row_transform, without row_persist
+    class ShortOpts(ctypes.Structure):
+        _fields_ = sp.Opts._fields_[:-1]
+    so = ShortOpts()
+    so.struct_size = ctypes.sizeof(ShortOpts)
+    so.length_rule, so.u_tile = 3, 0
+    rc = L.spfft_plan_describe(100, 100, ctypes.cast(ctypes.byref(so), ctypes.POINTER(sp.Opts)),
+                               ctypes.byref(info))
+    assert rc == 1 and "struct_size" in L.spfft_last_error().decode()
+    o.struct_size = 0
+    assert _describe_raw(100, 100, o, info) == 1
+    L.spfft_opts_default(ctypes.byref(o))
+    bad = sp.PlanInfo()
+    bad.struct_size = ctypes.sizeof(sp.PlanInfo) - 4
+    assert _describe_raw(100, 100, o, bad) == 1
+    assert "spfft_plan_info" in L.spfft_last_error().decode()
+    # spfft_result: checked before any device work (no GPU needed for the rejection)
+    res = sp.Result()
+    res.struct_size = 8
+    r = np.zeros(1, np.int32)
+    rc = L.spfft_csc_bluestein_mixed_radix(4, 4, 1, r.ctypes.data, r.ctypes.data, 128, ctypes.byref(res))
+    assert rc == 1 and "spfft_result" in L.spfft_last_error().decode()
+
+
+def test_reference_opts_run_bluestein_at_the_reference_length():
+    """The reference-shaped entry points plan the reference's algorithm: Bluestein at
+    next_3_smooth(2 cols - 1) (mixed radix) or next_7_smooth (cuFFT smooth), also for
+    widths where the native plan takes the Rader (c3) or direct (p14) transform."""
+    import ctypes
+    import cuspfft_amd as sp
+    L = sp.load()
+    for cols, native in ((16706, "rader"), (16384, "direct"), (2310, None), (8219, None)):
+        for rule, nxt in ((3, sp.next_3_smooth), (7, sp.next_7_smooth)):
+            o = sp.Opts()
+            L.spfft_reference_opts(rule, 128, ctypes.byref(o))
+            assert o.struct_size == ctypes.sizeof(sp.Opts) and o.u_tile == 128
+            info = sp.PlanInfo()
+            assert _describe_raw(cols, cols, o, info) == 0
+            assert info.fft_len == nxt(2 * cols - 1), (cols, rule, info.fft_len)
+            assert info.row_transform == sp.TRANSFORM_BLUESTEIN and not info.rader and not info.direct
+            d = sp.describe_plan(cols, cols, **sp.reference_opts(rule, 128))
+            assert d["fft_len"] == nxt(2 * cols - 1)
+        auto = sp.describe_plan(cols, cols)
+        if native:
+            assert auto[native] == 1
+
+
+def test_automatic_tile():
+    """u_tile 0: one tile of all R/2 + 1 computed rows (the column pass then pairs rows
+    for even R).  Four-step plans (c5's M = 82944) run k_fft1 / k_conv / k_ifft1 over it
+    t_rows at a time, the rows whose tile buffer fits in SPFFT_FOUR_STEP_TILE_BYTES (a
+    multiple of 4), so T stays in the last-level cache."""
+    import cuspfft_amd as sp
+    for n in (2310, 8219, 16706, 16384, 40421):
+        info = sp.describe_plan(n, n)
+        assert info["u_tile"] == n // 2 + 1
+    assert sp.describe_plan(16706, 16706)["colpass_paired"] == 1
+    assert sp.describe_plan(16706, 16706)["t_rows"] == 0  # on-chip row kernel: no T
+    c5 = sp.describe_plan(40421, 40421)
+    assert c5["row_kernel"] == sp.ROW_FOUR_STEP and c5["fft_len"] == 82944
+    t = c5["t_rows"]
+    assert t % 4 == 0 and t * 82944 * 8 <= 96 << 20 and (t + 4) * 82944 * 8 > 96 << 20
+    assert sp.describe_plan(40, 40421)["t_rows"] == 21  # every row of the tile
+    assert sp.describe_plan(40421, 40421, t_rows=7)["t_rows"] == 7
+    assert sp.describe_plan(40421, 40421, u_tile=5, t_rows=7)["t_rows"] == 5
+    with pytest.raises(sp.SpfftError):
+        sp.describe_plan(40421, 40421, t_rows=-1)

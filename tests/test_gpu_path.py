@@ -270,6 +270,41 @@ def test_reference_shaped_entry_points(gpu, oracle_mod):
     assert st == 1 and b"CSC tile size must be positive" in L.spfft_last_error()
 
 
+@pytest.mark.parametrize("cols", [16706, 16384])
+def test_reference_entry_points_run_bluestein(gpu, oracle_mod, cols):
+    """On widths where the native plan takes the Rader (c3's 16706) or direct (16384)
+    row transform, the reference-shaped entry points still run the reference's
+    algorithm, Bluestein at next_3_smooth / next_7_smooth(2 cols - 1), as reported in
+    spfft_result, and match the FP64 oracle."""
+    sp = gpu
+    L = sp.load()
+    m = sp.generate("uniform", cols, 2500, seed=11)
+    rows = np.array([0, 1, 2, cols // 2, cols - 1], np.int32)
+    ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, u_list=rows, nthreads=8)
+    for fn, M in ((L.spfft_csc_bluestein_mixed_radix, sp.next_3_smooth(2 * cols - 1)),
+                  (L.spfft_csc_bluestein_cufft_smooth, sp.next_7_smooth(2 * cols - 1)),
+                  (L.spfft_csc_bluestein_mixed_radix_graph, sp.next_3_smooth(2 * cols - 1))):
+        print(f"[ref-entry] cols={cols} {fn.__name__}", flush=True)
+        res = sp.Result()
+        st = fn(m.rows, m.cols, m.nnz, m.row_idx.ctypes.data, m.col_idx.ctypes.data, 128, ctypes.byref(res))
+        assert st == 0, L.spfft_last_error()
+        assert res.fft_len == M and res.row_transform == sp.TRANSFORM_BLUESTEIN
+        X = sp.device_ptr_to_host(res.d_output, m.rows, res.output_cols)[rows, : m.cols // 2 + 1]
+        assert L.spfft_free_output(res.d_output) == 0
+        assert oracle_mod.rel_l2(X, ref) <= REL_L2_PLAIN
+    # the native API on the same width: the Rader / direct transform
+    res = sp.Result()
+    o = sp._opts()
+    st = L.spfft_run_host_coo(m.rows, m.cols, m.nnz, m.row_idx.ctypes.data, m.col_idx.ctypes.data,
+                              ctypes.byref(o), ctypes.byref(res))
+    assert st == 0, L.spfft_last_error()
+    want = sp.TRANSFORM_RADER if cols == 16706 else sp.TRANSFORM_DIRECT
+    assert res.row_transform == want
+    X = sp.device_ptr_to_host(res.d_output, m.rows, res.output_cols)[rows, : m.cols // 2 + 1]
+    assert L.spfft_free_output(res.d_output) == 0
+    assert oracle_mod.rel_l2(X, ref) <= REL_L2_PLAIN
+
+
 def test_python_mirror(gpu, oracle_mod):
     sp = gpu
     m = sp.config_matrix("c1")
@@ -518,7 +553,7 @@ def test_python_mirror_graph(gpu, oracle_mod):
         sp.sparse_fft_csc_bluestein_mixed_radix_graph(m, 0)
 
 
-def test_cli_reference_flags(gpu):
+def test_cli_reference_flags(gpu, oracle_mod, tmp_path):
     """The benchmark driver with the reference's flags runs all three variants
     through the C++ drop-in entry points and prints the reference's table lines."""
     import os
@@ -534,6 +569,26 @@ def test_cli_reference_flags(gpu):
                   "Sparse CSC (Stockham stream)", "Sparse CSC (cufft stream)"):
         assert label in res.stdout
     assert "OOM" not in res.stdout
+    # the reference's check lines against an FP64 reference spectrum of the same matrix,
+    # and this build's native plan (--csc-auto)
+    m = sp.config_matrix("c1")
+    ref_path = os.path.join(str(tmp_path), "c1.ref")
+    oracle_mod.write_reference(ref_path, m.rows, m.cols, m.row_idx, m.col_idx, nthreads=8)
+    res = subprocess.run([cli, "gen:c1", "--csc-mixed-radix", "--csc-cufft-smooth", "--csc-auto",
+                          "--csc-stockham-stream", "--repeat", "3", "--check-ref", ref_path],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    checks = [ln for ln in res.stdout.splitlines() if ln.startswith("  check ")]
+    assert len(checks) == 4, res.stdout
+    for ln in checks:
+        f = ln.split()
+        max_abs, max_rel, rms = float(f[f.index("max_abs") + 1]), float(f[f.index("max_rel") + 1]), \
+            float(f[f.index("rms_abs") + 1])
+        # |X| reaches nnz = 16000 at X[0][0] and is ~sqrt(nnz) = 126 on average: FP32
+        # rounding leaves absolute errors of ~1e-4 (rel-L2 ~2e-7), larger at the largest
+        # entries; max_rel divides by max(1, |ref|), so small entries count absolutely
+        assert max_abs < 2e-2 and max_rel < 1e-3 and rms < 1e-3, ln
+    assert "Sparse CSC (auto)" in res.stdout
     bad = subprocess.run([cli, "gen:c1", "--csc-mixed-radix", "--csc-tile", "0"],
                          capture_output=True, text=True, timeout=60)
     assert bad.returncode == 1 and "--csc-tile must be positive" in bad.stderr
