@@ -777,18 +777,12 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     if (p->opts.u_tile == 0) p->opts.u_tile = rows / 2 + 1;
     // one tile is at most 65535 workgroup rows of the length-M1 kernels' grid
     p->opts.u_tile = std::min(p->opts.u_tile, 65535);
-    // four-step kernels: T holds t_rows rows at a time (0: as many as fit in
-    // SPFFT_FOUR_STEP_TILE_BYTES, a multiple of 4 -- the rows of a k_fft1 workgroup)
+    // four-step kernels: T holds t_rows rows at a time (0: the whole tile)
     if (o.t_rows < 0) return fail(SPFFT_ERR_INVALID_ARG, "t_rows must be >= 0");
     p->t_rows = 0;
     if (!a.fused) {
         const int tile = std::min(p->opts.u_tile, rows / 2 + 1);
-        long t = o.t_rows;
-        if (t == 0) {
-            t = (long)(SPFFT_FOUR_STEP_TILE_BYTES / ((size_t)M * sizeof(float2)));
-            if (t >= 4) t = t / 4 * 4;
-        }
-        p->t_rows = (int)std::max(1L, std::min<long>(t, tile));
+        p->t_rows = o.t_rows > 0 ? std::min(o.t_rows, tile) : tile;
     }
     a.R = rows;
     a.C = cols;

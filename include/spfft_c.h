@@ -68,11 +68,12 @@ typedef struct spfft_opts {
     int row_transform; /* SPFFT_TRANSFORM_*: Bluestein, a direct length-cols FFT, or Rader     */
     int row_persist;   /* fused row kernel: 0 persistent (the resident workgroups loop over the
                           rows and load the twiddle tables once), 1 one workgroup per row group */
-    int t_rows;        /* four-step kernels: rows of the tile buffer T (tile x M complex64); the
-                          column pass fills a u_tile of rows once, then k_fft1 / k_conv / k_ifft1
-                          run over it t_rows at a time.  0: the rows whose T fits in
-                          SPFFT_FOUR_STEP_TILE_BYTES, so T stays in the 256 MB last-level cache
-                          between the three kernels instead of round-tripping through HBM */
+    int t_rows;        /* four-step kernels: rows of the tile buffer T (t_rows x M complex64);
+                          the column pass fills a u_tile of rows once, then k_fft1 / k_conv /
+                          k_ifft1 run over it t_rows at a time.  0: the whole tile.  (Sub-tiles
+                          small enough for T to stay in the 256 MB last-level cache measured
+                          slower on the 40421^2 matrix: 96-512 rows 18.6-21.5 ms against
+                          18.0 ms for one tile.) */
 } spfft_opts;
 
 /* Row-pass kernels.  FOUR_STEP: k_fft1 + k_conv + k_ifft1 with the tile buffer T
@@ -127,8 +128,6 @@ typedef struct spfft_plan_info {
     int t_rows;             /* four-step kernels: rows of T per sub-tile (0: fused)      */
 } spfft_plan_info;
 
-/* Tile-buffer budget of the automatic u_tile of four-step plans (bytes of T). */
-#define SPFFT_FOUR_STEP_TILE_BYTES (96u << 20)
 
 typedef struct spfft_plan_s* spfft_plan;
 

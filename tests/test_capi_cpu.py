@@ -329,9 +329,7 @@ def test_reference_opts_run_bluestein_at_the_reference_length():
 
 def test_automatic_tile():
     """u_tile 0: one tile of all R/2 + 1 computed rows (the column pass then pairs rows
-    for even R).  Four-step plans (c5's M = 82944) run k_fft1 / k_conv / k_ifft1 over it
-    t_rows at a time, the rows whose tile buffer fits in SPFFT_FOUR_STEP_TILE_BYTES (a
-    multiple of 4), so T stays in the last-level cache."""
+    for even R).  t_rows: rows of the four-step tile buffer T per sub-tile, 0 = the tile."""
     import cuspfft_amd as sp
     for n in (2310, 8219, 16706, 16384, 40421):
         info = sp.describe_plan(n, n)
@@ -340,9 +338,7 @@ def test_automatic_tile():
     assert sp.describe_plan(16706, 16706)["t_rows"] == 0  # on-chip row kernel: no T
     c5 = sp.describe_plan(40421, 40421)
     assert c5["row_kernel"] == sp.ROW_FOUR_STEP and c5["fft_len"] == 82944
-    t = c5["t_rows"]
-    assert t % 4 == 0 and t * 82944 * 8 <= 96 << 20 and (t + 4) * 82944 * 8 > 96 << 20
-    assert sp.describe_plan(40, 40421)["t_rows"] == 21  # every row of the tile
+    assert c5["t_rows"] == c5["u_tile"] == 20211
     assert sp.describe_plan(40421, 40421, t_rows=7)["t_rows"] == 7
     assert sp.describe_plan(40421, 40421, u_tile=5, t_rows=7)["t_rows"] == 5
     with pytest.raises(sp.SpfftError):
