@@ -129,6 +129,7 @@ def load():
         "spfft_plan_get_info": (_i, [_vp, ctypes.POINTER(PlanInfo)]),
         "spfft_plan_describe": (_i, [_i, _i, ctypes.POINTER(Opts), ctypes.POINTER(PlanInfo)]),
         "spfft_csc_build": (_i, [_vp, _vp, _vp, _i, _vp]),
+        "spfft_csc_build_noalloc": (_i, [_i, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
         "spfft_csc_get": (_i, [_vp, _vp, ctypes.POINTER(CscView)]),
         "spfft_rowpass": (_i, [_vp, _vp, ctypes.c_int64, _vp]),
         "spfft_execute": (_i, [_vp, _vp, _vp, _i, _vp, ctypes.c_int64, _vp]),

@@ -29,7 +29,7 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
           "-Wno-unused-result", "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
-LIB_SOURCES = ["csc_build.hip", "rowpass.hip", "rowpass_conv_a.hip", "rowpass_conv_b.hip",
+LIB_SOURCES = ["csc_build.hip", "csc_noalloc.hip", "rowpass.hip", "rowpass_conv_a.hip", "rowpass_conv_b.hip",
                "rowpass_conv_c.hip", "rowpass_conv_d.hip", "rowpass_conv_e.hip",
                "rowpass_fft1_a.hip", "rowpass_fft1_b.hip", "rowpass_fft1_c.hip", "rowpass_mfma.hip",
                "rowpass_fused_a.hip", "rowpass_fused_b.hip", "rowpass_fused_c.hip",

@@ -175,6 +175,28 @@ spfft_status spfft_plan_get_info(spfft_plan plan, spfft_plan_info* info);
 
 /* Stage 1 only: device COO (int32 rows, cols) -> compact CSC held by the plan. */
 spfft_status spfft_csc_build(spfft_plan plan, const int* d_row, const int* d_col, int nnz, void* stream);
+/* Stage 1 with no device allocation.  No counterpart in the reference: it is for
+ * callers that cannot call hipMalloc (code under hipGraph capture, tight serving
+ * loops, harnesses that forbid allocation).  The same kernels and output as
+ * spfft_csc_build + spfft_csc_get, into caller-owned device arrays:
+ *   d_sparse_cols  cols entries (uint16 when rows, cols <= 65536, else int32); the
+ *                  first d_info[1] are the active columns
+ *   d_col_ptr      cols + 1 int32 (the first d_info[1] + 1 are written)
+ *   d_row_idx      nnz entries (uint16 / int32 as d_sparse_cols)
+ *   d_col_start    cols + 1 int32 dense column starts
+ *   d_info         3 int32: [0] stored nonzeros, [1] active columns, [2] nonzero if an
+ *                  index was outside [0, rows) x [0, cols) (such pairs are dropped)
+ * The temporaries live in one file-scope device array of SPFFT_NOALLOC_SCRATCH_BYTES
+ * per GPU, held until the process ends: cols <= SPFFT_NOALLOC_MAX_COLS and
+ * nnz <= SPFFT_NOALLOC_MAX_NNZ, else SPFFT_ERR_INVALID_ARG.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other host threads, or
+ * other streams without an ordering between them) race.  Asynchronous on `stream`. */
+#define SPFFT_NOALLOC_MAX_COLS 65536
+#define SPFFT_NOALLOC_MAX_NNZ (1 << 20)
+#define SPFFT_NOALLOC_SCRATCH_BYTES (18u << 20)
+spfft_status spfft_csc_build_noalloc(int rows, int cols, const int* d_row, const int* d_col, int nnz,
+                                     void* d_sparse_cols, int* d_col_ptr, void* d_row_idx, int* d_col_start,
+                                     int* d_info, void* stream);
 /* Synchronises `stream`, then fills `view` (and checks the input range flag). */
 spfft_status spfft_csc_get(spfft_plan plan, void* stream, spfft_csc_view* view);
 /* Stages 2 + 3 on the CSC of the last spfft_csc_build: writes d_out (pitch in elements). */
