@@ -47,6 +47,8 @@ def parse():
                          "default, or the shipped selector's choice among the plain FP32 candidates")
     ap.add_argument("--rule", type=int, default=3)
     ap.add_argument("--tile", type=int, default=0, help="u-tile rows (0 = one tile of every computed row)")
+    ap.add_argument("--row-threads", type=int, default=0,
+                    help="on-chip row kernels: threads per workgroup of the shape (0 = the first listed)")
     ap.add_argument("--t-rows", type=int, default=0,
                     help="four-step kernels: rows of the tile buffer T per sub-tile (0 = automatic)")
     ap.add_argument("--m1", type=int, default=0)
@@ -259,11 +261,12 @@ def main():
     m = sp.config_matrix(args.config)
     explicit = (args.tile > 0 or args.fft_len or args.m1 or args.precision or args.rule != 3
                 or args.colpass_rows or args.colpass_table or args.row_kernel or args.colpass_pair
-                or args.row_transform or args.row_persist or args.t_rows)
+                or args.row_transform or args.row_persist or args.t_rows or args.row_threads)
     opts = dict(length_rule=args.rule, m1=args.m1, fft_len=args.fft_len, precision=args.precision,
                 row_kernel=args.row_kernel, colpass_rows=args.colpass_rows,
                 colpass_table=args.colpass_table, colpass_pair=args.colpass_pair,
-                row_transform=args.row_transform, row_persist=args.row_persist, t_rows=args.t_rows)
+                row_transform=args.row_transform, row_persist=args.row_persist, t_rows=args.t_rows,
+                row_threads=args.row_threads)
     if explicit or args.plan == "default":
         plan_source = "options" if explicit else "planner"
     else:
@@ -436,6 +439,7 @@ def main():
             "form": "bf16x3-mfma" if info["precision"] else "plain-fp32",
             "length_rule": args.rule, "fft_len": info["fft_len"], "m1": info["m1"],
             "m2": info["m2"], "u_tile": info["u_tile"], "t_rows": info["t_rows"],
+            "row_threads": info["row_threads"],
             "conv_rows": info["conv_rows"],
             "conv_radix": info["conv_radix"], "colpass_rows": info["colpass_rows"],
             "colpass_full_table": info["colpass_full_table"],

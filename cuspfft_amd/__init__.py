@@ -47,7 +47,7 @@ class Opts(ctypes.Structure):
                 ("colpass_table", ctypes.c_int), ("row_limit", ctypes.c_int),
                 ("row_kernel", ctypes.c_int), ("colpass_pair", ctypes.c_int),
                 ("row_transform", ctypes.c_int), ("row_persist", ctypes.c_int),
-                ("t_rows", ctypes.c_int)]
+                ("t_rows", ctypes.c_int), ("row_threads", ctypes.c_int)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -61,7 +61,7 @@ class PlanInfo(ctypes.Structure):
                 ("precision", ctypes.c_int), ("workspace_bytes", ctypes.c_size_t),
                 ("row_kernel", ctypes.c_int), ("colpass_paired", ctypes.c_int),
                 ("direct", ctypes.c_int), ("rader", ctypes.c_int), ("row_transform", ctypes.c_int),
-                ("t_rows", ctypes.c_int)]
+                ("t_rows", ctypes.c_int), ("row_threads", ctypes.c_int)]
 
     def __init__(self):
         super().__init__()
@@ -303,7 +303,7 @@ def _stream_handle(stream=None):
 
 def _opts(length_rule=SMOOTH_3, u_tile=0, fft_len=0, m1=0, conv_rows=0, precision=PREC_FP32,
           colpass_rows=0, colpass_table=0, row_limit=0, row_kernel=ROW_AUTO, colpass_pair=PAIR_AUTO,
-          row_transform=TRANSFORM_AUTO, row_persist=0, t_rows=0):
+          row_transform=TRANSFORM_AUTO, row_persist=0, t_rows=0, row_threads=0):
     o = Opts()
     load().spfft_opts_default(ctypes.byref(o))
     o.length_rule, o.u_tile, o.precision = length_rule, u_tile, precision
@@ -314,6 +314,7 @@ def _opts(length_rule=SMOOTH_3, u_tile=0, fft_len=0, m1=0, conv_rows=0, precisio
     o.row_transform = row_transform
     o.row_persist = row_persist
     o.t_rows = t_rows
+    o.row_threads = row_threads
     return o
 
 
@@ -339,11 +340,12 @@ class Plan:
                  fft_len: int = 0, m1: int = 0, conv_rows: int = 0, precision: int = PREC_FP32,
                  colpass_rows: int = 0, colpass_table: int = 0, row_limit: int = 0,
                  row_kernel: int = ROW_AUTO, colpass_pair: int = PAIR_AUTO,
-                 row_transform: int = TRANSFORM_AUTO, row_persist: int = 0, t_rows: int = 0):
+                 row_transform: int = TRANSFORM_AUTO, row_persist: int = 0, t_rows: int = 0,
+                 row_threads: int = 0):
         L = load()
         o = _opts(length_rule, u_tile, fft_len, m1, conv_rows, precision, colpass_rows,
                   colpass_table, row_limit, row_kernel, colpass_pair, row_transform, row_persist,
-                  t_rows)
+                  t_rows, row_threads)
         self._h = _vp()
         _check(L.spfft_plan_create(int(rows), int(cols), ctypes.byref(o), ctypes.byref(self._h)))
         inf = PlanInfo()

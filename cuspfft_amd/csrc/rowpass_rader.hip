@@ -183,6 +183,311 @@ __global__ __launch_bounds__(NT) void k_rowrader(RowPassArgs a) {
     }
 }
 
+
+// ---- precision form: the length-M1 DFTs on the matrix cores ------------------
+//
+// k_rowrader_mfma is k_rowrader with phase A's forward and phase C's inverse
+// length-M1 DFTs (M1 <= 32, e.g. 29: the codelet that costs O(M1^2) VALU
+// operations) done as complex DFT-matrix products on the matrix cores.  With
+// W_M1^{kn} = C[k][n] - i S[k][n] (zero rows and columns from M1 to 32), the DFT of
+// 32 columns x (32 x 32, n1 x column) is
+//     forward  Yr = C Xr + S Xi,   Yi = C Xi - S Xr
+//     inverse  Yr = C Xr - S Xi,   Yi = C Xi + S Xr
+// four real 32 x 32 x 32 products, each two v_mfma_f32_32x32x16_bf16 k-steps.  Every
+// operand is split into BF16 parts v = vh + vl (vh = RNE(v), vl = RNE(v - vh), about
+// 16 significant bits together) and A B ~= Ah Bh + Ah Bl + Al Bh accumulates in FP32
+// (the dropped Al Bl term is ~2^-18 relative).  C and S are split on the host from
+// FP64 values.  A wave takes 32 columns q = s M2 + n2 of both row groups at a time
+// (tile), lane l holding column q = 32 tile + l % 32 and, per k-step, the 8 entries
+// k = 16 ks + 8 (l / 32) + j; the product leaves output rows 8 g + 4 (l / 32) + i in
+// accumulator element 4 g + i (operand and accumulator layouts of gfx950).
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// The lane's A-operand fragments of C and S: [part hi / lo][k-step].
+struct DftFrags {
+    bf16x8 c[2][2], s[2][2];
+};
+
+// Table layout (host-built, spfft_plan.cpp): [matrix C, S][part][k-step][lane], 8 BF16 each.
+__device__ __forceinline__ void load_dft_frags(const bf16x8* tab, int lane, DftFrags& f) {
+#pragma unroll
+    for (int part = 0; part < 2; part++)
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            f.c[part][ks] = tab[((0 * 2 + part) * 2 + ks) * 64 + lane];
+            f.s[part][ks] = tab[((1 * 2 + part) * 2 + ks) * 64 + lane];
+        }
+}
+
+// x = hi + lo in BF16 (round to nearest even both times).
+__device__ __forceinline__ void split_bf16x8(const float* x, bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const __bf16 h = (__bf16)x[j];
+        hi[j] = h;
+        lo[j] = (__bf16)(x[j] - (float)h);
+    }
+}
+
+__device__ __forceinline__ bf16x8 neg_bf16x8(bf16x8 v) {
+    u32x4 u = __builtin_bit_cast(u32x4, v);
+    u ^= 0x80008000u;
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+// The lane's share of its column: BF16 parts of the real and imaginary entries.
+struct ColFrags {
+    bf16x8 r[2][2], i[2][2];  // [part][k-step]
+};
+
+__device__ __forceinline__ void split_col(const float2* x, ColFrags& c) {
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) {
+        float re[8], im[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            re[j] = x[8 * ks + j].x;
+            im[j] = x[8 * ks + j].y;
+        }
+        split_bf16x8(re, c.r[0][ks], c.r[1][ks]);
+        split_bf16x8(im, c.i[0][ks], c.i[1][ks]);
+    }
+}
+
+#define SPFFT_MFMA3(acc, A, B)                                                        \
+    do {                                                                              \
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0][ks], B[0][ks], acc, 0, 0, 0); \
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0][ks], B[1][ks], acc, 0, 0, 0); \
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[1][ks], B[0][ks], acc, 0, 0, 0); \
+    } while (0)
+
+// y = F x (forward) or conj(F) x (inverse) for the wave's 32 columns.
+template <bool INV>
+__device__ __forceinline__ void dft32_mfma(const DftFrags& f, const ColFrags& x, floatx16& yr, floatx16& yi) {
+#pragma unroll
+    for (int e = 0; e < 16; e++) yr[e] = yi[e] = 0.f;
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) {
+        // the S term's operand with its sign: forward -Xr (into Yi), inverse -Xi (into Yr)
+        bf16x8 n[2][2];
+        n[0][ks] = neg_bf16x8(INV ? x.i[0][ks] : x.r[0][ks]);
+        n[1][ks] = neg_bf16x8(INV ? x.i[1][ks] : x.r[1][ks]);
+        SPFFT_MFMA3(yr, f.c, x.r);
+        SPFFT_MFMA3(yi, f.c, x.i);
+        if (INV) {
+            SPFFT_MFMA3(yr, f.s, n);
+            SPFFT_MFMA3(yi, f.s, x.r);
+        } else {
+            SPFFT_MFMA3(yr, f.s, x.i);
+            SPFFT_MFMA3(yi, f.s, n);
+        }
+    }
+}
+#undef SPFFT_MFMA3
+
+template <int M1, int NT, int R1, int R2, int R3>
+__global__ __launch_bounds__(NT) void k_rowrader_mfma(RowPassArgs a) {
+    constexpr int M2 = R1 * R2 * R3;
+    constexpr int L = M1 * M2;  // p - 1
+    constexpr int KB = M1;
+    constexpr int G = 2 * KB;
+    constexpr int NW = NT / 64;
+    constexpr int NQ = 2 * M2;              // columns of both row groups
+    constexpr int NTILE = (NQ + 31) / 32;   // 32-column tiles
+    constexpr int TPW = (NTILE + NW - 1) / NW;  // tiles per wave (at most)
+    static_assert(M1 <= 32 && M1 > 16, "one 32-point DFT matrix, two k-steps");
+    static_assert(NT % 64 == 0, "whole waves");
+    constexpr int P = rader_pad(M1, NT, R1, R2, R3);
+    constexpr int LP = kRowPitch<M2, P>;
+    constexpr int TP = rader_tp(M1, NT, R1, R2, R3) ? 1 : 0;
+    constexpr int RA = fused_tp_radix(R2, R3), NA = TP ? fused_tp_a(R1, R2, R3) : 0;
+    constexpr int NBT = TP ? fused_tp_b(R1, R2, R3) : 0;
+    constexpr int NBUF = rader_buf(M1, M2, LP);
+    extern __shared__ float2 smem[];
+    float2* s_tw = smem;
+    float2* s_wm = s_tw + M2;
+    float2* s_ta = s_wm + rader_w2(L);
+    float2* s_tb = NBT ? s_ta + NA : s_ta;
+    float2* buf = s_ta + NA + NBT;
+    float2* s_wc = buf + NBUF;
+    float2* s_red = s_wc + rader_w2(L + 1);
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6, h = lane >> 5, jl = lane & 31;
+    const int p = a.rader_p;
+    const bio::rsrc_t r_bh = bio::make_rsrc(a.bhat, (uint32_t)L * 8u);
+    const bio::rsrc_t r_x = bio::make_rsrc(a.X0 + (size_t)blockIdx.x * a.ldx, (uint32_t)a.C * 8u);
+    const bf16x8* frag_tab = reinterpret_cast<const bf16x8*>(a.mfma_frag);
+
+    // phase A: the wave's tiles of pass-1 entries (rows n1 of its columns), the tables
+    float2 xa[TPW][16];
+#pragma unroll
+    for (int k = 0; k < TPW; k++) {
+        const int tile = wave + k * NW;
+        const int q = tile * 32 + jl;
+        const bool lv = tile < NTILE && q < NQ;
+        const int s = q >= M2 ? 1 : 0, n2 = q - s * M2;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int n1 = 16 * (e >> 3) + 8 * h + (e & 7);
+            xa[k][e] = bio::load2_stream(r_x, lv && n1 < M1 ? (s * L + n2) * 8 : bio::kOob, M2 * n1 * 8);
+        }
+    }
+    for (int i = t; i < M2; i += NT) s_tw[i] = a.tw2[i];
+    for (int i = t; i < 256 + (L + 255) / 256; i += NT) s_wm[i] = a.wm2[i];
+    for (int i = t; i < 256 + (p + 255) / 256; i += NT) s_wc[i] = a.wc2[i];
+    if constexpr (TP) {
+        constexpr int nsa = NA ? M2 / RA : 1, nsb = NBT ? M2 / R1 : 1;
+        for (int i = t; i < NA; i += NT) s_ta[i] = a.tw2[(i / nsa + 1) * (i % nsa)];
+        for (int i = t; i < NBT; i += NT) s_tb[i] = a.tw2[(i / nsb + 1) * (i % nsb)];
+    }
+    DftFrags f;
+    load_dft_frags(frag_tab, lane, f);
+    __syncthreads();
+    float2 psum0 = make_float2(0.f, 0.f), psum1 = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < TPW; k++) {
+        const int tile = wave + k * NW;
+        if (tile >= NTILE) break;  // wave-uniform
+        const int q = tile * 32 + jl;
+        const bool lv = q < NQ;
+        const int s = q >= M2 ? 1 : 0, n2 = q - s * M2;
+        ColFrags xc;
+        split_col(xa[k], xc);
+        floatx16 yr, yi;
+        dft32_mfma<false>(f, xc, yr, yi);
+        // sum_q a_s[q] = sum over the columns of output row 0 (lanes h = 0, element 0)
+        if (lv && h == 0) {
+            const float2 y0 = make_float2(yr[0], yi[0]);
+            if (s == 0) psum0 = cadd(psum0, y0);
+            else psum1 = cadd(psum1, y0);
+        }
+        // W_L^{k1 n2} for k1 = 4 (2 g + h) + i: a lookup per i and per g, a product each
+        float2 wi[4], wg[4];
+        wi[0] = make_float2(1.f, 0.f);
+#pragma unroll
+        for (int i = 1; i < 4; i++) wi[i] = wm_pow(s_wm, i * n2);
+#pragma unroll
+        for (int g = 0; g < 4; g++) wg[g] = wm_pow(s_wm, 4 * (2 * g + h) * n2);
+        float2* col = buf + s * KB * LP + lds_idx<P>(n2);
+#pragma unroll
+        for (int g = 0; g < 4; g++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int k1 = 8 * g + 4 * h + i;
+                if (lv && k1 < M1) {
+                    const float2 w = i == 0 ? wg[g] : cmul(wg[g], wi[i]);
+                    col[k1 * LP] = cmul(make_float2(yr[4 * g + i], yi[4 * g + i]), w);
+                }
+            }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        psum0.x += __shfl_xor(psum0.x, off, 64);
+        psum0.y += __shfl_xor(psum0.y, off, 64);
+        psum1.x += __shfl_xor(psum1.x, off, 64);
+        psum1.y += __shfl_xor(psum1.y, off, 64);
+    }
+    if (lane == 0) {
+        s_red[2 * wave] = psum0;
+        s_red[2 * wave + 1] = psum1;
+    }
+    __syncthreads();
+
+    // phase B: the convolution of both row groups with B^
+    fused_conv<M2, G, KB, NT, R1, R2, R3, P, TP>(r_bh, buf, s_tw, 0, s_ta, s_tb);
+
+    // phase C: the wave's columns back from LDS, times W_L^{-k1 n2}, split into BF16
+    // parts (held across the barrier: buf then takes E_0 | E_1 in natural order)
+    ColFrags xcs[TPW];
+#pragma unroll
+    for (int k = 0; k < TPW; k++) {
+        const int tile = wave + k * NW;
+        const int q = tile * 32 + jl;
+        const bool lv = tile < NTILE && q < NQ;
+        const int s = q >= M2 ? 1 : 0, n2 = q - s * M2;
+        float2 xc[16];
+        if (tile < NTILE) {
+            // rows k1 = 8 (2 ks + h) + j: W_L^{k1 n2} = W^{8 (2 ks + h) n2} W^{j n2}
+            float2 wj[8], wk[2];
+            wj[0] = make_float2(1.f, 0.f);
+#pragma unroll
+            for (int j = 1; j < 8; j++) wj[j] = wm_pow(s_wm, j * n2);
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++) wk[ks] = wm_pow(s_wm, 8 * (2 * ks + h) * n2);
+            const float2* col = buf + s * KB * LP + lds_idx<P>(n2);
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int ks = e >> 3, j = e & 7;
+                const int k1 = 16 * ks + 8 * h + j;
+                float2 v = make_float2(0.f, 0.f);
+                if (lv && k1 < M1) {
+                    const float2 w = j == 0 ? wk[ks] : cmul(wk[ks], wj[j]);
+                    v = cmulc(col[k1 * LP], w);
+                }
+                xc[e] = v;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 16; e++) xc[e] = make_float2(0.f, 0.f);
+        }
+        split_col(xc, xcs[k]);
+        __builtin_amdgcn_sched_barrier(0);  // one tile's loads in flight at a time
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TPW; k++) {
+        const int tile = wave + k * NW;
+        if (tile >= NTILE) break;  // wave-uniform
+        const int q = tile * 32 + jl;
+        const bool lv = q < NQ;
+        const int s = q >= M2 ? 1 : 0, n2 = q - s * M2;
+        const float2 x0 = bio::load2(r_x, lv ? (2 * L + s) * 8 : bio::kOob, 0);
+        floatx16 yr, yi;
+        dft32_mfma<true>(f, xcs[k], yr, yi);
+        // c_s[m], m = n2 + M2 m1 (m1 = 8 g + 4 h + i): E_s[g^-m] = x_s[0] + c_s[m]
+        float2* e = buf + s * p;
+#pragma unroll
+        for (int g = 0; g < 4; g++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int m1 = 8 * g + 4 * h + i;
+                if (lv && m1 < M1) e[a.rader_k[n2 + M2 * m1]] = cadd(x0, make_float2(yr[4 * g + i], yi[4 * g + i]));
+            }
+        if (lv && n2 == 0 && h == 0) {
+            float2 sum = x0;
+            for (int w = 0; w < NW; w++) sum = cadd(sum, s_red[2 * w + s]);
+            e[0] = sum;
+        }
+    }
+    __syncthreads();
+
+    // the radix-2 combination, row u and its mirror row R - u
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    const int u = a.u_base + (int)blockIdx.x;
+    const bool mirror = u > 0 && a.R - u > u;
+    f32x2* o = reinterpret_cast<f32x2*>(a.out + (size_t)u * a.P);
+    f32x2* om = reinterpret_cast<f32x2*>(mirror ? a.out_m + (size_t)(a.R - u) * a.P : a.out);
+    for (int k = t; k < p; k += NT) {
+        const float2 e0 = buf[k], e1 = cmul(buf[p + k], cmul(s_wc[256 + (k >> 8)], s_wc[k & 255]));
+        const f32x2 y0 = {e0.x + e1.x, e0.y + e1.y};
+        const f32x2 y1 = {e0.x - e1.x, e0.y - e1.y};
+        __builtin_nontemporal_store(y0, o + k);
+        if (k == 0) __builtin_nontemporal_store(y1, o + p);
+        if (mirror) {
+            const f32x2 c1 = {y1[0], -y1[1]};
+            __builtin_nontemporal_store(c1, om + (p - k));
+            if (k == 0) {
+                const f32x2 c0 = {y0[0], -y0[1]};
+                __builtin_nontemporal_store(c0, om);
+            }
+        }
+    }
+}
+
 }  // namespace
 
 // Instantiated shapes X(M1, NT, R1, R2, R3), L = p - 1 = M1 R1 R2 R3:
@@ -200,11 +505,16 @@ __global__ __launch_bounds__(NT) void k_rowrader(RowPassArgs a) {
 // Radix-29 LDS stages at 576+ threads (168 VGPRs), radix 12 / 24 stages, or M1 = 32 or
 // 48 spill 50-175 VGPRs.
 #define SPFFT_RADER_LIST(X) X(29, 704, 36, 8, 1) X(29, 192, 8, 9, 1) X(29, 64, 4, 8, 1)
+// The precision form (k_rowrader_mfma, M1 <= 32 padded to one 32-point DFT matrix).
+#define SPFFT_RADER_MFMA_LIST(X) X(29, 704, 36, 8, 1) X(29, 576, 36, 8, 1) X(29, 192, 8, 9, 1) X(29, 64, 4, 8, 1)
 
 bool rader_shape(int i, RaderShape* out) {
     static const RaderShape kShapes[] = {
-#define SPFFT_SHAPE(m1, nt, r1, r2, r3) {m1, nt, {r1, r2, r3}, rader_lds_bytes_of(m1, nt, r1, r2, r3)},
+#define SPFFT_SHAPE(m1, nt, r1, r2, r3) {m1, nt, {r1, r2, r3}, rader_lds_bytes_of(m1, nt, r1, r2, r3), 0},
         SPFFT_RADER_LIST(SPFFT_SHAPE)
+#undef SPFFT_SHAPE
+#define SPFFT_SHAPE(m1, nt, r1, r2, r3) {m1, nt, {r1, r2, r3}, rader_lds_bytes_of(m1, nt, r1, r2, r3), 1},
+        SPFFT_RADER_MFMA_LIST(SPFFT_SHAPE)
 #undef SPFFT_SHAPE
     };
     if (i < 0 || i >= (int)(sizeof(kShapes) / sizeof(kShapes[0]))) return false;
@@ -214,8 +524,19 @@ bool rader_shape(int i, RaderShape* out) {
 
 hipError_t launch_rader(const RowPassArgs& a, hipStream_t s, bool prepare_only) {
 #define SPFFT_CASE(m1, nt, r1, r2, r3)                                                                   \
-    if (a.M1 == m1 && a.fused_nt == nt && a.f2.radix[0] == r1 && (a.f2.nstages > 1 ? a.f2.radix[1] : 1) == r2 && \
-        (a.f2.nstages > 2 ? a.f2.radix[2] : 1) == r3) {                                                  \
+    if (a.precision == 1 && a.M1 == m1 && a.fused_nt == nt && a.f2.radix[0] == r1 &&                     \
+        (a.f2.nstages > 1 ? a.f2.radix[1] : 1) == r2 && (a.f2.nstages > 2 ? a.f2.radix[2] : 1) == r3) {   \
+        constexpr size_t lds = rader_lds_bytes_of(m1, nt, r1, r2, r3);                                   \
+        const void* k = reinterpret_cast<const void*>(&k_rowrader_mfma<m1, nt, r1, r2, r3>);              \
+        if (prepare_only) return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+        hipLaunchKernelGGL((k_rowrader_mfma<m1, nt, r1, r2, r3>), dim3(a.tile_rows), dim3(nt), lds, s, a); \
+        return hipGetLastError();                                                                        \
+    }
+    SPFFT_RADER_MFMA_LIST(SPFFT_CASE)
+#undef SPFFT_CASE
+#define SPFFT_CASE(m1, nt, r1, r2, r3)                                                                   \
+    if (a.precision == 0 && a.M1 == m1 && a.fused_nt == nt && a.f2.radix[0] == r1 &&                     \
+        (a.f2.nstages > 1 ? a.f2.radix[1] : 1) == r2 && (a.f2.nstages > 2 ? a.f2.radix[2] : 1) == r3) {   \
         constexpr size_t lds = rader_lds_bytes_of(m1, nt, r1, r2, r3);                                   \
         const void* k = reinterpret_cast<const void*>(&k_rowrader<m1, nt, r1, r2, r3>);                   \
         if (prepare_only) return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \

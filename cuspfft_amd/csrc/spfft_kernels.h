@@ -60,6 +60,7 @@ hipError_t launch_fused(const RowPassArgs& a, hipStream_t s, bool prepare_only);
 struct RaderShape {
     int m1, nt, r[3];
     size_t lds_bytes;
+    int mfma;  // 1: the precision form (k_rowrader_mfma), 0: FP32 codelets (k_rowrader)
 };
 bool rader_shape(int i, RaderShape* out);
 hipError_t launch_rader(const RowPassArgs& a, hipStream_t s, bool prepare_only);

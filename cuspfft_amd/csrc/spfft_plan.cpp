@@ -622,11 +622,12 @@ namespace {
 // Shape planning only (no device allocation): M, the M1 x M2 split, the
 // convolution kernel and its rows per workgroup.
 // Fused row-kernel shape for M, or -1.  lds_limit: the device's LDS per workgroup.
-int find_fused(int M, int m1_req, size_t lds_limit, spfft::FusedShape* out, bool one_batch = false) {
+int find_fused(int M, int m1_req, size_t lds_limit, spfft::FusedShape* out, bool one_batch = false,
+               int nt_req = 0) {
     spfft::FusedShape f;
     for (int i = 0; spfft::fused_shape(i, &f); i++) {
         if (f.m1 * f.r[0] * f.r[1] * f.r[2] != M || (m1_req && f.m1 != m1_req) || f.lds_bytes > lds_limit ||
-            (one_batch && f.nb != 1))
+            (one_batch && f.nb != 1) || (nt_req && f.nt != nt_req))
             continue;
         *out = f;
         return i;
@@ -636,10 +637,12 @@ int find_fused(int M, int m1_req, size_t lds_limit, spfft::FusedShape* out, bool
 
 // Rader shape for the convolution length L = p - 1 (as a fused shape: one batch, the
 // two row groups of k_rowrader), or -1.
-int find_rader(int L, int m1_req, size_t lds_limit, spfft::FusedShape* out) {
+int find_rader(int L, int m1_req, size_t lds_limit, bool mfma, spfft::FusedShape* out, int nt_req = 0) {
     spfft::RaderShape r;
     for (int i = 0; spfft::rader_shape(i, &r); i++) {
-        if (r.m1 * r.r[0] * r.r[1] * r.r[2] != L || (m1_req && r.m1 != m1_req) || r.lds_bytes > lds_limit) continue;
+        if (r.m1 * r.r[0] * r.r[1] * r.r[2] != L || (m1_req && r.m1 != m1_req) || r.lds_bytes > lds_limit ||
+            r.mfma != (mfma ? 1 : 0) || (nt_req && r.nt != nt_req))
+            continue;
         *out = spfft::FusedShape{r.m1, 1, 2, r.nt, {r.r[0], r.r[1], r.r[2]}, r.lds_bytes};
         return i;
     }
@@ -700,22 +703,25 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
         return fail(SPFFT_ERR_INVALID_ARG, "row_transform must be 0, 1, 2 or 3");
     spfft::FusedShape fd{};
     const bool chip_ok = !mfma && o.fft_len == 0 && o.conv_rows == 0 && o.row_kernel != SPFFT_ROW_FOUR_STEP;
-    const bool direct_ok = chip_ok && find_fused(cols, o.m1, lds_limit, &fd, true) >= 0;
+    if (o.row_threads < 0 || o.row_threads % 64) return fail(SPFFT_ERR_INVALID_ARG, "row_threads must be 0 or a multiple of 64");
+    const bool direct_ok = chip_ok && find_fused(cols, o.m1, lds_limit, &fd, true, o.row_threads) >= 0;
     if (o.row_transform == SPFFT_TRANSFORM_DIRECT && !direct_ok)
         return fail(SPFFT_ERR_UNSUPPORTED, "no direct transform for cols = " + std::to_string(cols) +
                                                " with these options");
     const bool direct =
         direct_ok && (o.row_transform == SPFFT_TRANSFORM_AUTO || o.row_transform == SPFFT_TRANSFORM_DIRECT);
     // Rader: cols = 2p, p an odd prime, with a shape for the convolution length p - 1
-    const bool rader_ok = chip_ok && cols % 2 == 0 && is_odd_prime(cols / 2) &&
-                          find_rader(cols / 2 - 1, o.m1, lds_limit, &fd) >= 0;
+    // (the precision form has Rader shapes of its own: k_rowrader_mfma)
+    const bool rader_chip_ok = o.fft_len == 0 && o.conv_rows == 0 && o.row_kernel != SPFFT_ROW_FOUR_STEP;
+    const bool rader_ok = rader_chip_ok && cols % 2 == 0 && is_odd_prime(cols / 2) &&
+                          find_rader(cols / 2 - 1, o.m1, lds_limit, mfma, &fd, o.row_threads) >= 0;
     if (o.row_transform == SPFFT_TRANSFORM_RADER && !rader_ok)
         return fail(SPFFT_ERR_UNSUPPORTED, "no Rader transform for cols = " + std::to_string(cols) +
                                                " with these options");
     const bool rader =
         !direct && rader_ok && (o.row_transform == SPFFT_TRANSFORM_AUTO || o.row_transform == SPFFT_TRANSFORM_RADER);
     if (direct) {
-        find_fused(cols, o.m1, lds_limit, &fd, true);
+        find_fused(cols, o.m1, lds_limit, &fd, true, o.row_threads);
         M = cols;
     }
     if (rader) M = cols / 2 - 1;
@@ -731,7 +737,7 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     spfft::FusedShape fs = fd;
     const bool fused_ok = direct || rader ||
                           (!mfma && o.conv_rows == 0 && o.row_kernel != SPFFT_ROW_FOUR_STEP &&
-                           find_fused(M, o.m1, lds_limit, &fs) >= 0);
+                           find_fused(M, o.m1, lds_limit, &fs, false, o.row_threads) >= 0);
     if (o.row_kernel == SPFFT_ROW_FUSED && !fused_ok)
         return fail(SPFFT_ERR_UNSUPPORTED, "no fused row-kernel shape for M = " + std::to_string(M) +
                                                (mfma ? " in the BF16x3 form" : ""));
@@ -846,6 +852,7 @@ void fill_info(const spfft_plan_s* p, spfft_plan_info* info) {
                           : p->base.rader ? SPFFT_TRANSFORM_RADER
                                           : SPFFT_TRANSFORM_BLUESTEIN;
     info->t_rows = p->t_rows;
+    info->row_threads = p->base.fused ? p->base.fused_nt : 0;
 }
 
 }  // namespace
@@ -982,7 +989,33 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
         SPFFT_HIP(upload(p->wm2, w2), "upload twiddles");
     }
     SPFFT_HIP(upload(p->bhat, bh), "upload B^");
-    if (a.precision == 1) {
+    if (a.precision == 1 && a.rader) {
+        // k_rowrader_mfma: each lane's A-operand fragments of the 32 x 32 matrices
+        // C[k][n] = cos(2 pi k n / M1), S[k][n] = sin(2 pi k n / M1) (k, n < M1; zero
+        // beyond), split into BF16 hi + lo parts from FP64: [C/S][hi/lo][k-step][lane][j],
+        // lane l holding row k = l % 32, column n = 16 k-step + 8 (l / 32) + j
+        const int L = p->M1;
+        std::vector<uint16_t> fr((size_t)2 * 2 * 2 * 64 * 8);
+        for (int d = 0; d < 2; d++)
+            for (int ks = 0; ks < 2; ks++)
+                for (int l = 0; l < 64; l++)
+                    for (int j = 0; j < 8; j++) {
+                        const int k = l % 32, n = 16 * ks + 8 * (l / 32) + j;
+                        double v = 0.0;
+                        if (k < L && n < L) {
+                            const cd w = unit_root((long long)k * n, L, +1);  // cos + i sin
+                            v = d == 0 ? w.real() : w.imag();
+                        }
+                        const float hi = bf16_round((float)v);
+                        const float lo = bf16_round((float)(v - (double)hi));
+                        uint32_t bh, bl;
+                        std::memcpy(&bh, &hi, 4);
+                        std::memcpy(&bl, &lo, 4);
+                        fr[((((size_t)d * 2 + 0) * 2 + ks) * 64 + l) * 8 + j] = (uint16_t)(bh >> 16);
+                        fr[((((size_t)d * 2 + 1) * 2 + ks) * 64 + l) * 8 + j] = (uint16_t)(bl >> 16);
+                    }
+        SPFFT_HIP(upload(p->mfma_frag, fr), "upload DFT fragments");
+    } else if (a.precision == 1) {
         // cos and sin of 2 pi i / M1 split into BF16 hi + lo parts from FP64
         const int L = p->M1;
         std::vector<float> t((size_t)4 * L);
@@ -1024,7 +1057,8 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
     a.twM = p->twM.as<float2>();
     a.wm2 = a.fused ? p->wm2.as<float2>() : nullptr;
     a.bhat = p->bhat.as<float2>();
-    a.mfma_dft = a.precision == 1 ? p->mfma_dft.as<float>() : nullptr;
+    a.mfma_dft = a.precision == 1 && !a.rader ? p->mfma_dft.as<float>() : nullptr;
+    a.mfma_frag = a.precision == 1 && a.rader ? p->mfma_frag.p : nullptr;
     a.T = p->T.as<float2>();
     a.X0 = p->X0.as<float2>();
     if (a.fused)
@@ -1032,7 +1066,7 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
                   "fused row kernel LDS limit");
     p->table_bytes = p->h.bytes + p->wr_lo.bytes + p->wr_hi.bytes + p->wr_full.bytes + p->tw2.bytes + p->twM.bytes +
                      p->wm2.bytes + p->rader_k.bytes + p->colperm.bytes + p->wc2.bytes +
-                     p->bhat.bytes + p->T.bytes + p->X0.bytes;
+                     p->bhat.bytes + p->mfma_dft.bytes + p->mfma_frag.bytes + p->T.bytes + p->X0.bytes;
     *out = guard.release();
     return SPFFT_OK;
 }

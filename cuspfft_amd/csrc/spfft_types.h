@@ -55,6 +55,8 @@ struct RowPassArgs {
     const float2* wm2;         // W_M^e = wm2[256 + (e >> 8)] * wm2[e & 255], e < M (k_rowfused)
     const float2* bhat;        // [k1*M2 + k2] = B^[k1 + M1 k2] / M
     const float* mfma_dft;     // precision form: cos/sin(2 pi i / M1) hi and lo BF16 parts, [4][M1]
+    const void* mfma_frag;     // precision form of k_rowrader: the lanes' BF16 A-operand fragments of
+                               //   the 32-point DFT matrices C and S, [C/S][hi/lo][k-step][lane][8]
     int precision;             // 0: FP32 codelets, 1: BF16x3 MFMA length-M1 sub-FFTs
     int fused;                 // 1: k_rowfused<M1, fused_nb, fused_rpw, f2.radix...> replaces
                                //    k_fft1 + k_conv + k_ifft1 (no tile buffer T)

@@ -74,6 +74,8 @@ typedef struct spfft_opts {
                           small enough for T to stay in the 256 MB last-level cache measured
                           slower on the 40421^2 matrix: 96-512 rows 18.6-21.5 ms against
                           18.0 ms for one tile.) */
+    int row_threads;   /* on-chip row kernels (fused, direct, Rader): 0 the first listed shape for
+                          the length; else only a shape of this many threads per workgroup */
 } spfft_opts;
 
 /* Row-pass kernels.  FOUR_STEP: k_fft1 + k_conv + k_ifft1 with the tile buffer T
@@ -126,6 +128,7 @@ typedef struct spfft_plan_info {
     int rader;              /* 1: Rader convolutions of length fft_len = cols/2 - 1      */
     int row_transform;      /* SPFFT_TRANSFORM_BLUESTEIN, _DIRECT or _RADER (resolved)  */
     int t_rows;             /* four-step kernels: rows of T per sub-tile (0: fused)      */
+    int row_threads;        /* on-chip row kernels: threads per workgroup (0: four-step) */
 } spfft_plan_info;
 
 

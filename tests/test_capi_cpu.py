@@ -131,7 +131,10 @@ def test_planner_rader_transform():
             assert sp.describe_plan(rows, cols, row_transform=sp.TRANSFORM_RADER)["rader"] == 1
             b = sp.describe_plan(rows, cols, row_transform=sp.TRANSFORM_BLUESTEIN)
             assert b["rader"] == 0 and b["fft_len"] >= 2 * cols - 1
-        for kw in (dict(precision=sp.PREC_BF16X3), dict(row_kernel=sp.ROW_FOUR_STEP),
+        # the precision form has Rader shapes of its own (k_rowrader_mfma, M1 = 29)
+        mf = sp.describe_plan(64, cols, precision=sp.PREC_BF16X3)
+        assert mf["rader"] == 1 and mf["precision"] == 1 and mf["m1"] == 29 and mf["fft_len"] == p - 1
+        for kw in (dict(row_kernel=sp.ROW_FOUR_STEP),
                    dict(fft_len=sp.next_3_smooth(2 * cols - 1)), dict(conv_rows=4)):
             assert sp.describe_plan(64, cols, **kw)["rader"] == 0
             with pytest.raises(sp.SpfftError):

@@ -672,6 +672,39 @@ def test_rader_transform(gpu, oracle_mod, cols, rows, nnz, tile):
         np.testing.assert_array_equal(host[:, : cols // 2 + 1].numpy(), X)
 
 
+@pytest.mark.parametrize("cols,rows,nnz,tile", RADER_CASES)
+def test_rader_mfma_transform(gpu, oracle_mod, cols, rows, nnz, tile):
+    """The precision form of the Rader plan (k_rowrader_mfma: the length-29 DFTs as
+    BF16x3 DFT-matrix products on the matrix cores) against the FP64 oracle."""
+    sp = gpu
+    rng = np.random.default_rng(cols + rows)
+    r, c = _random_coo(rng, rows, cols, nnz)
+    print(f"[rader-mfma] {rows}x{cols} nnz={len(r)} tile={tile}", flush=True)
+    plan, X = _run(sp, rows, cols, r, c, u_tile=tile, precision=sp.PREC_BF16X3)
+    assert plan.info["rader"] == 1 and plan.info["precision"] == 1 and plan.info["m1"] == 29
+    ref = oracle_mod.sparse_rfft2_f64(rows, cols, r, c, nthreads=8)
+    err = oracle_mod.rel_l2(X, ref)
+    print(f"  rel_l2 rader-mfma={err:.3e}")
+    assert err <= REL_L2_BF16X3
+    # run to run the same bits
+    _, X2 = _run(sp, rows, cols, r, c, u_tile=tile, precision=sp.PREC_BF16X3)
+    np.testing.assert_array_equal(X, X2)
+
+
+@pytest.mark.parametrize("c0", [0, 1, 2, 3, 8352, 16705])
+def test_rader_mfma_single_nonzero_closed_form(gpu, c0):
+    """A single nonzero in each column class through the precision form: a pure tone."""
+    sp = gpu
+    rows, cols, r0 = 40, 16706, 17
+    plan, X = _run(sp, rows, cols, np.array([r0], np.int32), np.array([c0], np.int32), u_tile=rows,
+                   precision=sp.PREC_BF16X3)
+    assert plan.info["rader"] == 1 and plan.info["precision"] == 1
+    u = np.arange(rows)[:, None]
+    v = np.arange(cols // 2 + 1)[None, :]
+    ref = np.exp(-2j * np.pi * (((u * r0) % rows) / rows + ((v * c0) % cols) / cols))
+    assert np.abs(X - ref).max() < 1e-4
+
+
 def test_rader_row_limit_and_tiles(gpu):
     """A Rader plan computing only rows u < 37 (and their mirror rows) in tiles of 16 rows
     matches the full one-tile run bit for bit on those rows (both with the unpaired column
