@@ -155,6 +155,12 @@ def run_batch_bench(args):
         opts = [dict(u_tile=m.rows) for m in mats]
     t_tune = time.perf_counter() - t_tune
     tuned = sp.Batch(shapes, opts=opts, nnz_hint=hint, streams=args.streams)
+    _, members = tuned.mode()
+    # the same plans, every matrix on its own launches (worker streams), as one graph
+    tuned.set_mode(sp.BATCH_PER_MATRIX)
+    dt_per, _ = run(tuned, graph=True)
+    print(f"[c4] per-matrix launches, graph: {dt_per * 1e3:.2f} ms per batch", file=sys.stderr, flush=True)
+    tuned.set_mode(sp.BATCH_AUTO)
     dt_tuned, arena = run(tuned, graph=True)
     print(f"[c4] {args.c4_plans} plans, graph: {dt_tuned * 1e3:.2f} ms per batch (planning {t_tune:.0f} s)",
           file=sys.stderr, flush=True)
@@ -184,7 +190,9 @@ def run_batch_bench(args):
         "config": {"workload": f"c4: batch of {len(mats)} seeded binary symmetric matrices, n "
                                "log-uniform in [500, 12000], average degree U[4, 30], half uniform "
                                "and half Chung-Lu (exponent U[2.0, 2.5])",
-                   "streams": args.streams, "plans": forms, "plan_source": args.c4_plans},
+                   "streams": args.streams, "plans": forms, "plan_source": args.c4_plans,
+                   "shared_launch_members": members},
+        "per_matrix_graph_ms_per_batch": round(dt_per * 1e3, 3),
         "untuned_eager_ms_per_batch": round(dt_base * 1e3, 3),
         "graph_speedup_over_untuned_eager": round(dt_base / dt_tuned, 3),
         "planning_s": round(t_tune, 1),

@@ -30,6 +30,8 @@ PAIR_AUTO, PAIR_OFF = 0, 1
 # row transform (spfft_opts.row_transform): Bluestein, a direct length-cols FFT, or
 # Rader's convolutions for cols = 2p (p an odd prime)
 TRANSFORM_AUTO, TRANSFORM_BLUESTEIN, TRANSFORM_DIRECT, TRANSFORM_RADER = 0, 1, 2, 3
+# batch execution (spfft_batch_set_mode): shared descriptor-driven launches, or per matrix
+BATCH_AUTO, BATCH_PER_MATRIX = 0, 1
 
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "HIP", 3: "ALLOC", 4: "UNSUPPORTED", 5: "BAD_INPUT"}
 
@@ -154,6 +156,8 @@ def load():
         "spfft_batch_execute": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "spfft_batch_capture": (_i, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
         "spfft_batch_check_input": (_i, [_vp, _vp]),
+        "spfft_batch_set_mode": (_i, [_vp, _i]),
+        "spfft_batch_get_mode": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
         "spfft_profile_read": (_i, [_vp, _vp, ctypes.POINTER(Profile)]),
         "spfft_next_3_smooth": (_i, [_i]),
         "spfft_next_7_smooth": (_i, [_i]),
@@ -554,6 +558,18 @@ class Batch:
 
     def check_input(self, stream=None):
         _check(load().spfft_batch_check_input(self._h, _stream_handle(stream)))
+
+    def set_mode(self, mode: int):
+        """BATCH_AUTO: the members (one-tile on-chip plans) run as shared launches driven
+        by a device table of their arguments; BATCH_PER_MATRIX: every matrix on its own
+        launches.  Graphs captured before a mode change must not be replayed after it."""
+        _check(load().spfft_batch_set_mode(self._h, int(mode)))
+
+    def mode(self):
+        """(mode, members): the mode and how many matrices run through the shared launches."""
+        m, n = ctypes.c_int(), ctypes.c_int()
+        _check(load().spfft_batch_get_mode(self._h, ctypes.byref(m), ctypes.byref(n)))
+        return m.value, n.value
 
     def __del__(self):
         h = getattr(self, "_h", None)

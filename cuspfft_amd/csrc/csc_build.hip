@@ -241,7 +241,8 @@ template <bool FINAL, bool PACKED>
 __global__ void k_radix_scatter(const int* __restrict__ keys_in, const int* __restrict__ vals_in,
                                 int nnz, int R, int shift, const int* __restrict__ offs, int nblk,
                                 int* __restrict__ keys_out, int* __restrict__ vals_out,
-                                void* __restrict__ final_out) {
+                                void* __restrict__ final_out, const int* __restrict__ col_rbase = nullptr,
+                                int C = 0) {
     __shared__ int s_cnt[kSortWaves][256];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < kSortWaves * 256; i += kSortThreads) (&s_cnt[0][0])[i] = 0;
@@ -285,6 +286,8 @@ __global__ void k_radix_scatter(const int* __restrict__ keys_in, const int* __re
             const int pos = base + rank;
             int val = vals_in[i];
             if (FINAL) {
+                // the batched build: rows of the key's (column's) own matrix
+                if (col_rbase) val = (uint32_t)key < (uint32_t)C ? val - col_rbase[key] : 0;
                 if ((uint32_t)val >= (uint32_t)R) val = 0;
                 if (PACKED) ((uint16_t*)final_out)[pos] = (uint16_t)val;
                 else ((int*)final_out)[pos] = val;
@@ -328,6 +331,63 @@ int csc_sort_passes(int cols) {
     return p;
 }
 
+namespace {
+
+// The batched build's input: element e of the concatenated COO (position coo_off[0] + e
+// of the caller's arrays) belongs to the last matrix i with coo_off[i] <= coo_off[0] + e; its (r, c) becomes (r + r_off, c + c_off) of the
+// block-diagonal matrix of all members (mat[i] = {R, C, r_off, c_off}).  A pair outside
+// its own matrix flags err[i] and gets column -1, so that the build skips it; so do the
+// pairs of matrices that are not members (mat[i].x < 0), with no flag.
+__global__ void k_batch_coo(const int* __restrict__ row, const int* __restrict__ col,
+                            const long long* __restrict__ coo_off, int count, const int4* __restrict__ mat,
+                            int* __restrict__ mrow, int* __restrict__ mcol, int* __restrict__ err, long long nnz) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    const long long ea = coo_off[0] + e;  // position in the caller's arrays
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (coo_off[mid] <= ea) lo = mid;
+        else hi = mid - 1;
+    }
+    const int4 m = mat[lo];
+    if (m.x < 0) {  // not a member of the batched build: skipped, no error
+        mrow[e] = 0;
+        mcol[e] = -1;
+        return;
+    }
+    const int r = row[ea], c = col[ea];
+    if (r < 0 || r >= m.x || c < 0 || c >= m.y) {
+        err[lo] = 1;
+        mrow[e] = 0;
+        mcol[e] = -1;
+        return;
+    }
+    mrow[e] = r + m.z;
+    mcol[e] = c + m.w;
+}
+
+__global__ void k_fill_int(int* __restrict__ p, long n, int v) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+hipError_t launch_fill_int(int* p, long n, int v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_int, dim3(cdiv(n, 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_coo(const int* row, const int* col, const long long* coo_off, int count, const int4* mat,
+                            int* mrow, int* mcol, int* err, long long nnz, hipStream_t s) {
+    if (nnz <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_coo, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, s, row, col, coo_off, count,
+                       mat, mrow, mcol, err, nnz);
+    return hipGetLastError();
+}
+
 hipError_t launch_csc_build(const CscBuildArgs& a, hipStream_t s) {
     const int R = a.R, C = a.C, nnz = a.nnz;
     // 1. counts + validation
@@ -357,7 +417,10 @@ hipError_t launch_csc_build(const CscBuildArgs& a, hipStream_t s) {
         e = run_scan<0, false>(a.hist, 256 * nblk, a.scan_tmp, nullptr, a.hist_off, nullptr, nullptr, s);
         if (e != hipSuccess) return e;
         if (fin) {
-            if (a.packed)
+            if (a.col_rbase)  // batched: uint16 rows of each column's own matrix (all of R <= 65536)
+                hipLaunchKernelGGL((k_radix_scatter<true, true>), dim3(nblk), dim3(kSortThreads), 0, s, kin, vin,
+                                   nnz, 65536, shift, a.hist_off, nblk, nullptr, nullptr, a.row_idx, a.col_rbase, C);
+            else if (a.packed)
                 hipLaunchKernelGGL((k_radix_scatter<true, true>), dim3(nblk), dim3(kSortThreads), 0, s, kin, vin,
                                    nnz, R, shift, a.hist_off, nblk, nullptr, nullptr, a.row_idx);
             else

@@ -313,7 +313,7 @@ __device__ __forceinline__ void fused_forward(bio::rsrc_t r_bh, float2* buf, con
 // its mirror row R - u (X[R-u][C - w] = conj(X[u][w])).  The column pass then runs
 // with h = 1.
 template <int M1, int NB, int RPW, int NT, int R1, int R2, int R3, bool DIRECT>
-__global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
+__device__ __forceinline__ void rowfused_body(const RowPassArgs& a, const int bx, const int gx) {
     constexpr int M2 = R1 * R2 * R3;
     constexpr int KB = M1 / NB;
     constexpr int G = RPW * KB;
@@ -520,10 +520,24 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
         // persistent launch: the resident workgroups take the row groups blockIdx.x,
         // blockIdx.x + gridDim.x, ... and keep the tables in LDS between them
         const int nblk = (a.tile_rows + RPW - 1) / RPW;
-        for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) row_group(blk * RPW, blk == (int)blockIdx.x);
+        for (int blk = bx; blk < nblk; blk += gx) row_group(blk * RPW, blk == bx);
     } else {
-        row_group(blockIdx.x * RPW, true);
+        row_group(bx * RPW, true);
     }
+}
+
+template <int M1, int NB, int RPW, int NT, int R1, int R2, int R3, bool DIRECT>
+__global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
+    rowfused_body<M1, NB, RPW, NT, R1, R2, R3, DIRECT>(a, blockIdx.x, gridDim.x);
+}
+
+// Batched form: work item (matrix descriptor, row group) per workgroup, one row group
+// each (no persistent loop).
+template <int M1, int NB, int RPW, int NT, int R1, int R2, int R3, bool DIRECT>
+__global__ __launch_bounds__(NT) void k_rowfused_batch(const RowPassArgs* __restrict__ descs,
+                                                       const int2* __restrict__ work) {
+    const int2 w = work[blockIdx.x];
+    rowfused_body<M1, NB, RPW, NT, R1, R2, R3, DIRECT>(descs[w.x], w.y, 0x40000000);
 }
 
 // Instantiated shapes: X(M1, NB, RPW, NT, R1, R2, R3), M = M1 * R1 * R2 * R3 (a
@@ -579,14 +593,28 @@ __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
 
 
 // Each returns hipErrorInvalidValue when the shape is not in its group.
-hipError_t launch_fused_group_a(const RowPassArgs& a, hipStream_t s, bool prepare_only);
-hipError_t launch_fused_group_b(const RowPassArgs& a, hipStream_t s, bool prepare_only);
-hipError_t launch_fused_group_c(const RowPassArgs& a, hipStream_t s, bool prepare_only);
-hipError_t launch_fused_group_d(const RowPassArgs& a, hipStream_t s, bool prepare_only);
-hipError_t launch_fused_group_e(const RowPassArgs& a, hipStream_t s, bool prepare_only);
-hipError_t launch_fused_group_f(const RowPassArgs& a, hipStream_t s, bool prepare_only);
-hipError_t launch_fused_group_g(const RowPassArgs& a, hipStream_t s, bool prepare_only);
-hipError_t launch_fused_group_h(const RowPassArgs& a, hipStream_t s, bool prepare_only);
+// (batch: work items of k_rowfused_batch instead of one matrix's launch)
+struct FusedBatch {
+    const RowPassArgs* descs;
+    const int2* work;
+    int n;
+};
+hipError_t launch_fused_group_a(const RowPassArgs& a, hipStream_t s, bool prepare_only,
+                                const FusedBatch* batch = nullptr);
+hipError_t launch_fused_group_b(const RowPassArgs& a, hipStream_t s, bool prepare_only,
+                                const FusedBatch* batch = nullptr);
+hipError_t launch_fused_group_c(const RowPassArgs& a, hipStream_t s, bool prepare_only,
+                                const FusedBatch* batch = nullptr);
+hipError_t launch_fused_group_d(const RowPassArgs& a, hipStream_t s, bool prepare_only,
+                                const FusedBatch* batch = nullptr);
+hipError_t launch_fused_group_e(const RowPassArgs& a, hipStream_t s, bool prepare_only,
+                                const FusedBatch* batch = nullptr);
+hipError_t launch_fused_group_f(const RowPassArgs& a, hipStream_t s, bool prepare_only,
+                                const FusedBatch* batch = nullptr);
+hipError_t launch_fused_group_g(const RowPassArgs& a, hipStream_t s, bool prepare_only,
+                                const FusedBatch* batch = nullptr);
+hipError_t launch_fused_group_h(const RowPassArgs& a, hipStream_t s, bool prepare_only,
+                                const FusedBatch* batch = nullptr);
 
 // (a.direct: the direct-transform instantiation, single-batch shapes only; a.fused_cus > 0:
 // persistent launch, as many workgroups as can be resident on that many CUs, each looping
@@ -601,9 +629,25 @@ hipError_t launch_fused_group_h(const RowPassArgs& a, hipStream_t s, bool prepar
             if constexpr (nb == 1) k = &k_rowfused<m1, nb, rpw, nt, r1, r2, r3, nb == 1>;           \
             else return hipErrorInvalidValue;                                                       \
         }                                                                                           \
-        if (prepare_only)                                                                           \
+        if (prepare_only) {                                                                         \
+            auto* kb = &k_rowfused_batch<m1, nb, rpw, nt, r1, r2, r3, false>;                       \
+            if (a.direct) {                                                                         \
+                if constexpr (nb == 1) kb = &k_rowfused_batch<m1, nb, rpw, nt, r1, r2, r3, nb == 1>;  \
+            }                                                                                       \
+            hipError_t e0 = hipFuncSetAttribute(reinterpret_cast<const void*>(kb),                  \
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+            if (e0 != hipSuccess) return e0;                                                        \
             return hipFuncSetAttribute(reinterpret_cast<const void*>(k),                            \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);       \
+        }                                                                                           \
+        if (batch) {                                                                                \
+            auto* kb = &k_rowfused_batch<m1, nb, rpw, nt, r1, r2, r3, false>;                       \
+            if (a.direct) {                                                                         \
+                if constexpr (nb == 1) kb = &k_rowfused_batch<m1, nb, rpw, nt, r1, r2, r3, nb == 1>;  \
+            }                                                                                       \
+            hipLaunchKernelGGL(kb, dim3(batch->n), dim3(nt), lds, s, batch->descs, batch->work);     \
+            return hipGetLastError();                                                               \
+        }                                                                                           \
         const int nblk = (a.tile_rows + rpw - 1) / rpw;                                             \
         int grid = nblk;                                                                            \
         if (a.fused_cus > 0 && fused_persistent(m1, nb, rpw, r1, r2, r3, a.direct != 0)) {         \

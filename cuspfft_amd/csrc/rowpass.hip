@@ -41,6 +41,11 @@ namespace {
 constexpr int kColThreads = 512;
 constexpr int kColWaves = kColThreads / 64;
 
+// Columns per k_colpass tile for U output rows per lane (64 U rows per tile).
+__host__ __device__ constexpr int colpass_tile_cols(int rows_per_lane) {
+    return rows_per_lane == 1 ? 64 : rows_per_lane == 2 ? 32 : rows_per_lane == 4 ? 16 : 8;  // 8: U = 8, 16
+}
+
 // ---------------------------------------------------------------------------
 // k_colsched: for every TC-column tile of k_colpass, shares its columns out to
 // the kColWaves waves by longest-processing-time first: power-of-two classes of
@@ -50,11 +55,11 @@ constexpr int kColWaves = kColThreads / 64;
 // Columns past C are not listed.  With a column order (colperm), position c of
 // the tile is matrix column colperm[c].
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_colsched(const int* __restrict__ colstart,
-                                                 const int* __restrict__ colperm, int C, int TC,
-                                                 int* __restrict__ sched, int* __restrict__ off) {
+__device__ __forceinline__ void colsched_body(const int* __restrict__ colstart, const int* __restrict__ colperm,
+                                              int C, int TC, int* __restrict__ sched, int* __restrict__ off,
+                                              const int bx) {
     const int lane = threadIdx.x;
-    const int c = blockIdx.x * TC + lane;
+    const int c = bx * TC + lane;
     const int mc = c < C && colperm ? colperm[c] : c;  // the matrix column at position c
     const int cost = lane < TC && c < C ? colstart[mc + 1] - colstart[mc] + 1 : 0;
     const int cls = cost > 0 ? 31 - __clz(cost) : -1;
@@ -83,13 +88,28 @@ __global__ __launch_bounds__(64) void k_colsched(const int* __restrict__ colstar
     int sw = 0;
 #pragma unroll
     for (int k = 0; k < kColWaves; k++) sw = mine_w == k ? start[k] : sw;
-    if (cls >= 0) sched[blockIdx.x * TC + sw + mine_k] = lane;
+    if (cls >= 0) sched[bx * TC + sw + mine_k] = lane;
     if (lane <= kColWaves) {
         int v = 0;
 #pragma unroll
         for (int k = 0; k <= kColWaves; k++) v = lane == k ? start[k] : v;
-        off[blockIdx.x * (kColWaves + 1) + lane] = v;
+        off[bx * (kColWaves + 1) + lane] = v;
     }
+}
+
+__global__ __launch_bounds__(64) void k_colsched(const int* __restrict__ colstart,
+                                                 const int* __restrict__ colperm, int C, int TC,
+                                                 int* __restrict__ sched, int* __restrict__ off) {
+    colsched_body(colstart, colperm, C, TC, sched, off, blockIdx.x);
+}
+
+// Batched form: work item (matrix descriptor, tile) per workgroup.
+__global__ __launch_bounds__(64) void k_colsched_batch(const RowPassArgs* __restrict__ descs,
+                                                       const int2* __restrict__ work) {
+    const int2 w = work[blockIdx.x];
+    const RowPassArgs& a = descs[w.x];
+    colsched_body(a.colstart, a.colperm, a.C, colpass_tile_cols(a.colpass_rows), const_cast<int*>(a.col_sched),
+                  const_cast<int*>(a.col_sched_off), w.y);
 }
 
 // acc[0] += w, acc[t] += w s^t (t < N): a lane's rows 64 apart share one phase
@@ -135,13 +155,13 @@ __device__ __forceinline__ void accumulate_rows(float2 (&acc)[N], float2 w, floa
 // phase reduction and lookup per nonzero now serves twice the rows.
 // ---------------------------------------------------------------------------
 template <bool PACKED, bool FULL, int U, int TC, bool PAIR>
-__global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
+__device__ __forceinline__ void colpass_body(const RowPassArgs& a, const int bx, const int by) {
     extern __shared__ float2 smem[];
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     constexpr int TS = TC + 1;        // tile row stride
-    const int c0 = blockIdx.x * TC;
-    const int u0l = blockIdx.y * 64 * U;
+    const int c0 = bx * TC;
+    const int u0l = by * 64 * U;
     const uint32_t R = (uint32_t)a.R;
     const int nlo = 1 << a.hi_shift;
     // the tables first, so that their offsets fit the LDS instructions' immediate field
@@ -179,8 +199,8 @@ __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
     };
     const uint32_t u0 = (uint32_t)(a.u_base + u0l);
     // this wave's columns (k_colsched): sched[blk][off[w] .. off[w+1])
-    const int* sched = a.col_sched + blockIdx.x * TC;
-    const int* off = a.col_sched_off + blockIdx.x * (kColWaves + 1);
+    const int* sched = a.col_sched + bx * TC;
+    const int* off = a.col_sched_off + bx * (kColWaves + 1);
     const int i0 = __builtin_amdgcn_readfirstlane(off[wid]), i1 = __builtin_amdgcn_readfirstlane(off[wid + 1]);
     for (int i = i0; i < i1; i++) {
         const int cc = __builtin_amdgcn_readfirstlane(sched[i]);  // column of the tile
@@ -284,6 +304,20 @@ __global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
     }
 }
 
+template <bool PACKED, bool FULL, int U, int TC, bool PAIR>
+__global__ __launch_bounds__(kColThreads) void k_colpass(RowPassArgs a) {
+    colpass_body<PACKED, FULL, U, TC, PAIR>(a, blockIdx.x, blockIdx.y);
+}
+
+// Batched form: one launch for the matrices of one instantiation; work item w =
+// (matrix descriptor, column tile, row block) per workgroup.
+template <bool PACKED, bool FULL, int U, int TC, bool PAIR>
+__global__ __launch_bounds__(kColThreads) void k_colpass_batch(const RowPassArgs* __restrict__ descs,
+                                                               const int4* __restrict__ work) {
+    const int4 w = work[blockIdx.x];
+    colpass_body<PACKED, FULL, U, TC, PAIR>(descs[w.x], w.y, w.z);
+}
+
 }  // namespace
 
 bool rowpass_supports_m1(int m1) {
@@ -292,15 +326,6 @@ bool rowpass_supports_m1(int m1) {
         SPFFT_M1_LIST(SPFFT_CASE)
 #undef SPFFT_CASE
         default: return false;
-    }
-}
-
-int colpass_tile_cols(int rows_per_lane) {
-    switch (rows_per_lane) {
-        case 1: return 64;
-        case 2: return 32;
-        case 4: return 16;
-        default: return 8;  // 8 or 16 rows per lane
     }
 }
 
@@ -338,19 +363,27 @@ hipError_t colpass_prepare(size_t lds_limit) {
     // raise the dynamic LDS limit of every k_colpass instantiation (the tile is
     // 2 * 64U * (TC + 1) floats: past 64 KB for some shapes)
     const void* ks[] = {
-#define SPFFT_COLPASS_KERNELS(u, tc)                                         \
-    reinterpret_cast<const void*>(&k_colpass<true, true, u, tc, false>),    \
-        reinterpret_cast<const void*>(&k_colpass<true, false, u, tc, false>),  \
-        reinterpret_cast<const void*>(&k_colpass<false, true, u, tc, false>),  \
-        reinterpret_cast<const void*>(&k_colpass<false, false, u, tc, false>),
-        SPFFT_COLPASS_SHAPES(SPFFT_COLPASS_KERNELS)
+#define SPFFT_COLPASS_KERNELS(K, u, tc)                                    \
+    reinterpret_cast<const void*>(&K<true, true, u, tc, false>),            \
+        reinterpret_cast<const void*>(&K<true, false, u, tc, false>),       \
+        reinterpret_cast<const void*>(&K<false, true, u, tc, false>),       \
+        reinterpret_cast<const void*>(&K<false, false, u, tc, false>),
+#define SPFFT_COLPASS_SINGLE(u, tc) SPFFT_COLPASS_KERNELS(k_colpass, u, tc)
+#define SPFFT_COLPASS_BATCH(u, tc) SPFFT_COLPASS_KERNELS(k_colpass_batch, u, tc)
+        SPFFT_COLPASS_SHAPES(SPFFT_COLPASS_SINGLE) SPFFT_COLPASS_SHAPES(SPFFT_COLPASS_BATCH)
+#undef SPFFT_COLPASS_SINGLE
+#undef SPFFT_COLPASS_BATCH
 #undef SPFFT_COLPASS_KERNELS
-#define SPFFT_COLPASS_KERNELS(u, tc)                                         \
-    reinterpret_cast<const void*>(&k_colpass<true, true, u / 2, tc, true>),     \
-        reinterpret_cast<const void*>(&k_colpass<true, false, u / 2, tc, true>),  \
-        reinterpret_cast<const void*>(&k_colpass<false, true, u / 2, tc, true>),  \
-        reinterpret_cast<const void*>(&k_colpass<false, false, u / 2, tc, true>),
-        SPFFT_COLPASS_PAIR_SHAPES(SPFFT_COLPASS_KERNELS)
+#define SPFFT_COLPASS_KERNELS(K, u, tc)                                    \
+    reinterpret_cast<const void*>(&K<true, true, u / 2, tc, true>),         \
+        reinterpret_cast<const void*>(&K<true, false, u / 2, tc, true>),    \
+        reinterpret_cast<const void*>(&K<false, true, u / 2, tc, true>),    \
+        reinterpret_cast<const void*>(&K<false, false, u / 2, tc, true>),
+#define SPFFT_COLPASS_SINGLE(u, tc) SPFFT_COLPASS_KERNELS(k_colpass, u, tc)
+#define SPFFT_COLPASS_BATCH(u, tc) SPFFT_COLPASS_KERNELS(k_colpass_batch, u, tc)
+        SPFFT_COLPASS_PAIR_SHAPES(SPFFT_COLPASS_SINGLE) SPFFT_COLPASS_PAIR_SHAPES(SPFFT_COLPASS_BATCH)
+#undef SPFFT_COLPASS_SINGLE
+#undef SPFFT_COLPASS_BATCH
 #undef SPFFT_COLPASS_KERNELS
     };
     for (const void* k : ks) {
@@ -359,6 +392,58 @@ hipError_t colpass_prepare(size_t lds_limit) {
     }
     return hipSuccess;
 }
+
+hipError_t launch_colsched_batch(const RowPassArgs* descs, const int2* work, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_colsched_batch, dim3(n), dim3(64), 0, s, descs, work);
+    return hipGetLastError();
+}
+
+template <bool PACKED, bool FULL>
+static hipError_t launch_colpass_batch_u(const RowPassArgs& rep, const RowPassArgs* descs, const int4* work, int n,
+                                         size_t lds, hipStream_t s) {
+    const int U = rep.colpass_rows;
+    const int TC = colpass_tile_cols(U);
+    if (rep.pair_half) {
+#define SPFFT_COLPASS_LAUNCH(u, tc)                                                                          \
+    if (U == u && TC == tc) {                                                                                \
+        hipLaunchKernelGGL((k_colpass_batch<PACKED, FULL, u / 2, tc, true>), dim3(n), dim3(kColThreads), lds, s, \
+                           descs, work);                                                                     \
+        return hipGetLastError();                                                                            \
+    }
+        SPFFT_COLPASS_PAIR_SHAPES(SPFFT_COLPASS_LAUNCH)
+#undef SPFFT_COLPASS_LAUNCH
+        return hipErrorInvalidValue;
+    }
+#define SPFFT_COLPASS_LAUNCH(u, tc)                                                                             \
+    if (U == u && TC == tc) {                                                                                   \
+        hipLaunchKernelGGL((k_colpass_batch<PACKED, FULL, u, tc, false>), dim3(n), dim3(kColThreads), lds, s, descs, \
+                           work);                                                                               \
+        return hipGetLastError();                                                                               \
+    }
+    SPFFT_COLPASS_SHAPES(SPFFT_COLPASS_LAUNCH)
+#undef SPFFT_COLPASS_LAUNCH
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_colpass_batch(const RowPassArgs& rep, bool packed, const RowPassArgs* descs, const int4* work,
+                                int n, size_t lds, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (packed) return rep.wr_full ? launch_colpass_batch_u<true, true>(rep, descs, work, n, lds, s)
+                                   : launch_colpass_batch_u<true, false>(rep, descs, work, n, lds, s);
+    return rep.wr_full ? launch_colpass_batch_u<false, true>(rep, descs, work, n, lds, s)
+                       : launch_colpass_batch_u<false, false>(rep, descs, work, n, lds, s);
+}
+
+void colpass_grid(const RowPassArgs& a, int* tiles_x, int* tiles_y) {
+    const int U = a.colpass_rows;
+    const int TC = colpass_tile_cols(U);
+    *tiles_x = (a.C + TC - 1) / TC;
+    *tiles_y = a.pair_half ? ((a.R >> 2) + 1 + 64 * (U / 2) - 1) / (64 * (U / 2))
+                           : (a.tile_rows + 64 * U - 1) / (64 * U);
+}
+
+int colpass_cols_per_tile(int rows_per_lane) { return colpass_tile_cols(rows_per_lane); }
 
 template <bool PACKED, bool FULL>
 static void launch_colpass_u(const RowPassArgs& a, hipStream_t s) {

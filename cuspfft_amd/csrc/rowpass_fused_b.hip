@@ -3,7 +3,8 @@
 
 namespace spfft {
 
-hipError_t launch_fused_group_b(const RowPassArgs& a, hipStream_t s, bool prepare_only) {
+hipError_t launch_fused_group_b(const RowPassArgs& a, hipStream_t s, bool prepare_only,
+                                const FusedBatch* batch) {
     SPFFT_FUSED_GROUP_B(SPFFT_FUSED_LAUNCH_CASE)
     return hipErrorInvalidValue;
 }
@@ -20,14 +21,25 @@ bool fused_shape(int i, FusedShape* out) {
     return true;
 }
 
-hipError_t launch_fused(const RowPassArgs& a, hipStream_t s, bool prepare_only) {
-    hipError_t (*const groups[])(const RowPassArgs&, hipStream_t, bool) = {
+static hipError_t launch_fused_any(const RowPassArgs& a, hipStream_t s, bool prepare_only, const FusedBatch* batch) {
+    hipError_t (*const groups[])(const RowPassArgs&, hipStream_t, bool, const FusedBatch*) = {
         launch_fused_group_a, launch_fused_group_b, launch_fused_group_c, launch_fused_group_d,
         launch_fused_group_e, launch_fused_group_f, launch_fused_group_g, launch_fused_group_h};
     hipError_t e = hipErrorInvalidValue;
     for (auto* g : groups)
-        if ((e = g(a, s, prepare_only)) != hipErrorInvalidValue) break;
+        if ((e = g(a, s, prepare_only, batch)) != hipErrorInvalidValue) break;
     return e;
+}
+
+hipError_t launch_fused(const RowPassArgs& a, hipStream_t s, bool prepare_only) {
+    return launch_fused_any(a, s, prepare_only, nullptr);
+}
+
+hipError_t launch_fused_batch(const RowPassArgs& rep, const RowPassArgs* descs, const int2* work, int n,
+                              hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const FusedBatch b{descs, work, n};
+    return launch_fused_any(rep, s, false, &b);
 }
 
 }  // namespace spfft

@@ -3,7 +3,8 @@
 
 namespace spfft {
 
-hipError_t launch_fused_group_d(const RowPassArgs& a, hipStream_t s, bool prepare_only) {
+hipError_t launch_fused_group_d(const RowPassArgs& a, hipStream_t s, bool prepare_only,
+                                const FusedBatch* batch) {
     SPFFT_FUSED_GROUP_D(SPFFT_FUSED_LAUNCH_CASE)
     return hipErrorInvalidValue;
 }

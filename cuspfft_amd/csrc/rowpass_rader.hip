@@ -60,7 +60,7 @@ constexpr size_t rader_lds_bytes_of(int m1, int nt, int r1, int r2, int r3) {
 }
 
 template <int M1, int NT, int R1, int R2, int R3>
-__global__ __launch_bounds__(NT) void k_rowrader(RowPassArgs a) {
+__device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx) {
     constexpr int M2 = R1 * R2 * R3;
     constexpr int L = M1 * M2;  // p - 1
     constexpr int KB = M1;      // one batch: the M1 rows k1 of each row group
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(NT) void k_rowrader(RowPassArgs a) {
     const bio::rsrc_t r_bh = bio::make_rsrc(a.bhat, (uint32_t)L * 8u);
 
     // phase A: a_s[n2 + M2 n1] (n1 < M1) and x_s[0]; the tables while they load
-    const bio::rsrc_t r_x = bio::make_rsrc(a.X0 + (size_t)blockIdx.x * a.ldx, (uint32_t)a.C * 8u);
+    const bio::rsrc_t r_x = bio::make_rsrc(a.X0 + (size_t)bx * a.ldx, (uint32_t)a.C * 8u);
     float2 v[M1];
     float2 x0 = make_float2(0.f, 0.f);
     if (live) {
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(NT) void k_rowrader(RowPassArgs a) {
 
     // the radix-2 combination, row u and its mirror row R - u
     typedef float f32x2 __attribute__((ext_vector_type(2)));
-    const int u = a.u_base + (int)blockIdx.x;
+    const int u = a.u_base + bx;
     const bool mirror = u > 0 && a.R - u > u;
     f32x2* o = reinterpret_cast<f32x2*>(a.out + (size_t)u * a.P);
     f32x2* om = reinterpret_cast<f32x2*>(mirror ? a.out_m + (size_t)(a.R - u) * a.P : a.out);
@@ -183,6 +183,19 @@ __global__ __launch_bounds__(NT) void k_rowrader(RowPassArgs a) {
     }
 }
 
+
+template <int M1, int NT, int R1, int R2, int R3>
+__global__ __launch_bounds__(NT) void k_rowrader(RowPassArgs a) {
+    rowrader_body<M1, NT, R1, R2, R3>(a, blockIdx.x);
+}
+
+// Batched form: work item (matrix descriptor, row of its tile) per workgroup.
+template <int M1, int NT, int R1, int R2, int R3>
+__global__ __launch_bounds__(NT) void k_rowrader_batch(const RowPassArgs* __restrict__ descs,
+                                                       const int2* __restrict__ work) {
+    const int2 w = work[blockIdx.x];
+    rowrader_body<M1, NT, R1, R2, R3>(descs[w.x], w.y);
+}
 
 // ---- precision form: the length-M1 DFTs on the matrix cores ------------------
 //
@@ -289,7 +302,7 @@ __device__ __forceinline__ void dft32_mfma(const DftFrags& f, const ColFrags& x,
 #undef SPFFT_MFMA3
 
 template <int M1, int NT, int R1, int R2, int R3>
-__global__ __launch_bounds__(NT) void k_rowrader_mfma(RowPassArgs a) {
+__device__ __forceinline__ void rowrader_mfma_body(const RowPassArgs& a, const int bx) {
     constexpr int M2 = R1 * R2 * R3;
     constexpr int L = M1 * M2;  // p - 1
     constexpr int KB = M1;
@@ -318,7 +331,7 @@ __global__ __launch_bounds__(NT) void k_rowrader_mfma(RowPassArgs a) {
     const int lane = t & 63, wave = t >> 6, h = lane >> 5, jl = lane & 31;
     const int p = a.rader_p;
     const bio::rsrc_t r_bh = bio::make_rsrc(a.bhat, (uint32_t)L * 8u);
-    const bio::rsrc_t r_x = bio::make_rsrc(a.X0 + (size_t)blockIdx.x * a.ldx, (uint32_t)a.C * 8u);
+    const bio::rsrc_t r_x = bio::make_rsrc(a.X0 + (size_t)bx * a.ldx, (uint32_t)a.C * 8u);
     const bf16x8* frag_tab = reinterpret_cast<const bf16x8*>(a.mfma_frag);
 
     // phase A: the wave's tiles of pass-1 entries (rows n1 of its columns), the tables
@@ -467,7 +480,7 @@ __global__ __launch_bounds__(NT) void k_rowrader_mfma(RowPassArgs a) {
 
     // the radix-2 combination, row u and its mirror row R - u
     typedef float f32x2 __attribute__((ext_vector_type(2)));
-    const int u = a.u_base + (int)blockIdx.x;
+    const int u = a.u_base + bx;
     const bool mirror = u > 0 && a.R - u > u;
     f32x2* o = reinterpret_cast<f32x2*>(a.out + (size_t)u * a.P);
     f32x2* om = reinterpret_cast<f32x2*>(mirror ? a.out_m + (size_t)(a.R - u) * a.P : a.out);
@@ -486,6 +499,18 @@ __global__ __launch_bounds__(NT) void k_rowrader_mfma(RowPassArgs a) {
             }
         }
     }
+}
+
+template <int M1, int NT, int R1, int R2, int R3>
+__global__ __launch_bounds__(NT) void k_rowrader_mfma(RowPassArgs a) {
+    rowrader_mfma_body<M1, NT, R1, R2, R3>(a, blockIdx.x);
+}
+
+template <int M1, int NT, int R1, int R2, int R3>
+__global__ __launch_bounds__(NT) void k_rowrader_mfma_batch(const RowPassArgs* __restrict__ descs,
+                                                            const int2* __restrict__ work) {
+    const int2 w = work[blockIdx.x];
+    rowrader_mfma_body<M1, NT, R1, R2, R3>(descs[w.x], w.y);
 }
 
 }  // namespace
@@ -522,13 +547,39 @@ bool rader_shape(int i, RaderShape* out) {
     return true;
 }
 
+hipError_t launch_rader_batch(const RowPassArgs& a, const RowPassArgs* descs, const int2* work, int n,
+                              hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+#define SPFFT_CASE(K, prec, m1, nt, r1, r2, r3)                                                          \
+    if (a.precision == prec && a.M1 == m1 && a.fused_nt == nt && a.f2.radix[0] == r1 &&                  \
+        (a.f2.nstages > 1 ? a.f2.radix[1] : 1) == r2 && (a.f2.nstages > 2 ? a.f2.radix[2] : 1) == r3) {   \
+        hipLaunchKernelGGL((K<m1, nt, r1, r2, r3>), dim3(n), dim3(nt), rader_lds_bytes_of(m1, nt, r1, r2, r3), \
+                           s, descs, work);                                                              \
+        return hipGetLastError();                                                                        \
+    }
+#define SPFFT_CASE_MFMA(m1, nt, r1, r2, r3) SPFFT_CASE(k_rowrader_mfma_batch, 1, m1, nt, r1, r2, r3)
+#define SPFFT_CASE_FP32(m1, nt, r1, r2, r3) SPFFT_CASE(k_rowrader_batch, 0, m1, nt, r1, r2, r3)
+    SPFFT_RADER_MFMA_LIST(SPFFT_CASE_MFMA)
+    SPFFT_RADER_LIST(SPFFT_CASE_FP32)
+#undef SPFFT_CASE_MFMA
+#undef SPFFT_CASE_FP32
+#undef SPFFT_CASE
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_rader(const RowPassArgs& a, hipStream_t s, bool prepare_only) {
 #define SPFFT_CASE(m1, nt, r1, r2, r3)                                                                   \
     if (a.precision == 1 && a.M1 == m1 && a.fused_nt == nt && a.f2.radix[0] == r1 &&                     \
         (a.f2.nstages > 1 ? a.f2.radix[1] : 1) == r2 && (a.f2.nstages > 2 ? a.f2.radix[2] : 1) == r3) {   \
         constexpr size_t lds = rader_lds_bytes_of(m1, nt, r1, r2, r3);                                   \
         const void* k = reinterpret_cast<const void*>(&k_rowrader_mfma<m1, nt, r1, r2, r3>);              \
-        if (prepare_only) return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+        if (prepare_only) {                                                                              \
+            const hipError_t e0 = hipFuncSetAttribute(                                                   \
+                reinterpret_cast<const void*>(&k_rowrader_mfma_batch<m1, nt, r1, r2, r3>),               \
+                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                   \
+            if (e0 != hipSuccess) return e0;                                                             \
+            return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);         \
+        }                                                                                                \
         hipLaunchKernelGGL((k_rowrader_mfma<m1, nt, r1, r2, r3>), dim3(a.tile_rows), dim3(nt), lds, s, a); \
         return hipGetLastError();                                                                        \
     }
@@ -539,7 +590,13 @@ hipError_t launch_rader(const RowPassArgs& a, hipStream_t s, bool prepare_only) 
         (a.f2.nstages > 1 ? a.f2.radix[1] : 1) == r2 && (a.f2.nstages > 2 ? a.f2.radix[2] : 1) == r3) {   \
         constexpr size_t lds = rader_lds_bytes_of(m1, nt, r1, r2, r3);                                   \
         const void* k = reinterpret_cast<const void*>(&k_rowrader<m1, nt, r1, r2, r3>);                   \
-        if (prepare_only) return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+        if (prepare_only) {                                                                              \
+            const hipError_t e0 = hipFuncSetAttribute(                                                   \
+                reinterpret_cast<const void*>(&k_rowrader_batch<m1, nt, r1, r2, r3>),                    \
+                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                   \
+            if (e0 != hipSuccess) return e0;                                                             \
+            return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);         \
+        }                                                                                                \
         hipLaunchKernelGGL((k_rowrader<m1, nt, r1, r2, r3>), dim3(a.tile_rows), dim3(nt), lds, s, a);   \
         return hipGetLastError();                                                                        \
     }

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "../../include/spfft_c.h"
+#include "spfft_kernels.h"
 #include "spfft_plan_internal.h"
 
 using namespace spfft_internal;
@@ -35,6 +36,7 @@ using namespace spfft_internal;
 struct spfft_graph_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<DevBuf> keep;  // device buffers the captured work reads (descriptor copies)
     ~spfft_graph_s() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -53,7 +55,46 @@ struct spfft_batch_s {
     std::vector<hipEvent_t> join;  // one per slot
     hipEvent_t fork = nullptr;
     std::vector<DevBuf> ws_T, ws_X0;
+    // descriptor-driven form (see run_batch_desc)
+    struct Group {
+        int rep = 0;        // member whose descriptor picks the kernel
+        size_t lds = 0;     // dynamic LDS of the launch (column pass: the group's largest)
+        int n = 0;          // work items
+        DevBuf work;        // int4 (column pass) or int2 (row kernels) items
+    };
+    struct Step {  // a few members of one lane: their column passes, then their row kernels
+        int lane = 0;
+        std::vector<Group> colpass, rows;
+    };
+    struct Chunk {
+        std::vector<Step> steps;  // in enqueue order; a lane's steps run in order on its stream
+    };
+    std::vector<hipStream_t> lane_streams;
+    std::vector<hipEvent_t> lane_done;
+    hipEvent_t lane_fork = nullptr;
+    int mode = 0;                       // SPFFT_BATCH_*
+    int req_streams = 4;                // worker streams asked for at creation
+    std::vector<double> work;           // per-matrix work estimates (longest first)
+    std::vector<int64_t> coo_off_h;     // the COO offsets coo_off_d holds
+    bool desc_on = false;               // some members run through the descriptors
+    std::vector<int> members, fallback; // descriptor members; matrices on their own launches
+    std::vector<int> r_off, c_off;      // block-diagonal offsets of the members (-1: not a member)
+    std::vector<int64_t> x0_off;        // member's X0 offset in the arena (elements)
+    int r_total = 0, c_total = 0;
+    int64_t nnz_cap = -1;
+    DevBuf mat, merr, coo_off_d, mrow, mcol;
+    DevBuf counts, colstart, sparse_cols, col_ptr, totals, err, scan_tmp, hist, hist_off, keys0, keys1, vals0,
+        vals1, row_idx, col_rbase, sched, sched_off;
+    DevBuf x0_arena, colsched_work;
+    int colsched_n = 0;
+    std::vector<Chunk> chunks;
+    std::vector<spfft::RowPassArgs> h_desc;  // members' descriptors (output pointers patched per call)
+    DevBuf d_desc;
+    void* desc_out = nullptr;                // the output arena d_desc was written for
     ~spfft_batch_s() {
+        for (hipStream_t s : lane_streams) (void)hipStreamDestroy(s);
+        for (hipEvent_t e : lane_done) (void)hipEventDestroy(e);
+        if (lane_fork) (void)hipEventDestroy(lane_fork);
         for (hipStream_t s : streams) (void)hipStreamDestroy(s);
         for (hipEvent_t e : join) (void)hipEventDestroy(e);
         if (fork) (void)hipEventDestroy(fork);
@@ -69,26 +110,392 @@ double work_estimate(const spfft_plan_s& p, int nnz) {
     return rows * (double)nnz + 4.0 * rows * (double)p.M * std::log2((double)std::max(p.M, 2));
 }
 
+// ---- descriptor-driven form -------------------------------------------------
+//
+// The members (matrices whose row pass is one on-chip kernel launch: fused, direct or
+// Rader, one tile) run as one short launch sequence for the whole batch:
+//   1. one CSC build of the block-diagonal matrix of all members (their COO offset by
+//      the members' first rows and columns; columns padded to multiples of 64), whose
+//      final pass stores each row relative to its own matrix in uint16: every member's
+//      CSC is a slice of it (column starts at its first column, rows as they are);
+//   2. one column-schedule launch over every member's column tiles;
+//   3. per group of members with the same column-pass kernel, one launch whose
+//      workgroups take (member, column tile, row block) items from a work list;
+//   4. per group of members with the same row kernel shape, one launch of (member,
+//      row group) items.
+// Each workgroup reads its member's RowPassArgs from a device table (the descriptors).
+// Members are ordered longest first, so the big matrices' work is dispatched first.
+// Steps 3 and 4 run in chunks whose pass-1 rows fit kBatchX0Budget bytes, each chunk on
+// concurrent lane streams forked from (and joined back to) the launch stream.  The other
+// matrices (four-step row pass, several tiles, a row limit) keep their own launches on
+// worker streams, concurrently.
+constexpr size_t kBatchX0Budget = (size_t)24 << 30;
+// Members per step of a lane.  A chunk's members are shared out longest first to the
+// least loaded of the lane streams (as many as the batch's worker streams); each lane
+// runs its members in steps of this many, a step being its column-pass group launches
+// and then its row-kernel group launches.  The lanes run concurrently, so column
+// passes (VALU-bound table sums) of one lane overlap row kernels of another.  Measured
+// on the 512-matrix batch (n 500-12000): whole lanes 134.1 ms, 32 members 132.5 ms,
+// 8 members 129.9 ms; one chunk (64 GiB budget) or 8 lanes change it by < 1 ms.
+constexpr size_t kBatchStepMembers = 8;
+
+size_t round_up(size_t n, size_t m) { return (n + m - 1) / m * m; }
+
+spfft_status ensure_desc_capacity(spfft_batch_s& b, int64_t nnz) {
+    if (nnz <= b.nnz_cap) return SPFFT_OK;
+    if (nnz > INT_MAX) return fail(SPFFT_ERR_UNSUPPORTED, "batched CSC build: more than INT_MAX pairs in all");
+    const size_t cap = (size_t)std::max<int64_t>(nnz, 1);
+    const size_t hist_n = spfft::csc_sort_hist_elems((int)cap);
+    const size_t scan_n = std::max(spfft::csc_scan_tmp_elems(b.c_total), spfft::csc_scan_tmp_elems((int)hist_n));
+    SPFFT_HIP(b.mrow.alloc(cap * 4), "hipMalloc(batch COO)");
+    SPFFT_HIP(b.mcol.alloc(cap * 4), "hipMalloc(batch COO)");
+    SPFFT_HIP(b.keys0.alloc(cap * 4), "hipMalloc(batch keys)");
+    SPFFT_HIP(b.keys1.alloc(cap * 4), "hipMalloc(batch keys)");
+    SPFFT_HIP(b.vals0.alloc(cap * 4), "hipMalloc(batch vals)");
+    SPFFT_HIP(b.vals1.alloc(cap * 4), "hipMalloc(batch vals)");
+    SPFFT_HIP(b.row_idx.alloc(cap * 2), "hipMalloc(batch row_idx)");
+    SPFFT_HIP(b.hist.alloc(hist_n * 4), "hipMalloc(batch hist)");
+    SPFFT_HIP(b.hist_off.alloc((hist_n + 1) * 4), "hipMalloc(batch hist)");
+    SPFFT_HIP(b.scan_tmp.alloc(scan_n * sizeof(int2)), "hipMalloc(batch scan)");
+    b.nnz_cap = (int64_t)cap;
+    return SPFFT_OK;
+}
+
+// The group launches of one step: its members' column passes grouped by kernel (whole
+// W_R table, rows per lane, pairing), their row kernels grouped by shape.
+spfft_status build_step(spfft_batch_s& b, const int* ks, int n, spfft_batch_s::Step& step) {
+    std::vector<std::pair<std::vector<int>, std::vector<int4>>> cgroups;  // key, items
+    std::vector<int> creps;
+    std::vector<size_t> clds;
+    std::vector<std::pair<std::vector<int>, std::vector<int2>>> rgroups;
+    std::vector<int> rreps;
+    for (int j = 0; j < n; j++) {
+        const int k = ks[j];
+        const spfft::RowPassArgs& a = b.h_desc[k];
+        const std::vector<int> ckey = {a.wr_full ? 1 : 0, a.colpass_rows, a.pair_half};
+        size_t g = 0;
+        while (g < cgroups.size() && cgroups[g].first != ckey) g++;
+        if (g == cgroups.size()) {
+            cgroups.push_back({ckey, {}});
+            creps.push_back(k);
+            clds.push_back(0);
+        }
+        clds[g] = std::max(clds[g], spfft::colpass_lds_bytes(a));
+        int tx = 0, ty = 0;
+        spfft::colpass_grid(a, &tx, &ty);
+        for (int y = 0; y < ty; y++)
+            for (int x = 0; x < tx; x++) cgroups[g].second.push_back(make_int4(k, x, y, 0));
+        const std::vector<int> rkey = {a.rader, a.precision, a.M1, a.fused_nb, a.fused_rpw, a.fused_nt,
+                                       a.f2.nstages > 0 ? a.f2.radix[0] : 1, a.f2.nstages > 1 ? a.f2.radix[1] : 1,
+                                       a.f2.nstages > 2 ? a.f2.radix[2] : 1, a.direct};
+        g = 0;
+        while (g < rgroups.size() && rgroups[g].first != rkey) g++;
+        if (g == rgroups.size()) {
+            rgroups.push_back({rkey, {}});
+            rreps.push_back(k);
+        }
+        const int nblk = a.rader ? a.tile_rows : (a.tile_rows + a.fused_rpw - 1) / a.fused_rpw;
+        for (int blk = 0; blk < nblk; blk++) rgroups[g].second.push_back(make_int2(k, blk));
+    }
+    for (size_t g = 0; g < cgroups.size(); g++) {
+        spfft_batch_s::Group grp;
+        grp.rep = creps[g];
+        grp.lds = clds[g];
+        grp.n = (int)cgroups[g].second.size();
+        SPFFT_HIP(upload(grp.work, cgroups[g].second), "upload batch work");
+        step.colpass.push_back(std::move(grp));
+    }
+    for (size_t g = 0; g < rgroups.size(); g++) {
+        spfft_batch_s::Group grp;
+        grp.rep = rreps[g];
+        grp.n = (int)rgroups[g].second.size();
+        SPFFT_HIP(upload(grp.work, rgroups[g].second), "upload batch work");
+        step.rows.push_back(std::move(grp));
+    }
+    return SPFFT_OK;
+}
+
+// The members, their offsets, work lists and fixed buffers (once, at creation).
+spfft_status setup_desc(spfft_batch_s& b, const std::vector<double>& work) {
+    b.desc_on = false;
+    b.members.clear();
+    b.fallback.clear();
+    bool small = true;  // uint16 rows and columns for every matrix
+    for (int i = 0; i < b.count; i++) small = small && b.plans[i]->R <= 65536 && b.plans[i]->C <= 65536;
+    std::vector<int> order(b.count);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+    for (int i : order) {
+        const spfft_plan_s& p = *b.plans[i];
+        const bool one_tile = p.opts.u_tile >= p.R / 2 + 1 && p.opts.row_limit == 0;
+        if (b.mode == SPFFT_BATCH_AUTO && small && p.base.fused && one_tile) b.members.push_back(i);
+        else b.fallback.push_back(i);
+    }
+    b.r_off.assign(b.count, -1);
+    b.c_off.assign(b.count, -1);
+    b.x0_off.assign(b.count, 0);
+    int64_t rt = 0, ct = 0;
+    for (int i : b.members) {
+        b.r_off[i] = (int)rt;
+        b.c_off[i] = (int)ct;
+        rt += b.plans[i]->R;
+        ct += (int64_t)round_up((size_t)b.plans[i]->C, 64);
+    }
+    if (b.members.empty() || ct >= (1 << 24) || rt > INT_MAX) {
+        // (column keys of the batched sort must stay below 2^24: three 8-bit passes)
+        for (int i : b.members) b.fallback.push_back(i);
+        b.members.clear();
+        return SPFFT_OK;
+    }
+    b.desc_on = true;
+    b.r_total = (int)rt;
+    b.c_total = (int)ct;
+    // per-matrix {R, C, first row, first column} (not a member: R = -1)
+    std::vector<int4> mat(b.count, make_int4(-1, 0, 0, 0));
+    for (int i : b.members) mat[i] = make_int4(b.plans[i]->R, b.plans[i]->C, b.r_off[i], b.c_off[i]);
+    SPFFT_HIP(upload(b.mat, mat), "upload batch matrices");
+    SPFFT_HIP(b.merr.alloc((size_t)b.count * 4), "hipMalloc(batch err)");
+    SPFFT_HIP(b.coo_off_d.alloc((size_t)(b.count + 1) * 8), "hipMalloc(batch offsets)");
+    const size_t C1 = (size_t)b.c_total + 1;
+    SPFFT_HIP(b.counts.alloc(C1 * 4), "hipMalloc(batch counts)");
+    SPFFT_HIP(b.colstart.alloc(C1 * 4), "hipMalloc(batch colstart)");
+    SPFFT_HIP(b.sparse_cols.alloc(C1 * 4), "hipMalloc(batch sparse_cols)");
+    SPFFT_HIP(b.col_ptr.alloc(C1 * 4), "hipMalloc(batch col_ptr)");
+    SPFFT_HIP(b.totals.alloc(8), "hipMalloc(batch totals)");
+    SPFFT_HIP(b.err.alloc(4), "hipMalloc(batch err)");
+    SPFFT_HIP(b.sched.alloc(C1 * 4), "hipMalloc(batch schedule)");
+    SPFFT_HIP(b.sched_off.alloc((C1 / 8 + 1) * 9 * 4), "hipMalloc(batch schedule)");
+    {
+        std::vector<int> rb(b.c_total, 0);
+        for (int i : b.members)
+            std::fill(rb.begin() + b.c_off[i], rb.begin() + b.c_off[i] + (int)round_up(b.plans[i]->C, 64), b.r_off[i]);
+        SPFFT_HIP(upload(b.col_rbase, rb), "upload batch row bases");
+    }
+    // descriptors (pointers into the shared arrays; X0 and the output set below / per call)
+    const int nm = (int)b.members.size();
+    b.h_desc.assign(nm, spfft::RowPassArgs{});
+    for (int k = 0; k < nm; k++) {
+        const int i = b.members[k];
+        spfft::RowPassArgs& a = b.h_desc[k];
+        batch_descriptor(*b.plans[i], a);
+        a.colstart = b.colstart.as<int>() + b.c_off[i];
+        a.col_sched = b.sched.as<int>() + b.c_off[i];
+        a.col_sched_off = b.sched_off.as<int>() + (size_t)(b.c_off[i] / 8) * 9;
+        a.row_idx = b.row_idx.p;  // set again once row_idx is allocated
+    }
+    // chunks of members whose pass-1 rows fit the budget; the X0 arena holds the largest
+    size_t arena = 0, in_chunk = 0;
+    std::vector<std::vector<int>> chunk_members(1);
+    for (int k = 0; k < nm; k++) {
+        const spfft::RowPassArgs& a = b.h_desc[k];
+        const size_t bytes = (size_t)a.tile_rows * (size_t)a.ldx * sizeof(float2);
+        if (in_chunk > 0 && in_chunk + bytes > kBatchX0Budget) {
+            chunk_members.emplace_back();
+            in_chunk = 0;
+        }
+        b.x0_off[b.members[k]] = (int64_t)(in_chunk / sizeof(float2));
+        in_chunk += round_up(bytes, 256);
+        arena = std::max(arena, in_chunk);
+        chunk_members.back().push_back(k);
+    }
+    SPFFT_HIP(b.x0_arena.alloc(arena), "hipMalloc(batch pass-1 rows)");
+    for (int k = 0; k < nm; k++) b.h_desc[k].X0 = b.x0_arena.as<float2>() + b.x0_off[b.members[k]];
+    // work lists
+    {
+        std::vector<int2> w;
+        for (int k = 0; k < nm; k++) {
+            const spfft::RowPassArgs& a = b.h_desc[k];
+            const int tc = spfft::colpass_cols_per_tile(a.colpass_rows);
+            for (int t = 0; t < (a.C + tc - 1) / tc; t++) w.push_back(make_int2(k, t));
+        }
+        b.colsched_n = (int)w.size();
+        SPFFT_HIP(upload(b.colsched_work, w), "upload batch work");
+    }
+    // lanes: a chunk's members shared out longest first to the least loaded of the lane
+    // streams, which run concurrently (different kernels then share the CUs, as the
+    // worker streams of the per-matrix form do)
+    const int nlanes = std::max(1, std::min(b.req_streams > 0 ? b.req_streams : 4, 8));
+    if ((int)b.lane_streams.size() != nlanes) {
+        for (hipStream_t st : b.lane_streams) (void)hipStreamDestroy(st);
+        for (hipEvent_t e : b.lane_done) (void)hipEventDestroy(e);
+        b.lane_streams.assign(nlanes, nullptr);
+        b.lane_done.assign(nlanes, nullptr);
+        for (int l = 0; l < nlanes; l++) {
+            SPFFT_HIP(hipStreamCreateWithFlags(&b.lane_streams[l], hipStreamNonBlocking), "hipStreamCreate");
+            SPFFT_HIP(hipEventCreateWithFlags(&b.lane_done[l], hipEventDisableTiming), "hipEventCreate");
+        }
+    }
+    if (!b.lane_fork) SPFFT_HIP(hipEventCreateWithFlags(&b.lane_fork, hipEventDisableTiming), "hipEventCreate");
+    b.chunks.clear();
+    b.chunks.resize(chunk_members.size());
+    for (size_t c = 0; c < chunk_members.size(); c++) {
+        std::vector<std::vector<int>> lane_members(nlanes);
+        std::vector<double> lane_load(nlanes, 0.0);
+        for (int k : chunk_members[c]) {  // (already longest first)
+            const int l = (int)(std::min_element(lane_load.begin(), lane_load.end()) - lane_load.begin());
+            lane_members[l].push_back(k);
+            lane_load[l] += work[b.members[k]];
+        }
+        size_t most = 0;
+        for (int l = 0; l < nlanes; l++) most = std::max(most, lane_members[l].size());
+        for (size_t s0 = 0; s0 < most; s0 += kBatchStepMembers)
+            for (int l = 0; l < nlanes; l++) {
+                if (s0 >= lane_members[l].size()) continue;
+                const size_t s1 = std::min(lane_members[l].size(), s0 + kBatchStepMembers);
+                spfft_batch_s::Step step;
+                step.lane = l;
+                const spfft_status st = build_step(b, lane_members[l].data() + s0, (int)(s1 - s0), step);
+                if (st != SPFFT_OK) return st;
+                b.chunks[c].steps.push_back(std::move(step));
+            }
+    }
+    SPFFT_HIP(b.d_desc.alloc((size_t)nm * sizeof(spfft::RowPassArgs)), "hipMalloc(batch descriptors)");
+    b.desc_out = nullptr;
+    return SPFFT_OK;
+}
+
+// The members' descriptors for output arena d_out, into d_desc (synchronous upload).
+spfft_status write_descriptors(spfft_batch_s& b, void* d_out, DevBuf& d_desc) {
+    for (size_t k = 0; k < b.members.size(); k++) {
+        spfft::RowPassArgs& a = b.h_desc[k];
+        a.row_idx = b.row_idx.p;
+        a.out = a.out_m = static_cast<float2*>(d_out) + b.out_off[b.members[k]];
+    }
+    if (!d_desc.p || d_desc.bytes < b.h_desc.size() * sizeof(spfft::RowPassArgs))
+        SPFFT_HIP(d_desc.alloc(b.h_desc.size() * sizeof(spfft::RowPassArgs)), "hipMalloc(batch descriptors)");
+    SPFFT_HIP(hipMemcpy(d_desc.p, b.h_desc.data(), b.h_desc.size() * sizeof(spfft::RowPassArgs),
+                        hipMemcpyHostToDevice),
+              "hipMemcpy(batch descriptors)");
+    return SPFFT_OK;
+}
+
+// The matrices outside the descriptors: longest processing time first onto the least
+// loaded of the worker streams, which share one row-pass workspace per stream.
+spfft_status assign_slots(spfft_batch_s& b, const std::vector<double>& work) {
+    for (hipStream_t st : b.streams) (void)hipStreamDestroy(st);
+    for (hipEvent_t e : b.join) (void)hipEventDestroy(e);
+    b.streams.clear();
+    b.join.clear();
+    b.nslots = std::max(1, std::min(b.req_streams > 0 ? b.req_streams : 4, std::max((int)b.fallback.size(), 1)));
+    std::vector<int> order = b.fallback;
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+    std::vector<double> load(b.nslots, 0.0);
+    b.slot_of.assign(b.count, 0);
+    b.slot_list.assign(b.nslots, {});
+    for (int i : order) {
+        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        b.slot_of[i] = k;
+        b.slot_list[k].push_back(i);
+        load[k] += work[i];
+    }
+    b.streams.resize(b.nslots);
+    b.join.resize(b.nslots);
+    b.ws_T = std::vector<DevBuf>(b.nslots);
+    b.ws_X0 = std::vector<DevBuf>(b.nslots);
+    for (int k = 0; k < b.nslots; k++) {
+        SPFFT_HIP(hipStreamCreateWithFlags(&b.streams[k], hipStreamNonBlocking), "hipStreamCreate");
+        SPFFT_HIP(hipEventCreateWithFlags(&b.join[k], hipEventDisableTiming), "hipEventCreate");
+        size_t tmax = 0, xmax = 0;
+        for (int i : b.slot_list[k]) {
+            size_t t, x;
+            workspace_bytes(*b.plans[i], &t, &x);
+            tmax = std::max(tmax, t);
+            xmax = std::max(xmax, x);
+        }
+        SPFFT_HIP(b.ws_T[k].alloc(tmax), "hipMalloc(batch tile buffer)");
+        SPFFT_HIP(b.ws_X0[k].alloc(xmax), "hipMalloc(batch pass-1 buffer)");
+    }
+    return SPFFT_OK;
+}
+
+// The members' work on stream s (descriptors d_desc, COO offsets d_coo_off on the device).
+spfft_status run_batch_desc(spfft_batch_s& b, const int* d_row, const int* d_col, const int64_t* d_coo_off,
+                            int64_t nnz_total, const spfft::RowPassArgs* d_desc, hipStream_t s) {
+    SPFFT_HIP(spfft::launch_fill_int(b.merr.as<int>(), b.count, 0, s), "batch launch");
+    SPFFT_HIP(spfft::launch_batch_coo(d_row, d_col, reinterpret_cast<const long long*>(d_coo_off), b.count,
+                                      b.mat.as<int4>(), b.mrow.as<int>(), b.mcol.as<int>(), b.merr.as<int>(),
+                                      (long long)nnz_total, s),
+              "batch launch");
+    spfft::CscBuildArgs c{};
+    c.R = b.r_total;
+    c.C = b.c_total;
+    c.nnz = (int)nnz_total;
+    c.packed = false;
+    c.row = b.mrow.as<int>();
+    c.col = b.mcol.as<int>();
+    c.counts = b.counts.as<int>();
+    c.colstart = b.colstart.as<int>();
+    c.sparse_cols = b.sparse_cols.p;
+    c.col_ptr = b.col_ptr.as<int>();
+    c.row_idx = b.row_idx.p;
+    c.totals = b.totals.as<int>();
+    c.err = b.err.as<int>();
+    c.scan_tmp = b.scan_tmp.as<int2>();
+    c.hist = b.hist.as<int>();
+    c.hist_off = b.hist_off.as<int>();
+    c.keys[0] = b.keys0.as<int>();
+    c.keys[1] = b.keys1.as<int>();
+    c.vals[0] = b.vals0.as<int>();
+    c.vals[1] = b.vals1.as<int>();
+    c.col_rbase = b.col_rbase.as<int>();
+    SPFFT_HIP(spfft::launch_csc_build(c, s), "batch CSC build");
+    SPFFT_HIP(spfft::launch_colsched_batch(d_desc, b.colsched_work.as<int2>(), b.colsched_n, s), "batch launch");
+    for (const auto& ch : b.chunks) {
+        SPFFT_HIP(hipEventRecord(b.lane_fork, s), "hipEventRecord");
+        for (hipStream_t ls : b.lane_streams)
+            SPFFT_HIP(hipStreamWaitEvent(ls, b.lane_fork, 0), "hipStreamWaitEvent");
+        for (const auto& st : ch.steps) {
+            hipStream_t ls = b.lane_streams[st.lane];
+            for (const auto& g : st.colpass)
+                SPFFT_HIP(spfft::launch_colpass_batch(b.h_desc[g.rep], true, d_desc, g.work.as<int4>(), g.n, g.lds,
+                                                      ls),
+                          "batch column pass");
+            for (const auto& g : st.rows) {
+                const spfft::RowPassArgs& rep = b.h_desc[g.rep];
+                SPFFT_HIP(rep.rader ? spfft::launch_rader_batch(rep, d_desc, g.work.as<int2>(), g.n, ls)
+                                    : spfft::launch_fused_batch(rep, d_desc, g.work.as<int2>(), g.n, ls),
+                          "batch row pass");
+            }
+        }
+        for (size_t l = 0; l < b.lane_streams.size(); l++) {
+            SPFFT_HIP(hipEventRecord(b.lane_done[l], b.lane_streams[l]), "hipEventRecord");
+            SPFFT_HIP(hipStreamWaitEvent(s, b.lane_done[l], 0), "hipStreamWaitEvent");
+        }
+    }
+    return SPFFT_OK;
+}
+
+// The whole batch on stream s: the matrices outside the descriptors on the worker streams
+// (forked from s), the members on s, then s waits for the workers.
 spfft_status run_batch(spfft_batch_s& b, const int* d_row, const int* d_col, const int64_t* coo_off,
-                       void* d_out, hipStream_t s) {
+                       void* d_out, hipStream_t s, const spfft::RowPassArgs* d_desc, const int64_t* d_coo_off) {
     if (!coo_off) return fail(SPFFT_ERR_INVALID_ARG, "NULL COO offsets");
     if (!d_out) return fail(SPFFT_ERR_INVALID_ARG, "NULL output arena");
-    SPFFT_HIP(hipEventRecord(b.fork, s), "hipEventRecord");
-    for (int k = 0; k < b.nslots; k++) SPFFT_HIP(hipStreamWaitEvent(b.streams[k], b.fork, 0), "hipStreamWaitEvent");
-    for (int k = 0; k < b.nslots; k++) {
-        for (int i : b.slot_list[k]) {
-            spfft_plan_s& p = *b.plans[i];
-            const int64_t n = coo_off[i + 1] - coo_off[i];
-            if (n < 0 || n > 0x7fffffff) return fail(SPFFT_ERR_INVALID_ARG, "bad COO offsets");
-            spfft_status st = do_csc_build(p, d_row + coo_off[i], d_col + coo_off[i], (int)n, b.streams[k]);
-            if (st != SPFFT_OK) return st;
-            st = do_rowpass(p, static_cast<float2*>(d_out) + b.out_off[i], p.P, b.streams[k],
-                            b.ws_T[k].as<float2>(), b.ws_X0[k].as<float2>());
-            if (st != SPFFT_OK) return st;
+    const bool workers = !b.fallback.empty();
+    if (workers) {
+        SPFFT_HIP(hipEventRecord(b.fork, s), "hipEventRecord");
+        for (int k = 0; k < b.nslots; k++)
+            SPFFT_HIP(hipStreamWaitEvent(b.streams[k], b.fork, 0), "hipStreamWaitEvent");
+        for (int k = 0; k < b.nslots; k++) {
+            for (int i : b.slot_list[k]) {
+                spfft_plan_s& p = *b.plans[i];
+                const int64_t n = coo_off[i + 1] - coo_off[i];
+                if (n < 0 || n > 0x7fffffff) return fail(SPFFT_ERR_INVALID_ARG, "bad COO offsets");
+                spfft_status st = do_csc_build(p, d_row + coo_off[i], d_col + coo_off[i], (int)n, b.streams[k]);
+                if (st != SPFFT_OK) return st;
+                st = do_rowpass(p, static_cast<float2*>(d_out) + b.out_off[i], p.P, b.streams[k],
+                                b.ws_T[k].as<float2>(), b.ws_X0[k].as<float2>());
+                if (st != SPFFT_OK) return st;
+            }
+            SPFFT_HIP(hipEventRecord(b.join[k], b.streams[k]), "hipEventRecord");
         }
-        SPFFT_HIP(hipEventRecord(b.join[k], b.streams[k]), "hipEventRecord");
-        SPFFT_HIP(hipStreamWaitEvent(s, b.join[k], 0), "hipStreamWaitEvent");
     }
+    if (b.desc_on) {
+        spfft_status st = run_batch_desc(b, d_row, d_col, d_coo_off, coo_off[b.count] - coo_off[0], d_desc, s);
+        if (st != SPFFT_OK) return st;
+    }
+    if (workers)
+        for (int k = 0; k < b.nslots; k++) SPFFT_HIP(hipStreamWaitEvent(s, b.join[k], 0), "hipStreamWaitEvent");
     return SPFFT_OK;
 }
 
@@ -159,37 +566,12 @@ spfft_status spfft_batch_create(int count, const int* rows, const int* cols, con
         b->out_total += (int64_t)p->R * p->P;
         work[i] = work_estimate(*p, nnz_hint ? nnz_hint[i] : 8 * cols[i]);
     }
-    b->nslots = std::max(1, std::min(streams > 0 ? streams : 4, count));
-    // longest processing time first onto the least loaded stream
-    std::vector<int> order(count);
-    std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
-    std::vector<double> load(b->nslots, 0.0);
-    b->slot_of.assign(count, 0);
-    b->slot_list.assign(b->nslots, {});
-    for (int i : order) {
-        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        b->slot_of[i] = k;
-        b->slot_list[k].push_back(i);
-        load[k] += work[i];
-    }
-    b->streams.resize(b->nslots);
-    b->join.resize(b->nslots);
-    b->ws_T = std::vector<DevBuf>(b->nslots);
-    b->ws_X0 = std::vector<DevBuf>(b->nslots);
-    for (int k = 0; k < b->nslots; k++) {
-        SPFFT_HIP(hipStreamCreateWithFlags(&b->streams[k], hipStreamNonBlocking), "hipStreamCreate");
-        SPFFT_HIP(hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming), "hipEventCreate");
-        size_t tmax = 0, xmax = 0;
-        for (int i : b->slot_list[k]) {
-            size_t t, x;
-            workspace_bytes(*b->plans[i], &t, &x);
-            tmax = std::max(tmax, t);
-            xmax = std::max(xmax, x);
-        }
-        SPFFT_HIP(b->ws_T[k].alloc(tmax), "hipMalloc(batch tile buffer)");
-        SPFFT_HIP(b->ws_X0[k].alloc(xmax), "hipMalloc(batch pass-1 buffer)");
-    }
+    b->mode = SPFFT_BATCH_AUTO;
+    b->req_streams = streams;
+    b->work = work;
+    spfft_status st_d = setup_desc(*b, work);
+    if (st_d != SPFFT_OK) return st_d;
+    if ((st_d = assign_slots(*b, work)) != SPFFT_OK) return st_d;
     SPFFT_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
     *out = b.release();
     return SPFFT_OK;
@@ -232,15 +614,52 @@ static spfft_status check_batch_io(spfft_batch b, const int64_t* coo_off, const 
     return SPFFT_OK;
 }
 
+// Device copies of the members' descriptors (for output arena d_out) and of the COO
+// offsets, for the eager form: rewritten (after the stream has drained, since queued work
+// may still read them) only when they change.
+static spfft_status prepare_eager(spfft_batch b, const int64_t* coo_off, void* d_out, hipStream_t s) {
+    if (!b->desc_on) return SPFFT_OK;
+    const bool new_out = d_out != b->desc_out;
+    const bool new_off = b->coo_off_h.size() != (size_t)b->count + 1 ||
+                         !std::equal(b->coo_off_h.begin(), b->coo_off_h.end(), coo_off);
+    if (!new_out && !new_off) return SPFFT_OK;
+    SPFFT_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (new_out) {
+        spfft_status st = write_descriptors(*b, d_out, b->d_desc);
+        if (st != SPFFT_OK) return st;
+        b->desc_out = d_out;
+    }
+    if (new_off) {
+        b->coo_off_h.assign(coo_off, coo_off + b->count + 1);
+        SPFFT_HIP(hipMemcpy(b->coo_off_d.p, coo_off, (size_t)(b->count + 1) * 8, hipMemcpyHostToDevice),
+                  "hipMemcpy(batch offsets)");
+    }
+    return SPFFT_OK;
+}
+
+static spfft_status ensure_capacity_all(spfft_batch b, const int64_t* coo_off) {
+    for (int i : b->fallback) {
+        spfft_status st = ensure_csc_capacity(*b->plans[i], (int)(coo_off[i + 1] - coo_off[i]));
+        if (st != SPFFT_OK) return st;
+    }
+    if (b->desc_on) {
+        const bool had = b->row_idx.p != nullptr;
+        spfft_status st = ensure_desc_capacity(*b, coo_off[b->count] - coo_off[0]);
+        if (st != SPFFT_OK) return st;
+        if (had != (b->row_idx.p != nullptr) || b->h_desc.empty() || b->h_desc[0].row_idx != b->row_idx.p)
+            b->desc_out = nullptr;  // row_idx moved: the descriptors are rewritten
+    }
+    return SPFFT_OK;
+}
+
 spfft_status spfft_batch_execute(spfft_batch b, const int* d_row, const int* d_col, const int64_t* coo_off,
                                  void* d_out, void* stream) {
     spfft_status st = check_batch_io(b, coo_off, d_out);
     if (st != SPFFT_OK) return st;
-    for (int i = 0; i < b->count; i++) {
-        st = ensure_csc_capacity(*b->plans[i], (int)(coo_off[i + 1] - coo_off[i]));
-        if (st != SPFFT_OK) return st;
-    }
-    return run_batch(*b, d_row, d_col, coo_off, d_out, (hipStream_t)stream);
+    if ((st = ensure_capacity_all(b, coo_off)) != SPFFT_OK) return st;
+    if ((st = prepare_eager(b, coo_off, d_out, (hipStream_t)stream)) != SPFFT_OK) return st;
+    return run_batch(*b, d_row, d_col, coo_off, d_out, (hipStream_t)stream, b->d_desc.as<spfft::RowPassArgs>(),
+                     b->coo_off_d.as<int64_t>());
 }
 
 spfft_status spfft_batch_capture(spfft_batch b, const int* d_row, const int* d_col, const int64_t* coo_off,
@@ -249,18 +668,31 @@ spfft_status spfft_batch_capture(spfft_batch b, const int* d_row, const int* d_c
     *out = nullptr;
     spfft_status st0 = check_batch_io(b, coo_off, d_out);
     if (st0 != SPFFT_OK) return st0;
-    for (int i = 0; i < b->count; i++) {
-        spfft_status st = ensure_csc_capacity(*b->plans[i], (int)(coo_off[i + 1] - coo_off[i]));
+    if ((st0 = ensure_capacity_all(b, coo_off)) != SPFFT_OK) return st0;
+    for (int i = 0; i < b->count; i++) b->plans[i]->profiling = false;
+    std::unique_ptr<spfft_graph_s> g(new (std::nothrow) spfft_graph_s());
+    if (!g) return fail(SPFFT_ERR_ALLOC, "host allocation failed");
+    // the graph's own copies of the descriptors and offsets (later eager calls may
+    // rewrite the batch's)
+    const spfft::RowPassArgs* d_desc = nullptr;
+    const int64_t* d_off = nullptr;
+    if (b->desc_on) {
+        g->keep.resize(2);
+        spfft_status st = write_descriptors(*b, d_out, g->keep[0]);
         if (st != SPFFT_OK) return st;
-        b->plans[i]->profiling = false;
+        b->desc_out = nullptr;  // h_desc now holds this arena's pointers; eager calls rewrite theirs
+        SPFFT_HIP(g->keep[1].alloc((size_t)(b->count + 1) * 8), "hipMalloc(graph offsets)");
+        SPFFT_HIP(hipMemcpy(g->keep[1].p, coo_off, (size_t)(b->count + 1) * 8, hipMemcpyHostToDevice),
+                  "hipMemcpy(graph offsets)");
+        d_desc = g->keep[0].as<spfft::RowPassArgs>();
+        d_off = g->keep[1].as<int64_t>();
     }
     hipStream_t cs;
     SPFFT_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
-    std::unique_ptr<spfft_graph_s> g(new (std::nothrow) spfft_graph_s());
     spfft_status st = SPFFT_OK;
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        st = run_batch(*b, d_row, d_col, coo_off, d_out, cs);
+        st = run_batch(*b, d_row, d_col, coo_off, d_out, cs, d_desc, d_off);
         hipGraph_t graph = nullptr;
         e = hipStreamEndCapture(cs, &graph);
         g->graph = graph;
@@ -275,10 +707,38 @@ spfft_status spfft_batch_capture(spfft_batch b, const int* d_row, const int* d_c
 
 spfft_status spfft_batch_check_input(spfft_batch b, void* stream) {
     if (!b) return fail(SPFFT_ERR_INVALID_ARG, "NULL batch");
-    for (int i = 0; i < b->count; i++) {
+    if (b->desc_on) {
+        SPFFT_HIP(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+        std::vector<int> err(b->count, 0);
+        SPFFT_HIP(hipMemcpy(err.data(), b->merr.p, (size_t)b->count * 4, hipMemcpyDeviceToHost), "hipMemcpy(err)");
+        for (int i : b->members)
+            if (err[i])
+                return fail(SPFFT_ERR_BAD_INPUT, "matrix " + std::to_string(i) + ": COO index outside [0, rows) x [0, cols)");
+    }
+    for (int i : b->fallback) {
         spfft_status st = spfft_check_input(b->plans[i].get(), stream);
         if (st != SPFFT_OK) return fail(st, "matrix " + std::to_string(i) + ": " + spfft_last_error());
     }
+    return SPFFT_OK;
+}
+
+spfft_status spfft_batch_set_mode(spfft_batch b, int mode) {
+    if (!b) return fail(SPFFT_ERR_INVALID_ARG, "NULL batch");
+    if (mode != SPFFT_BATCH_AUTO && mode != SPFFT_BATCH_PER_MATRIX)
+        return fail(SPFFT_ERR_INVALID_ARG, "batch mode must be SPFFT_BATCH_AUTO or SPFFT_BATCH_PER_MATRIX");
+    if (mode == b->mode) return SPFFT_OK;
+    b->mode = mode;
+    std::vector<double> work(b->count);
+    for (int i = 0; i < b->count; i++) work[i] = b->work[i];
+    spfft_status st = setup_desc(*b, work);
+    if (st != SPFFT_OK) return st;
+    return assign_slots(*b, work);
+}
+
+spfft_status spfft_batch_get_mode(spfft_batch b, int* mode, int* members) {
+    if (!b) return fail(SPFFT_ERR_INVALID_ARG, "NULL batch");
+    if (mode) *mode = b->mode;
+    if (members) *members = (int)b->members.size();
     return SPFFT_OK;
 }
 

@@ -25,12 +25,18 @@ struct CscBuildArgs {
     int* hist_off;           // csc_sort_hist_elems(nnz) + 1
     int* keys[2];            // nnz each (only used when more than one pass)
     int* vals[2];
+    const int* col_rbase;    // batched build (NULL otherwise): per column, the first row of its
+                             // member matrix; row_idx then holds uint16 rows of that matrix
 };
 
 size_t csc_scan_tmp_elems(int n);
 size_t csc_sort_hist_elems(int nnz);
 int csc_sort_passes(int cols);
 hipError_t launch_csc_build(const CscBuildArgs& a, hipStream_t s);
+hipError_t launch_fill_int(int* p, long n, int v, hipStream_t s);
+// the batched build's block-diagonal COO (mat[i] = {R, C, first row, first column})
+hipError_t launch_batch_coo(const int* row, const int* col, const long long* coo_off, int count, const int4* mat,
+                            int* mrow, int* mcol, int* err, long long nnz, hipStream_t s);
 
 // ---- row pass (rowpass.hip) ----------------------------------------------
 bool rowpass_supports_m1(int m1);        // k_fft1 / k_ifft1 instantiated for this M1
@@ -45,6 +51,23 @@ size_t colsched_elems(int cols, size_t* off_elems);
 hipError_t launch_colsched(const int* colstart, const int* colperm, int cols, int rows_per_lane, int* sched,
                            int* off, hipStream_t s);
 hipError_t colpass_prepare(size_t lds_limit);      // once per process, before launches
+// ---- batched form: one launch per kernel instantiation for many matrices ----------
+// descs: device array of the matrices' RowPassArgs; work: per workgroup, the matrix
+// (descriptor index) and that matrix's block indices.
+hipError_t launch_colsched_batch(const RowPassArgs* descs, const int2* work, int n, hipStream_t s);
+// rep: one of the group's descriptors (its colpass_rows, pair_half and wr_full pick the
+// kernel); lds: the group's largest colpass_lds_bytes
+hipError_t launch_colpass_batch(const RowPassArgs& rep, bool packed, const RowPassArgs* descs, const int4* work,
+                                int n, size_t lds, hipStream_t s);
+// the (column tile, row block) grid of k_colpass for one tile of a matrix
+void colpass_grid(const RowPassArgs& a, int* tiles_x, int* tiles_y);
+int colpass_cols_per_tile(int rows_per_lane);
+// row kernels (k_rowfused / k_rowrader / k_rowrader_mfma) of rep's shape; work[i] = (descriptor,
+// row group); each matrix's tile_rows, u_base, out, ... come from its descriptor
+hipError_t launch_fused_batch(const RowPassArgs& rep, const RowPassArgs* descs, const int2* work, int n,
+                              hipStream_t s);
+hipError_t launch_rader_batch(const RowPassArgs& rep, const RowPassArgs* descs, const int2* work, int n,
+                              hipStream_t s);
 size_t conv_lds_bytes(const RowPassArgs& a);
 // Fused row kernel (k_rowfused) shapes: M1, LDS batches, rows per workgroup,
 // threads, stage radices r[3] (1 = no stage).  Returns false past the last one.

@@ -413,6 +413,24 @@ spfft_status do_csc_build(spfft_plan_s& p, const int* d_row, const int* d_col, i
     return SPFFT_OK;
 }
 
+size_t device_lds_limit() { return g_lds_limit; }
+
+// The one-tile row-pass arguments of plan p for the batched form: p.base with the
+// column-pass choices that a CSC build and row pass of this plan would resolve (rows per
+// lane, row pairing, whole W_R table); the CSC, X0 and output pointers are the caller's.
+void batch_descriptor(const spfft_plan_s& p, spfft::RowPassArgs& a) {
+    a = p.base;
+    a.P = p.P;
+    int U = p.base.colpass_rows;
+    if (U == 0) U = std::min(p.opts.u_tile, p.R / 2 + 1) >= 2048 ? 8 : 4;
+    a.colpass_rows = U;
+    a.pair_half = pair_eligible(p) && U >= 2 ? 1 : 0;
+    if (a.wr_full && spfft::colpass_lds_bytes(a) > g_lds_limit) a.wr_full = nullptr;
+    a.u_base = 0;
+    a.tile_rows = p.R / 2 + 1;
+    a.sub_rows = 0;
+}
+
 // Half-period row pairing in the column pass (SPFFT_PAIR_AUTO): R even, one tile
 // holds every computed row [0, R/2], no sampled row limit, and not one row per
 // lane (a paired lane owns U/2 rows and their U/2 partners).

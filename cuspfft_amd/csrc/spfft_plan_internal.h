@@ -30,6 +30,20 @@ struct DevBuf {
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            reset();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
     ~DevBuf() { reset(); }
     void reset() {
         if (p) (void)hipFree(p);
@@ -127,5 +141,7 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts, bool own
                             spfft_plan_s** out);
 // Bytes of the row-pass workspace: T (tile x M) and X0 (tile x ldx), complex64.
 void workspace_bytes(const spfft_plan_s& p, size_t* t_bytes, size_t* x0_bytes);
+size_t device_lds_limit();
+void batch_descriptor(const spfft_plan_s& p, spfft::RowPassArgs& a);
 
 }  // namespace spfft_internal

@@ -305,6 +305,19 @@ spfft_status spfft_batch_capture(spfft_batch batch, const int* d_row, const int*
                                  const int64_t* coo_off, void* d_out, spfft_graph* graph);
 /* Synchronises `stream`; SPFFT_ERR_BAD_INPUT names the first matrix with an out-of-range index. */
 spfft_status spfft_batch_check_input(spfft_batch batch, void* stream);
+/* How the batch runs.  AUTO (the default): the matrices whose row pass is one on-chip
+ * kernel (fused, direct or Rader plans with one tile; every matrix with rows, cols <=
+ * 65536) -- the members -- run as one launch sequence for all of them: one CSC build of
+ * their block-diagonal matrix, then per kernel shape one launch whose workgroups take
+ * (matrix, block) items from a device table of the matrices' arguments.  The other
+ * matrices keep their own launches on the worker streams, concurrently.  PER_MATRIX:
+ * every matrix on its own launches on the worker streams.  Both compute the same output.
+ * A mode change rebuilds the batch's shared buffers: graphs captured before it must not be
+ * launched after it. */
+enum { SPFFT_BATCH_AUTO = 0, SPFFT_BATCH_PER_MATRIX = 1 };
+spfft_status spfft_batch_set_mode(spfft_batch batch, int mode);
+/* The mode, and how many matrices run through the shared launches (members). */
+spfft_status spfft_batch_get_mode(spfft_batch batch, int* mode, int* members);
 
 /* ---- learned plan selector network ----------------------------------------
  * A D -> H -> H -> 1 ReLU MLP in FP32 with hand-written forward, backward and

@@ -98,3 +98,16 @@ def test_c4_batch_512_graph_vs_eager(gpu, oracle_mod):
             worst = max(worst, oracle_mod.rel_l2(X, ref))
     print(f"[c4] worst sampled rel_l2 = {worst:.3e}", flush=True)
     assert worst <= REL_L2_PLAIN
+    # the shared descriptor-driven launches against every matrix on its own launches
+    mode, members = b.mode()
+    print(f"[c4] {members} of 512 matrices in the shared launches", flush=True)
+    assert mode == sp.BATCH_AUTO and members > 256
+    del g, replay
+    b.set_mode(sp.BATCH_PER_MATRIX)
+    per = b.alloc_output()
+    b.execute(d_row, d_col, off, per)
+    torch.cuda.synchronize()
+    b.check_input()
+    for i in range(512):
+        V = mats[i].cols // 2 + 1
+        assert torch.equal(b.output_of(per, i)[:, :V], b.output_of(eager, i)[:, :V]), i

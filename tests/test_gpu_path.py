@@ -873,6 +873,71 @@ def test_batch_execute_and_graph(gpu, oracle_mod, streams):
         torch.cuda.synchronize()
 
 
+def test_batch_shared_launches_match_per_matrix(gpu, oracle_mod):
+    """The descriptor-driven batch (one CSC build of the block-diagonal matrix, one launch
+    per kernel shape over (matrix, block) work items) against every matrix on its own
+    launches: bit-identical outputs, also through graph replays; matrices that cannot be
+    members (four-step plans, several tiles) run beside them; a bad index is reported for
+    its own matrix."""
+    import torch
+    sp = gpu
+    mats = _batch_mats(24, 7)
+    for cols, rows in ((1858, 300), (4178, 97), (1024, 500), (2048, 64), (16706, 40)):
+        mats.append(_rect(sp, rows, cols, rows * 12, cols))
+    opts = [dict(u_tile=m.rows) for m in mats]
+    opts[3] = dict(u_tile=max(mats[3].rows // 3, 1))                 # several tiles
+    opts[5] = dict(u_tile=mats[5].rows, row_kernel=sp.ROW_FOUR_STEP)  # four-step kernels
+    opts[-1] = dict(u_tile=mats[-1].rows, precision=sp.PREC_BF16X3)  # the MFMA Rader kernel
+    shapes = [(m.rows, m.cols) for m in mats]
+    d_r, d_c, off = sp.Batch.pack(mats)
+    auto = sp.Batch(shapes, opts=opts, nnz_hint=[m.nnz for m in mats], streams=2)
+    mode, members = auto.mode()
+    print(f"[batch] {len(mats)} matrices, {members} members", flush=True)
+    assert mode == sp.BATCH_AUTO and 0 < members <= len(mats) - 2
+    per = sp.Batch(shapes, opts=opts, nnz_hint=[m.nnz for m in mats], streams=2)
+    per.set_mode(sp.BATCH_PER_MATRIX)
+    assert per.mode() == (sp.BATCH_PER_MATRIX, 0)
+    a_auto, a_per = auto.alloc_output(), per.alloc_output()
+    auto.execute(d_r, d_c, off, a_auto)
+    per.execute(d_r, d_c, off, a_per)
+    torch.cuda.synchronize()
+    auto.check_input()
+    per.check_input()
+    for i, m in enumerate(mats):
+        V = m.cols // 2 + 1
+        assert torch.equal(auto.output_of(a_auto, i)[:, :V], per.output_of(a_per, i)[:, :V]), i
+        if i % 4 == 0 or i >= 24:
+            X = auto.output_of(a_auto, i)[:, :V].cpu().numpy()
+            ref = oracle_mod.sparse_rfft2_f64(m.rows, m.cols, m.row_idx, m.col_idx, nthreads=8)
+            tol = REL_L2_BF16X3 if auto.infos[i]["precision"] == 1 else REL_L2_PLAIN
+            assert oracle_mod.rel_l2(X, ref) <= tol, i
+    g = auto.capture(d_r, d_c, off, auto.alloc_output())
+    replay = g._keep[3]
+    for rep in range(2):
+        replay.zero_()
+        g.launch()
+        torch.cuda.synchronize()
+        for i, m in enumerate(mats):
+            V = m.cols // 2 + 1
+            assert torch.equal(auto.output_of(replay, i)[:, :V], auto.output_of(a_auto, i)[:, :V]), (rep, i)
+        auto.execute(d_r, d_c, off, a_auto)  # eager work between replays
+        torch.cuda.synchronize()
+    # an index outside a member matrix: reported for that matrix
+    k = next(i for i, inf in enumerate(auto.infos)
+             if inf["row_kernel"] == sp.ROW_FUSED and inf["u_tile"] >= mats[i].rows // 2 + 1 and mats[i].nnz)
+    col = d_c.clone()
+    col[int(off[k])] = mats[k].cols + 5
+    auto.execute(d_r, col, off, a_auto)
+    with pytest.raises(sp.SpfftError, match=f"matrix {k}:"):
+        auto.check_input()
+
+
+def _rect(sp, rows, cols, nnz, seed):
+    rng = np.random.default_rng(seed)
+    r, c = _random_coo(rng, rows, cols, nnz)
+    return sp.COOMatrix(rows, cols, r, c)
+
+
 def test_batch_graph_with_direct_widths(gpu, oracle_mod):
     """A batch mixing direct-transform widths (1024, 1152, 2048) with Bluestein ones,
     one tile per matrix (paired column passes for even rows): eager against the FP64
