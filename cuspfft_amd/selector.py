@@ -14,7 +14,9 @@ Features
   plan:   log2 M, M / (2 cols - 1), the exponents of 2, 3, 5, 7 in M, log2 M1,
           log2 M2, fast convolution kernel, convolution stages, arithmetic form,
           column-pass rows per lane (one-hot 1 / 2 / 4 / 8, auto = all zero), fused
-          row kernel
+          row kernel, direct transform, Rader transform, log2 threads per workgroup
+          of an on-chip row kernel (0 for the four-step kernels), log2 of the tile's
+          share of the R/2 + 1 computed rows (0: one tile)
 """
 from __future__ import annotations
 
@@ -27,8 +29,8 @@ import numpy as np
 
 from . import TRANSFORM_DIRECT, TRANSFORM_RADER, _check, _vp, describe_plan, load
 
-N_FEATURES = 29
-# the network trained on the autotuner timings of 2000 seeded matrices (train_selector)
+N_FEATURES = 33
+# the network trained on the autotuner timings of the seeded training set (train_selector)
 DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "selector.json")
 
 
@@ -43,13 +45,15 @@ def _exponents(M: int):
     return out
 
 
-def features(rec: dict, opts: dict, info: dict | None = None) -> np.ndarray:
-    """Feature row of matrix `rec` (autotune.features_of) with plan `opts`."""
+def features(rec: dict, opts: dict, info: dict | None = None, trial: bool = False) -> np.ndarray:
+    """Feature row of matrix `rec` (autotune.features_of) with plan `opts` (trial: the
+    options of a timed trial, trial_opts)."""
     if info is None:
-        # timings recorded before the fused row kernel existed carry no row_kernel:
-        # they ran the four-step kernels
         o = {k: v for k, v in opts.items() if k != "u_tile"}
-        o.setdefault("row_kernel", 1)
+        if trial and o.get("row_transform") not in (TRANSFORM_DIRECT, TRANSFORM_RADER):
+            # trials recorded before the fused row kernel existed carry no row_kernel:
+            # they ran the four-step kernels
+            o.setdefault("row_kernel", 1)
         info = describe_plan(rec["rows"], rec["cols"], **o)
     rows, cols, nnz = rec["rows"], rec["cols"], rec["nnz"]
     act = max(rec.get("active_cols", cols), 1)
@@ -63,6 +67,11 @@ def features(rec: dict, opts: dict, info: dict | None = None) -> np.ndarray:
           float(info["precision"])]
     f += [1.0 if cpr == r else 0.0 for r in (1, 2, 4, 8)]
     f += [1.0 if info.get("row_kernel") == 2 else 0.0]
+    nt = info.get("row_threads", 0)
+    full = rows // 2 + 1
+    tile = min(opts.get("u_tile") or full, opts.get("t_rows") or full, full)
+    f += [float(bool(info.get("direct"))), float(bool(info.get("rader"))), math.log2(nt) if nt > 0 else 0.0,
+          math.log2(tile / full)]
     assert len(f) == N_FEATURES
     return np.asarray(f, np.float32)
 
@@ -90,7 +99,7 @@ def dataset(records: list[dict]):
     X, y, g = [], [], []
     for i, rec in enumerate(records):
         for t in rec["trials"]:
-            X.append(features(rec, trial_opts(t, rec)))
+            X.append(features(rec, trial_opts(t, rec), trial=True))
             y.append(math.log(max(t["ms"], 1e-4)))
             g.append(i)
     return np.asarray(X, np.float32), np.asarray(y, np.float32), np.asarray(g, np.int64)
@@ -215,15 +224,11 @@ _default = None
 
 def choose_plan(m, max_m1: int = 2, fp32_only: bool = False) -> dict:
     """Plan options for matrix m (COOMatrix) chosen by the shipped network among
-    the autotuner's candidates (cuspfft_amd.autotune.candidates), for Plan(**opts);
-    the planner's plan for widths with a Rader transform.
+    the autotuner's candidates (cuspfft_amd.autotune.candidates: Bluestein, and the
+    direct or Rader transform where the width has one), for Plan(**opts).
     fp32_only: among the plain FP32 plans only (no BF16x3 precision form)."""
     global _default
     from . import autotune
-    if describe_plan(m.rows, m.cols)["rader"]:
-        # Rader's convolutions (half the Bluestein row-pass work) are not among the
-        # plans the network was trained on: the planner's plan
-        return dict(length_rule=3)
     if _default is None:
         _default = Selector.load(DEFAULT_MODEL)
     rec = autotune.features_of(m)
@@ -242,7 +247,7 @@ def regret(sel: Selector, records: list[dict]) -> dict:
     for rec in records:
         trials = rec["trials"]
         best = min(t["ms"] for t in trials)
-        X = np.stack([features(rec, trial_opts(t, rec)) for t in trials])
+        X = np.stack([features(rec, trial_opts(t, rec), trial=True) for t in trials])
         pick = trials[int(np.argmin(sel.predict(X)))]
         ratios.append(pick["ms"] / best)
         dflt = [t for t in trials if t.get("default")]

@@ -53,7 +53,7 @@ def test_features_shape_and_determinism():
 def test_trial_opts_replans_each_transform():
     """A timed trial is planned again as the transform it ran: Bluestein at its length,
     the direct transform, or Rader's convolutions (flagged, or recognised by their length
-    in records written before the flags); choose_plan gives Rader widths the planner's plan."""
+    in records written before the flags)."""
     import cuspfft_amd as sp
     from cuspfft_amd import selector
     rec = {"cols": 16706}
@@ -66,19 +66,52 @@ def test_trial_opts_replans_each_transform():
         assert sp.describe_plan(16706, 16706, **o)["rader"] == 1
     d = selector.trial_opts({"opts": dict(length_rule=3), "fft_len": 16384}, {"cols": 16384})
     assert d["row_transform"] == sp.TRANSFORM_DIRECT
-    m = sp.generate("uniform", 16706, 2000, seed=3)
-    assert sp.describe_plan(m.rows, m.cols, **selector.choose_plan(m))["rader"] == 1
+
+
+def test_candidates_cover_transforms_and_tiles():
+    """The autotuner's search space holds, beside the Bluestein plans, Rader's
+    transform in both arithmetic forms (and both workgroup sizes of the BF16x3 kernel
+    at 16706) and the direct transform where the width has them; Bluestein candidates
+    say so where AUTO would pick another transform; the tile variants stay below the
+    R/2 + 1 computed rows."""
+    import cuspfft_amd as sp
+    from cuspfft_amd import autotune
+    cands = autotune.candidates(16706, 16706, 2)
+    infos = [sp.describe_plan(16706, 16706, **c) for c in cands]
+    rader = [(i["precision"], i["row_threads"]) for i in infos if i["rader"]]
+    assert sorted(rader) == [(0, 704), (1, 576), (1, 704)]
+    assert all(c.get("row_transform") == sp.TRANSFORM_BLUESTEIN for c, i in zip(cands, infos) if not i["rader"])
+    assert len({(i["fft_len"], i["m1"], i["precision"], i["row_kernel"], i["rader"], i["row_threads"])
+                for i in infos}) == len(infos)
+    d = [sp.describe_plan(4096, 4096, **c) for c in autotune.candidates(4096, 4096, 2)]
+    assert sum(i["direct"] for i in d) == 1
+    c1 = autotune.candidates(2310, 2310, 2)
+    assert all("row_transform" not in c for c in c1)  # no other transform at 2310
+    four = [c for c in c1 if sp.describe_plan(2310, 2310, **c)["row_kernel"] == sp.ROW_FOUR_STEP][0]
+    t = autotune.tile_candidates(2310, four, sp.describe_plan(2310, 2310, **four))
+    assert [x["t_rows"] for x in t] == [1024]
+    fused = [c for c in c1 if sp.describe_plan(2310, 2310, **c)["row_kernel"] == sp.ROW_FUSED][0]
+    assert [x["u_tile"] for x in autotune.tile_candidates(2310, fused, {"row_kernel": sp.ROW_FUSED})] == [1024]
+    assert autotune.tile_candidates(1000, fused, {"row_kernel": sp.ROW_FUSED}) == []
 
 
 def test_training_set_is_seeded():
+    import cuspfft_amd as sp
     from cuspfft_amd import autotune
-    a, ta = autotune.training_matrices(6, nnz_cap=20000)
-    b, tb = autotune.training_matrices(6, nnz_cap=20000)
-    assert ta == tb
+    a, ta = autotune.training_matrices(6, nnz_cap=20000, per_rader_width=1)
+    b, tb = autotune.training_matrices(6, nnz_cap=20000, per_rader_width=1)
+    assert ta == tb and len(ta) == 6 + len(sp.rader_widths())
     for x, y in zip(a, b):
         assert x.rows == y.rows and x.nnz == y.nnz
         np.testing.assert_array_equal(x.col_idx, y.col_idx)
     assert {t["kind"] for t in ta} == {"uniform", "chunglu"}
+    assert sorted(x.cols for x in a[6:]) == sorted(sp.rader_widths())
+    # the default cap does not bind anywhere in the 2000-matrix set: its densest
+    # matrices (density up to 1e-2 at n up to 40000) keep their ~16M nonzeros
+    specs = autotune.training_specs(2000)
+    big = [s["nnz"] for s in specs if s["nnz"] > 2_000_000]
+    assert len(big) > 50 and max(big) > 15_000_000
+    assert all(s["nnz"] < 16_000_000 for s in specs)
 
 
 @pytest.mark.gpu
@@ -133,12 +166,12 @@ def test_selector_beats_default_plan_on_heldout():
 
 @pytest.mark.gpu
 def test_shipped_selector_plan_runs(oracle_mod):
-    """The shipped network picks a plan for the c1 and c2 shapes that computes the
-    transform within the plain FP32 bar."""
+    """The shipped network picks a plan for the c1, c2 and c3 shapes that computes the
+    transform within the bar of its arithmetic form."""
     import torch
     import cuspfft_amd as sp
     from cuspfft_amd import selector
-    for cfg in ("c1", "c2"):
+    for cfg in ("c1", "c2", "c3"):
         m = sp.config_matrix(cfg)
         opts = selector.choose_plan(m)
         plan = sp.Plan(m.rows, m.cols, u_tile=m.rows, **opts)
