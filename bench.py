@@ -297,8 +297,6 @@ def main():
     transform = "direct" if info["direct"] else "rader" if info["rader"] else "bluestein"
     if plan_source == "planner":
         plan_source = f"planner ({transform}; selector not used)"
-    elif plan_source == "selector" and info["rader"]:
-        plan_source = "planner (rader; selector bypassed: its network has no Rader plans)"
     d_row = torch.as_tensor(m.row_idx).to(dev)
     d_col = torch.as_tensor(m.col_idx).to(dev)
     out = plan.alloc_output()
