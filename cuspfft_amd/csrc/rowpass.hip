@@ -40,10 +40,16 @@ namespace {
 
 constexpr int kColThreads = 512;
 constexpr int kColWaves = kColThreads / 64;
+// k_colpass_mx: columns per wave (tiles of 8 * CPW columns)
+constexpr int kColpassMxCpw = 1;
 
 // Columns per k_colpass tile for U output rows per lane (64 U rows per tile).
 __host__ __device__ constexpr int colpass_tile_cols(int rows_per_lane) {
-    return rows_per_lane == 1 ? 64 : rows_per_lane == 2 ? 32 : rows_per_lane == 4 ? 16 : 8;  // 8: U = 8, 16
+    return rows_per_lane == kColpassMfma ? kColWaves * kColpassMxCpw
+           : rows_per_lane == 1          ? 64
+           : rows_per_lane == 2          ? 32
+           : rows_per_lane == 4          ? 16
+                                         : 8;  // U = 8, 16
 }
 
 // ---------------------------------------------------------------------------
@@ -318,6 +324,172 @@ __global__ __launch_bounds__(kColThreads) void k_colpass_batch(const RowPassArgs
     colpass_body<PACKED, FULL, U, TC, PAIR>(descs[w.x], w.y, w.z);
 }
 
+// ---------------------------------------------------------------------------
+// k_colpass_mx: the column pass on the matrix cores in FP32 (colpass_rows
+// kColpassMfma).  A workgroup (8 waves) takes 8 CPW columns x 1024 rows, CPW
+// columns per wave.  With u = ub + 32 u1 + u0 (u1, u0 < 32) the phase splits as
+//   W_R^{u r} = W_R^{(ub + 32 u1) r} * W_R^{u0 r}
+// so a column's block of 1024 row sums is a complex product of a 32 x nnz and an
+// nnz x 32 matrix.  v_mfma_f32_32x32x2f32 takes one nonzero per instruction, its
+// two K slots holding the real and imaginary parts (lane l supplies A[l % 32][l / 32]
+// and B[l / 32][l % 32]; element v of lane l is D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32]):
+//   re:  A = (a.x, -a.y), B = (b.x, b.y)  ->  sum a.x b.x - a.y b.y
+//   im:  A = (a.x,  a.y), B = (b.y, b.x)  ->  sum a.x b.y + a.y b.x
+// with a = W_R^{(ub + 32 u1) r} (u1 = l % 32) and b = W_R^{u0 r} (u0 = l % 32), both
+// from the exact integer phase and the LDS table as in k_colpass.  Products and sums
+// are FP32 (the matrix cores' FP32 path), so the result is the plain form's.  Per
+// nonzero and 1024 rows: two table lookups per lane and two MFMAs, against a lookup
+// and 7 complex products per lane and 512 rows in k_colpass.
+// PAIR (R even, one tile of rows [0, H]): as in k_colpass, each 64-nonzero chunk is
+// permuted so that its even rows come first; the even-r and odd-r nonzeros go into
+// their own accumulators E and O (two straight loops), and rows u and H - u come from
+// E + O and conj(E - O), written in two passes through the same LDS tile.
+// ---------------------------------------------------------------------------
+typedef float mx_f32x16 __attribute__((ext_vector_type(16)));
+
+template <bool PACKED, bool FULL, bool PAIR, int CPW>
+__device__ __forceinline__ void colpass_mx_body(const RowPassArgs& a, const int bx, const int by) {
+    extern __shared__ float2 smem[];
+    constexpr int TC = kColWaves * CPW, RB = 1024, TS = TC + 1;
+    const int lane = threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c0 = bx * TC;
+    const int u0l = by * RB;
+    const uint32_t R = (uint32_t)a.R;
+    const int nlo = 1 << a.hi_shift;
+    float2* s_lo = smem;  // FULL: the whole W_R table
+    float2* s_hi = s_lo + nlo;
+    float* t_re = reinterpret_cast<float*>(FULL ? s_lo + R : s_hi + a.n_hi);  // [RB][TS]
+    float* t_im = t_re + RB * TS;                                              // [RB][TS]
+    if (FULL) {
+        for (int i = threadIdx.x; i < (int)R; i += kColThreads) s_lo[i] = a.wr_full[i];
+    } else {
+        for (int i = threadIdx.x; i < nlo; i += kColThreads) s_lo[i] = a.wr_lo[i];
+        for (int i = threadIdx.x; i < a.n_hi; i += kColThreads) s_hi[i] = a.wr_hi[i];
+    }
+    __syncthreads();
+    const uint32_t mask = (uint32_t)nlo - 1u;
+    const int hs = a.hi_shift;
+    auto wr = [&](uint32_t ph) -> float2 {
+        return FULL ? s_lo[ph] : cmul(s_hi[ph >> hs], s_lo[ph & mask]);
+    };
+    // x < 32 R: q = floor(x inv_r) is floor(x / R) or one less
+    auto modr = [&](uint32_t x) -> uint32_t {
+        const uint32_t q = (uint32_t)((float)x * a.inv_r);
+        const uint32_t y = x - __umul24(q, R);
+        return min(y, y - R);
+    };
+    const uint32_t ub = (uint32_t)(a.u_base + u0l);
+    const uint32_t i32 = (uint32_t)(lane & 31);
+    const bool khi = lane >= 32;
+    const bool chirp = !(a.direct || a.rader);
+    // per column of this wave (positions c0 + wid + 8 k): E (or all) and O sums
+    mx_f32x16 er[CPW], ei[CPW], orr[CPW], oi[CPW];
+    float2 hc[CPW];
+#pragma unroll
+    for (int k = 0; k < CPW; k++) {
+        er[k] = ei[k] = orr[k] = oi[k] = mx_f32x16{};
+        hc[k] = make_float2(1.f, 0.f);
+        const int pos = c0 + wid + kColWaves * k;  // this wave's X0 column position
+        if (pos >= a.C) continue;
+        const int mc = a.colperm ? a.colperm[pos] : pos;
+        const int beg = __builtin_amdgcn_readfirstlane(a.colstart[mc]);
+        const int end = __builtin_amdgcn_readfirstlane(a.colstart[mc + 1]);
+        if (chirp) hc[k] = a.h[pos];
+        for (int p0 = beg; p0 < end; p0 += 64) {
+            const int n = min(64, end - p0);
+            // per-nonzero constants, one nonzero per lane: r, (ub r) mod R, (32 r) mod R
+            uint32_t r = 0;
+            if (lane < n)
+                r = PACKED ? (uint32_t)((const uint16_t*)a.row_idx)[p0 + lane]
+                           : (uint32_t)((const int*)a.row_idx)[p0 + lane];
+            int n_even = n;
+            if (PAIR) {
+                // the chunk's even rows first, then its odd rows (one forward permute), so
+                // that each accumulator pair takes one straight loop
+                const uint64_t valid = n >= 64 ? ~0ull : (1ull << n) - 1;
+                const uint64_t om = __ballot((r & 1u) != 0) & valid, em = ~om & valid;
+                n_even = n - __popcll(om);
+                const uint64_t m = (r & 1u) ? om : em;
+                int dst = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                dst = lane >= n ? lane : (r & 1u) ? n_even + dst : dst;
+                r = (uint32_t)__builtin_amdgcn_ds_permute(dst << 2, (int)r);
+            }
+            const uint32_t base = PACKED ? fastmod_u32(ub * r, a.r_magic, R)
+                                         : (uint32_t)(((uint64_t)ub * r) % (uint64_t)R);
+            const uint32_t s32 = fastmod_u32(r << 5, a.r_magic, R);
+            auto run = [&](int jb, int je, mx_f32x16& accr, mx_f32x16& acci) {
+                for (int j = jb; j < je; j++) {
+                    const uint32_t rj = (uint32_t)__builtin_amdgcn_readlane((int)r, j);
+                    const uint32_t bj = (uint32_t)__builtin_amdgcn_readlane((int)base, j);
+                    const uint32_t sj = (uint32_t)__builtin_amdgcn_readlane((int)s32, j);
+                    const float2 wa = wr(modr(bj + __umul24(i32, sj)));  // row block u1 = l % 32
+                    const float2 wb = wr(modr(__umul24(i32, rj)));       // row u0 = l % 32
+                    const float are = khi ? -wa.y : wa.x, aim = khi ? wa.y : wa.x;
+                    const float bre = khi ? wb.y : wb.x, bim = khi ? wb.x : wb.y;
+                    accr = __builtin_amdgcn_mfma_f32_32x32x2f32(are, bre, accr, 0, 0, 0);
+                    acci = __builtin_amdgcn_mfma_f32_32x32x2f32(aim, bim, acci, 0, 0, 0);
+                }
+            };
+            run(0, n_even, er[k], ei[k]);
+            if (PAIR) run(n_even, n, orr[k], oi[k]);
+        }
+    }
+    const int half = a.R >> 1, qmax = half >> 1;  // PAIR: rows u <= qmax and H - u
+    const int base_row = 4 * (lane >> 5);
+    for (int pass = 0; pass < (PAIR ? 2 : 1); pass++) {
+        // the wave's 1024-row sums (pass 1 of PAIR: the partner rows) into its tile columns
+#pragma unroll
+        for (int k = 0; k < CPW; k++) {
+            const int col = wid + kColWaves * k;
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                float2 x = make_float2(er[k][v], ei[k][v]);
+                if (PAIR) {
+                    x = pass == 0 ? make_float2(er[k][v] + orr[k][v], ei[k][v] + oi[k][v])
+                                  : make_float2(er[k][v] - orr[k][v], oi[k][v] - ei[k][v]);
+                }
+                if (chirp) x = cmul(x, hc[k]);
+                const int row = 32 * (8 * (v >> 2) + base_row + (v & 3)) + (lane & 31);
+                t_re[row * TS + col] = x.x;
+                t_im[row * TS + col] = x.y;
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < RB * TC; e += kColThreads) {
+            const int row = e / TC, col = e % TC;
+            const int u = u0l + row;
+            int ul;
+            bool ok;
+            if (PAIR) {
+                ul = pass == 0 ? u : half - u;
+                ok = u <= qmax && (pass == 0 || ul != u);  // H - u == u only for u = H/2: written once
+            } else {
+                ul = u;
+                ok = u < a.tile_rows;
+            }
+            if (ok) {  // X0 is read once, by the row pass: non-temporal
+                typedef float f32x2 __attribute__((ext_vector_type(2)));
+                const f32x2 x = {t_re[row * TS + col], t_im[row * TS + col]};
+                __builtin_nontemporal_store(x, reinterpret_cast<f32x2*>(&a.X0[(size_t)ul * a.ldx + c0 + col]));
+            }
+        }
+        if (PAIR && pass == 0) __syncthreads();
+    }
+}
+
+template <bool PACKED, bool FULL, bool PAIR>
+__global__ __launch_bounds__(kColThreads) void k_colpass_mx(RowPassArgs a) {
+    colpass_mx_body<PACKED, FULL, PAIR, kColpassMxCpw>(a, blockIdx.x, blockIdx.y);
+}
+
+template <bool PACKED, bool FULL, bool PAIR>
+__global__ __launch_bounds__(kColThreads) void k_colpass_mx_batch(const RowPassArgs* __restrict__ descs,
+                                                                  const int4* __restrict__ work) {
+    const int4 w = work[blockIdx.x];
+    colpass_mx_body<PACKED, FULL, PAIR, kColpassMxCpw>(descs[w.x], w.y, w.z);
+}
+
 }  // namespace
 
 bool rowpass_supports_m1(int m1) {
@@ -332,8 +504,10 @@ bool rowpass_supports_m1(int m1) {
 size_t colpass_lds_bytes(const RowPassArgs& a) {
     const int U = a.colpass_rows > 0 ? a.colpass_rows : 1;
     const int TC = colpass_tile_cols(U);
-    // (paired: U/2 rows per lane and their U/2 partner rows: the same tile)
-    const size_t tile = (size_t)2 * 64 * U * (TC + 1) * sizeof(float) + (size_t)2 * TC * sizeof(int);
+    // (paired: U/2 rows per lane and their U/2 partner rows: the same tile; the
+    // matrix-core kernel: 1024 rows, partner rows in a second pass)
+    const size_t tile = U == kColpassMfma ? (size_t)2 * 1024 * (TC + 1) * sizeof(float)
+                                          : (size_t)2 * 64 * U * (TC + 1) * sizeof(float) + (size_t)2 * TC * sizeof(int);
     if (a.wr_full) return tile + (size_t)a.R * sizeof(float2);
     return tile + ((size_t)(1 << a.hi_shift) + a.n_hi) * sizeof(float2);
 }
@@ -386,7 +560,20 @@ hipError_t colpass_prepare(size_t lds_limit) {
 #undef SPFFT_COLPASS_BATCH
 #undef SPFFT_COLPASS_KERNELS
     };
+    const void* kmx[] = {
+#define SPFFT_COLPASS_MX(K)                                                                                   \
+    reinterpret_cast<const void*>(&K<true, true, false>), reinterpret_cast<const void*>(&K<true, false, false>),   \
+        reinterpret_cast<const void*>(&K<false, true, false>), reinterpret_cast<const void*>(&K<false, false, false>), \
+        reinterpret_cast<const void*>(&K<true, true, true>), reinterpret_cast<const void*>(&K<true, false, true>),     \
+        reinterpret_cast<const void*>(&K<false, true, true>), reinterpret_cast<const void*>(&K<false, false, true>),
+        SPFFT_COLPASS_MX(k_colpass_mx) SPFFT_COLPASS_MX(k_colpass_mx_batch)
+#undef SPFFT_COLPASS_MX
+    };
     for (const void* k : ks) {
+        const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit);
+        if (e != hipSuccess) return e;
+    }
+    for (const void* k : kmx) {
         const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit);
         if (e != hipSuccess) return e;
     }
@@ -404,6 +591,13 @@ static hipError_t launch_colpass_batch_u(const RowPassArgs& rep, const RowPassAr
                                          size_t lds, hipStream_t s) {
     const int U = rep.colpass_rows;
     const int TC = colpass_tile_cols(U);
+    if (U == kColpassMfma) {
+        if (rep.pair_half)
+            hipLaunchKernelGGL((k_colpass_mx_batch<PACKED, FULL, true>), dim3(n), dim3(kColThreads), lds, s, descs, work);
+        else
+            hipLaunchKernelGGL((k_colpass_mx_batch<PACKED, FULL, false>), dim3(n), dim3(kColThreads), lds, s, descs, work);
+        return hipGetLastError();
+    }
     if (rep.pair_half) {
 #define SPFFT_COLPASS_LAUNCH(u, tc)                                                                          \
     if (U == u && TC == tc) {                                                                                \
@@ -439,6 +633,10 @@ void colpass_grid(const RowPassArgs& a, int* tiles_x, int* tiles_y) {
     const int U = a.colpass_rows;
     const int TC = colpass_tile_cols(U);
     *tiles_x = (a.C + TC - 1) / TC;
+    if (U == kColpassMfma) {
+        *tiles_y = ((a.pair_half ? (a.R >> 2) + 1 : a.tile_rows) + 1023) / 1024;
+        return;
+    }
     *tiles_y = a.pair_half ? ((a.R >> 2) + 1 + 64 * (U / 2) - 1) / (64 * (U / 2))
                            : (a.tile_rows + 64 * U - 1) / (64 * U);
 }
@@ -450,6 +648,16 @@ static void launch_colpass_u(const RowPassArgs& a, hipStream_t s) {
     const int U = a.colpass_rows;
     const int TC = colpass_tile_cols(U);
     const size_t lds = colpass_lds_bytes(a);
+    if (U == kColpassMfma) {
+        // rows u < tile_rows, or (paired) u <= R/4 and their partners R/2 - u
+        const int rows = a.pair_half ? (a.R >> 2) + 1 : a.tile_rows;
+        const dim3 gp((a.C + TC - 1) / TC, (rows + 1023) / 1024);
+        if (a.pair_half)
+            hipLaunchKernelGGL((k_colpass_mx<PACKED, FULL, true>), gp, dim3(kColThreads), lds, s, a);
+        else
+            hipLaunchKernelGGL((k_colpass_mx<PACKED, FULL, false>), gp, dim3(kColThreads), lds, s, a);
+        return;
+    }
     if (a.pair_half) {
         // rows u <= R/4, U/2 per lane, each also giving row R/2 - u; the tile is rows [0, R/2]
         const int rows = (a.R >> 2) + 1, per_blk = 64 * (U / 2);

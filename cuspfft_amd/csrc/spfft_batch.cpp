@@ -75,6 +75,7 @@ struct spfft_batch_s {
     int mode = 0;                       // SPFFT_BATCH_*
     int req_streams = 4;                // worker streams asked for at creation
     std::vector<double> work;           // per-matrix work estimates (longest first)
+    std::vector<int64_t> nnz_hint;      // per-matrix nonzero counts given at creation (else 8 cols)
     std::vector<int64_t> coo_off_h;     // the COO offsets coo_off_d holds
     bool desc_on = false;               // some members run through the descriptors
     std::vector<int> members, fallback; // descriptor members; matrices on their own launches
@@ -277,7 +278,7 @@ spfft_status setup_desc(spfft_batch_s& b, const std::vector<double>& work) {
     for (int k = 0; k < nm; k++) {
         const int i = b.members[k];
         spfft::RowPassArgs& a = b.h_desc[k];
-        batch_descriptor(*b.plans[i], a);
+        batch_descriptor(*b.plans[i], b.nnz_hint[i], a);
         a.colstart = b.colstart.as<int>() + b.c_off[i];
         a.col_sched = b.sched.as<int>() + b.c_off[i];
         a.col_sched_off = b.sched_off.as<int>() + (size_t)(b.c_off[i] / 8) * 9;
@@ -552,6 +553,7 @@ spfft_status spfft_batch_create(int count, const int* rows, const int* cols, con
     b->count = count;
     b->plans.resize(count);
     b->out_off.resize(count);
+    b->nnz_hint.resize(count);
     std::vector<double> work(count);
     for (int i = 0; i < count; i++) {
         spfft_opts o;
@@ -565,6 +567,7 @@ spfft_status spfft_batch_create(int count, const int* rows, const int* cols, con
         b->out_off[i] = b->out_total;
         b->out_total += (int64_t)p->R * p->P;
         work[i] = work_estimate(*p, nnz_hint ? nnz_hint[i] : 8 * cols[i]);
+        b->nnz_hint[i] = nnz_hint ? nnz_hint[i] : 8 * (int64_t)cols[i];
     }
     b->mode = SPFFT_BATCH_AUTO;
     b->req_streams = streams;

@@ -43,6 +43,9 @@ bool rowpass_supports_m1(int m1);        // k_fft1 / k_ifft1 instantiated for th
 bool mfma_supports_m1(int m1);           // precision-form (BF16x3 MFMA) sub-FFT length
 bool conv_fast_supported(int r1, int r2); // k_conv_fast<r1, r2> instantiated (r2 = 1: one stage)
 int conv_max_butterflies(int r);         // butterflies per thread per k_conv_fast stage
+// colpass_rows value of the matrix-core column pass (k_colpass_mx: 8 columns x 1024
+// rows per workgroup, FP32 MFMA); the other values are k_colpass's rows per lane
+constexpr int kColpassMfma = 32;
 size_t colpass_lds_bytes(const RowPassArgs& a);   // needs a.colpass_rows and a.wr_full set
 // k_colsched: the per-tile column schedule of k_colpass (once per CSC build);
 // sizes in ints of sched (returned) and of off (*off_elems)

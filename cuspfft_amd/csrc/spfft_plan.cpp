@@ -403,7 +403,7 @@ spfft_status do_csc_build(spfft_plan_s& p, const int* d_row, const int* d_col, i
     // reduction and a table lookup; eight rows per lane (8-column tiles of 512
     // rows) once the tile has at least four such row blocks, else four
     p.colpass_rows = p.base.colpass_rows;
-    if (p.colpass_rows == 0) p.colpass_rows = std::min(p.opts.u_tile, p.R / 2 + 1) >= 2048 ? 8 : 4;
+    if (p.colpass_rows == 0) p.colpass_rows = auto_colpass_rows(p, nnz);
     SPFFT_HIP(spfft::launch_colsched(a.colstart, p.base.colperm, p.C, p.colpass_rows, p.col_sched.as<int>(),
                                      p.col_sched_off.as<int>(), s),
               "column schedule launch");
@@ -415,14 +415,26 @@ spfft_status do_csc_build(spfft_plan_s& p, const int* d_row, const int* d_col, i
 
 size_t device_lds_limit() { return g_lds_limit; }
 
+// Column-pass kernel of the automatic plan for nnz nonzeros: rows per lane from the
+// number of computed rows in a tile (the extra rows cost one complex product per nonzero
+// instead of a phase reduction and a table lookup): 8 (8-column tiles of 512 rows) once
+// a tile has at least 2048 rows, else 4; and at that size the matrix-core kernel when
+// columns average at least kAutoMfmaDegree nonzeros (measured: 14-30% faster at average
+// degree 24 for R = 3000-16706, 10-20% slower at degree 8).
+int auto_colpass_rows(const spfft_plan_s& p, int64_t nnz) {
+    const int rows = std::min(p.opts.u_tile, p.R / 2 + 1);
+    if (rows < 2048) return 4;
+    return nnz >= (int64_t)kAutoMfmaDegree * p.C ? spfft::kColpassMfma : 8;
+}
+
 // The one-tile row-pass arguments of plan p for the batched form: p.base with the
 // column-pass choices that a CSC build and row pass of this plan would resolve (rows per
 // lane, row pairing, whole W_R table); the CSC, X0 and output pointers are the caller's.
-void batch_descriptor(const spfft_plan_s& p, spfft::RowPassArgs& a) {
+void batch_descriptor(const spfft_plan_s& p, int64_t nnz, spfft::RowPassArgs& a) {
     a = p.base;
     a.P = p.P;
     int U = p.base.colpass_rows;
-    if (U == 0) U = std::min(p.opts.u_tile, p.R / 2 + 1) >= 2048 ? 8 : 4;
+    if (U == 0) U = auto_colpass_rows(p, nnz);
     a.colpass_rows = U;
     a.pair_half = pair_eligible(p) && U >= 2 ? 1 : 0;
     if (a.wr_full && spfft::colpass_lds_bytes(a) > g_lds_limit) a.wr_full = nullptr;
@@ -824,8 +836,9 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     // just below 1/R: floor(x * inv_r) is floor(x / R) or one less for x < 2^28
     a.inv_r = (float)((1.0 / (double)rows) * (1.0 - 0x1p-20));
     a.precision = mfma ? 1 : 0;
-    if (o.colpass_rows < 0 || o.colpass_rows > 16 || (o.colpass_rows & (o.colpass_rows - 1)))
-        return fail(SPFFT_ERR_INVALID_ARG, "colpass_rows must be 0, 1, 2, 4, 8 or 16");
+    if (o.colpass_rows < 0 || (o.colpass_rows > 16 && o.colpass_rows != spfft::kColpassMfma) ||
+        (o.colpass_rows & (o.colpass_rows - 1)))
+        return fail(SPFFT_ERR_INVALID_ARG, "colpass_rows must be 0, 1, 2, 4, 8, 16 or 32 (matrix cores)");
     a.colpass_rows = o.colpass_rows ? o.colpass_rows : 0;  // 0: set from the average degree at CSC build
     if (o.colpass_table < 0 || o.colpass_table > 2)
         return fail(SPFFT_ERR_INVALID_ARG, "colpass_table must be 0, 1 or 2");
@@ -834,7 +847,11 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     const bool full_fits = (size_t)rows * 8 <= budget;
     if (o.colpass_table == 2 && !full_fits)
         return fail(SPFFT_ERR_UNSUPPORTED, "W_R table does not fit in LDS");
-    p->colpass_full = o.colpass_table == 2 || (o.colpass_table == 0 && full_fits);
+    // automatic: the whole table only for short columns.  Next to the column pass's row
+    // tile (37-74 KB) a whole table of more rows leaves one workgroup per CU, and the
+    // two-level table's extra product costs less than the lost occupancy (R = 6000: column
+    // pass 0.104 ms whole, 0.041 ms two-level; R = 8000: 0.184 / 0.067 ms)
+    p->colpass_full = o.colpass_table == 2 || (o.colpass_table == 0 && full_fits && rows <= kAutoFullTableRows);
     if (a.fused) {
         a.G2 = 0;
         return SPFFT_OK;

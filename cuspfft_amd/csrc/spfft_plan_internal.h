@@ -142,6 +142,12 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts, bool own
 // Bytes of the row-pass workspace: T (tile x M) and X0 (tile x ldx), complex64.
 void workspace_bytes(const spfft_plan_s& p, size_t* t_bytes, size_t* x0_bytes);
 size_t device_lds_limit();
-void batch_descriptor(const spfft_plan_s& p, spfft::RowPassArgs& a);
+void batch_descriptor(const spfft_plan_s& p, int64_t nnz, spfft::RowPassArgs& a);
+// the automatic column-pass kernel (rows per lane, or spfft::kColpassMfma) for nnz nonzeros
+int auto_colpass_rows(const spfft_plan_s& p, int64_t nnz);
+// automatic W_R table choice: the whole table in LDS only up to this many rows
+constexpr int kAutoFullTableRows = 2048;
+// automatic column pass: the matrix-core kernel from this average column degree
+constexpr int kAutoMfmaDegree = 16;
 
 }  // namespace spfft_internal

@@ -382,7 +382,8 @@ def test_fused_row_kernel(gpu, oracle_mod, M, rows, tile):
         assert oracle_mod.rel_l2(X, X4) <= 2 * REL_L2_PLAIN
 
 
-@pytest.mark.parametrize("rows_per_lane", [1, 2, 4, 8, 16])
+# rows_per_lane 32: the matrix-core column pass (k_colpass_mx, 1024-row blocks)
+@pytest.mark.parametrize("rows_per_lane", [1, 2, 4, 8, 16, 32])
 @pytest.mark.parametrize("table", [1, 2])
 @pytest.mark.parametrize("shape", [(2310, 2310, 16000, 128), (300, 4000, 9000, 100), (4095, 33, 2000, 4095)])
 def test_colpass_variants(gpu, oracle_mod, rows_per_lane, table, shape):
@@ -402,7 +403,7 @@ def test_colpass_variants(gpu, oracle_mod, rows_per_lane, table, shape):
 # Half-period row pairing (even R, one tile of all rows): rows u <= R/4 and R/2 - u
 # from one phase lookup.  R/2 odd and even (the self-paired row u = R/4), the
 # smallest R, and int32 indices (R > 65536).
-@pytest.mark.parametrize("rows_per_lane", [2, 4, 8, 16])
+@pytest.mark.parametrize("rows_per_lane", [2, 4, 8, 16, 32])
 @pytest.mark.parametrize("table", [1, 2])
 @pytest.mark.parametrize("shape", [(2310, 2310, 16000), (300, 4000, 9000), (2, 7, 9), (4, 3, 7),
                                    (6, 40, 100), (1026, 513, 5000), (70000, 50, 3000)])
