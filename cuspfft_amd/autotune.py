@@ -24,7 +24,8 @@ from . import (Plan, SpfftError, describe_plan, fused_lengths, rader_widths, PRE
 # M1 lengths tried as alternatives to the planner's choice (register codelets
 # with a fast convolution kernel for M / M1).
 ALT_M1 = (16, 18, 24, 27, 32, 36, 40, 48, 54, 64, 20, 25, 28, 42, 45, 49, 56, 60, 63, 12, 8, 9)
-COLPASS_ROWS = (1, 2, 4, 8)
+# column-pass rows per lane; 32: the matrix-core kernel (1024-row blocks)
+COLPASS_ROWS = (1, 2, 4, 8, 32)
 # threads per workgroup tried for the on-chip row kernels that have more than one shape
 ROW_THREADS = (64, 192, 576, 704)
 # tile candidates of full-size tuning: rows of T per sub-tile (four-step kernels) and

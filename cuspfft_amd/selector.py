@@ -13,7 +13,8 @@ Features
           fraction of active columns and the 8-bin log2 column-degree histogram
   plan:   log2 M, M / (2 cols - 1), the exponents of 2, 3, 5, 7 in M, log2 M1,
           log2 M2, fast convolution kernel, convolution stages, arithmetic form,
-          column-pass rows per lane (one-hot 1 / 2 / 4 / 8, auto = all zero), fused
+          column-pass kernel (one-hot rows per lane 1 / 2 / 4 / 8 and the matrix-core
+          kernel, auto = all zero), fused
           row kernel, direct transform, Rader transform, log2 threads per workgroup
           of an on-chip row kernel (0 for the four-step kernels), log2 of the tile's
           share of the R/2 + 1 computed rows (0: one tile)
@@ -29,7 +30,7 @@ import numpy as np
 
 from . import TRANSFORM_DIRECT, TRANSFORM_RADER, _check, _vp, describe_plan, load
 
-N_FEATURES = 33
+N_FEATURES = 34
 # the network trained on the autotuner timings of the seeded training set (train_selector)
 DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "selector.json")
 
@@ -65,7 +66,7 @@ def features(rec: dict, opts: dict, info: dict | None = None, trial: bool = Fals
     f += [math.log2(M), M / (2.0 * cols - 1.0)] + _exponents(M)
     f += [math.log2(m1), math.log2(max(m2, 1)), float(info["conv_fast"]), float(info["conv_stages"]),
           float(info["precision"])]
-    f += [1.0 if cpr == r else 0.0 for r in (1, 2, 4, 8)]
+    f += [1.0 if cpr == r else 0.0 for r in (1, 2, 4, 8, 32)]
     f += [1.0 if info.get("row_kernel") == 2 else 0.0]
     nt = info.get("row_threads", 0)
     full = rows // 2 + 1
