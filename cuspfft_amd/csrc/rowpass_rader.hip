@@ -58,8 +58,23 @@ constexpr bool rader_tp(int m1, int nt, int r1, int r2, int r3) {
 constexpr size_t rader_lds_bytes_of(int m1, int nt, int r1, int r2, int r3) {
     return rader_lds_bytes(m1, nt, r1, r2, r3, rader_pad(m1, nt, r1, r2, r3), rader_tp(m1, nt, r1, r2, r3));
 }
+// L2 prefetch of the pass-1 row that the workgroup kRaderAhead places later takes (the
+// same XCD: workgroups go to the 8 XCDs in turn), by direct-to-LDS loads of one 4-byte
+// word per 128-byte line into a scratch of 64 words that all waves share (its contents
+// are never read), when that fits in LDS.  A workgroup's row loads run at the rate its
+// CU can keep lines in flight (about 17 GB/s per CU from HBM); from L2 they arrive
+// sooner.  No registers wait for these loads, and they are issued after the row's last
+// global load and barrier, so only the final wait of the workgroup includes them.
+constexpr int kRaderAhead = 256;  // c3 row kernel: 128 1.113 ms, 256 1.046, 512 1.069, 768 1.071
+constexpr size_t rader_pf_bytes(int) { return 64 * sizeof(float); }
+constexpr bool rader_pf(int m1, int nt, int r1, int r2, int r3) {
+    return rader_lds_bytes_of(m1, nt, r1, r2, r3) + rader_pf_bytes(nt) <= 160 * 1024;
+}
+constexpr size_t rader_launch_lds(int m1, int nt, int r1, int r2, int r3) {
+    return rader_lds_bytes_of(m1, nt, r1, r2, r3) + (rader_pf(m1, nt, r1, r2, r3) ? rader_pf_bytes(nt) : 0);
+}
 
-template <int M1, int NT, int R1, int R2, int R3>
+template <int M1, int NT, int R1, int R2, int R3, bool PF>
 __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx) {
     constexpr int M2 = R1 * R2 * R3;
     constexpr int L = M1 * M2;  // p - 1
@@ -159,6 +174,24 @@ __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx
         }
     }
     __syncthreads();
+    if constexpr (PF) {
+        // after the row's last global load and last barrier: no later wait but the final one
+        // includes these
+        const int nb = bx + kRaderAhead;
+        if (nb < a.tile_rows) {
+            const char* rowp = reinterpret_cast<const char*>(a.X0 + (size_t)nb * a.ldx);
+            const int lines = (a.C * 8 + 127) >> 7;
+            const int wv = __builtin_amdgcn_readfirstlane(wave);
+            float* scr = reinterpret_cast<float*>(s_red + 2 * NW);
+            for (int i = wv; i * 64 < lines; i += NW) {
+                const int line = i * 64 + lane;
+                if (line < lines)
+                    __builtin_amdgcn_global_load_lds(
+                        (const __attribute__((address_space(1))) void*)(rowp + (size_t)line * 128),
+                        (__attribute__((address_space(3))) void*)scr, 4, 0, 0);
+            }
+        }
+    }
 
     // the radix-2 combination, row u and its mirror row R - u
     typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -181,12 +214,14 @@ __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx
             }
         }
     }
+    // the prefetch writes this workgroup's LDS: it lands before the workgroup ends
+    if constexpr (PF) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 
 template <int M1, int NT, int R1, int R2, int R3>
 __global__ __launch_bounds__(NT) void k_rowrader(RowPassArgs a) {
-    rowrader_body<M1, NT, R1, R2, R3>(a, blockIdx.x);
+    rowrader_body<M1, NT, R1, R2, R3, rader_pf(M1, NT, R1, R2, R3)>(a, blockIdx.x);
 }
 
 // Batched form: work item (matrix descriptor, row of its tile) per workgroup.
@@ -194,7 +229,7 @@ template <int M1, int NT, int R1, int R2, int R3>
 __global__ __launch_bounds__(NT) void k_rowrader_batch(const RowPassArgs* __restrict__ descs,
                                                        const int2* __restrict__ work) {
     const int2 w = work[blockIdx.x];
-    rowrader_body<M1, NT, R1, R2, R3>(descs[w.x], w.y);
+    rowrader_body<M1, NT, R1, R2, R3, false>(descs[w.x], w.y);
 }
 
 // ---- precision form: the length-M1 DFTs on the matrix cores ------------------
@@ -589,15 +624,16 @@ hipError_t launch_rader(const RowPassArgs& a, hipStream_t s, bool prepare_only) 
     if (a.precision == 0 && a.M1 == m1 && a.fused_nt == nt && a.f2.radix[0] == r1 &&                     \
         (a.f2.nstages > 1 ? a.f2.radix[1] : 1) == r2 && (a.f2.nstages > 2 ? a.f2.radix[2] : 1) == r3) {   \
         constexpr size_t lds = rader_lds_bytes_of(m1, nt, r1, r2, r3);                                   \
+        constexpr size_t lds_pf = rader_launch_lds(m1, nt, r1, r2, r3);                                  \
         const void* k = reinterpret_cast<const void*>(&k_rowrader<m1, nt, r1, r2, r3>);                   \
         if (prepare_only) {                                                                              \
             const hipError_t e0 = hipFuncSetAttribute(                                                   \
                 reinterpret_cast<const void*>(&k_rowrader_batch<m1, nt, r1, r2, r3>),                    \
                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                   \
             if (e0 != hipSuccess) return e0;                                                             \
-            return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);         \
+            return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pf);      \
         }                                                                                                \
-        hipLaunchKernelGGL((k_rowrader<m1, nt, r1, r2, r3>), dim3(a.tile_rows), dim3(nt), lds, s, a);   \
+        hipLaunchKernelGGL((k_rowrader<m1, nt, r1, r2, r3>), dim3(a.tile_rows), dim3(nt), lds_pf, s, a); \
         return hipGetLastError();                                                                        \
     }
     SPFFT_RADER_LIST(SPFFT_CASE)
