@@ -52,6 +52,16 @@ enum { kFwd = 0, kInv = 1, kPivot = 2 };
 // the W_M^e table: 256 low entries and (M + 255) / 256 high ones, M <= 65536
 constexpr int kFusedWmEntries = 256 + 256;
 
+// Workgroup barrier for LDS data only: the LDS accesses before it complete and are seen
+// by every wave after it.  Unlike __syncthreads it does not wait for global memory
+// operations, so loads in flight (the row prefetch of the Rader kernel) stay in flight.
+// The row kernels exchange no data through global memory within a workgroup.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Persistent launches (resident workgroups looping over the row groups, the tables
 // loaded once) for the direct transform, whose rows are short: on p14 (16384^2) the
 // row kernel went 0.98 -> 0.84 ms.  The Bluestein rows are long enough that the
@@ -178,6 +188,40 @@ constexpr size_t fused_lds_bytes_of(int m1, int nb, int rpw, int r1, int r2, int
                 : 0);
 }
 
+// L2 prefetch of the pass-1 rows of the row group 256 workgroups later (it runs on the
+// same XCD: workgroups go to the 8 XCDs in turn), by direct-to-LDS loads of one 4-byte
+// word per 128-byte line into a 256-byte scratch past the shape's LDS that all waves
+// share (never read).  For one-batch shapes launched one workgroup per row group, when
+// the scratch fits without lowering the number of resident workgroups per CU.  Measured:
+// c2's row kernel (17496 = 24 * 27 * 27) 0.383 -> 0.368 ms; the two-batch c3 Bluestein
+// shape (34992) 1.73 -> 1.86 ms and the persistent direct p14 kernel 0.84 -> 0.89 ms, so
+// those have none: their loads of B^ (or next row group) soon after wait for it.
+constexpr int kFusedAhead = 256;
+constexpr size_t kPrefetchScratch = 64 * sizeof(float);
+constexpr bool fused_pf(int m1, int nb, int rpw, int r1, int r2, int r3) {
+    return nb == 1 && fused_lds_bytes_of(m1, nb, rpw, r1, r2, r3) + kPrefetchScratch <= 160 * 1024 &&
+           (160 * 1024) / (fused_lds_bytes_of(m1, nb, rpw, r1, r2, r3) + kPrefetchScratch) ==
+               (160 * 1024) / fused_lds_bytes_of(m1, nb, rpw, r1, r2, r3);
+}
+constexpr size_t fused_launch_lds(int m1, int nb, int rpw, int r1, int r2, int r3) {
+    return fused_lds_bytes_of(m1, nb, rpw, r1, r2, r3) + (fused_pf(m1, nb, rpw, r1, r2, r3) ? kPrefetchScratch : 0);
+}
+// rows [row0, row0 + nrows) of the tile's pass-1 buffer into L2; nw waves share the lines
+__device__ __forceinline__ void prefetch_rows(const RowPassArgs& a, int row0, int nrows, int nw, float* scr) {
+    if (row0 >= a.tile_rows) return;
+    nrows = min(nrows, a.tile_rows - row0);
+    const char* p = reinterpret_cast<const char*>(a.X0 + (size_t)row0 * a.ldx);
+    const int lines = (int)(((size_t)(nrows - 1) * a.ldx * 8 + (size_t)a.C * 8 + 127) >> 7);
+    const int lane = threadIdx.x & 63;
+    for (int i = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); i * 64 < lines; i += nw) {
+        const int line = i * 64 + lane;
+        if (line < lines)
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(p + (size_t)line * 128),
+                                             (__attribute__((address_space(3))) void*)scr, 4, 0, 0);
+    }
+}
+
+
 // W_M^e, e = k1 n2 < M, from the two-level LDS table hi[e >> 8] * lo[e & 255].
 __device__ __forceinline__ float2 wm_pow(const float2* s_wm, int e) {
     return cmul(s_wm[256 + (e >> 8)], s_wm[e & 255]);
@@ -232,7 +276,7 @@ __device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const
             for (int r = 0; r < R; r++) v[b][r] = src[RSPLIT ? lds_idx<P>(r * nb) : lds_idx<P>(j + r * nb)];
         }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int b = 0; b < MB; b++) {
         const int t = threadIdx.x + b * NT;
@@ -268,7 +312,7 @@ __device__ __forceinline__ void fused_stage(bio::rsrc_t r_bh, float2* buf, const
                 dst[WSPLIT ? lds_idx<P>(r * stride) : lds_idx<P>(base + r * stride)] = v[b][r];
         }
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // The convolution of the batch's rows: forward stages R1 .. Rn, B^, inverse
@@ -312,7 +356,7 @@ __device__ __forceinline__ void fused_forward(bio::rsrc_t r_bh, float2* buf, con
 // -- entries X[k1 + M1 n2], k1 < M1 -- and writes those of row u (v <= C/2) and of
 // its mirror row R - u (X[R-u][C - w] = conj(X[u][w])).  The column pass then runs
 // with h = 1.
-template <int M1, int NB, int RPW, int NT, int R1, int R2, int R3, bool DIRECT>
+template <int M1, int NB, int RPW, int NT, int R1, int R2, int R3, bool DIRECT, bool PF>
 __device__ __forceinline__ void rowfused_body(const RowPassArgs& a, const int bx, const int gx) {
     constexpr int M2 = R1 * R2 * R3;
     constexpr int KB = M1 / NB;
@@ -393,7 +437,14 @@ __device__ __forceinline__ void rowfused_body(const RowPassArgs& a, const int bx
             }
         }
         if (fill) fill_tables();
-        __syncthreads();
+        lds_barrier();
+        if constexpr (PF && !fused_persistent(M1, NB, RPW, R1, R2, R3, DIRECT)) {
+            // after the row group's own loads; the barriers below are LDS-only, so it stays
+            // in flight until phase B's loads of B^ wait for it
+            prefetch_rows(a, row0 + kFusedAhead * RPW, RPW, NT / 64,
+                          reinterpret_cast<float*>(reinterpret_cast<char*>(smem) +
+                                                   fused_lds_bytes_of(M1, NB, RPW, R1, R2, R3)));
+        }
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
             if (live[c]) {
@@ -422,7 +473,7 @@ __device__ __forceinline__ void rowfused_body(const RowPassArgs& a, const int bx
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
         if constexpr (DIRECT)
             fused_forward<M2, G, KB, NT, R1, R2, R3, P, TP>(r_bh, buf, s_tw, s_ta);
         else
@@ -524,11 +575,13 @@ __device__ __forceinline__ void rowfused_body(const RowPassArgs& a, const int bx
     } else {
         row_group(bx * RPW, true);
     }
+    // the prefetch writes this workgroup's LDS: it lands before the workgroup ends
+    if constexpr (PF) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 template <int M1, int NB, int RPW, int NT, int R1, int R2, int R3, bool DIRECT>
 __global__ __launch_bounds__(NT) void k_rowfused(RowPassArgs a) {
-    rowfused_body<M1, NB, RPW, NT, R1, R2, R3, DIRECT>(a, blockIdx.x, gridDim.x);
+    rowfused_body<M1, NB, RPW, NT, R1, R2, R3, DIRECT, fused_pf(M1, NB, RPW, R1, R2, R3)>(a, blockIdx.x, gridDim.x);
 }
 
 // Batched form: work item (matrix descriptor, row group) per workgroup, one row group
@@ -537,7 +590,7 @@ template <int M1, int NB, int RPW, int NT, int R1, int R2, int R3, bool DIRECT>
 __global__ __launch_bounds__(NT) void k_rowfused_batch(const RowPassArgs* __restrict__ descs,
                                                        const int2* __restrict__ work) {
     const int2 w = work[blockIdx.x];
-    rowfused_body<M1, NB, RPW, NT, R1, R2, R3, DIRECT>(descs[w.x], w.y, 0x40000000);
+    rowfused_body<M1, NB, RPW, NT, R1, R2, R3, DIRECT, false>(descs[w.x], w.y, 0x40000000);
 }
 
 // Instantiated shapes: X(M1, NB, RPW, NT, R1, R2, R3), M = M1 * R1 * R2 * R3 (a
@@ -624,6 +677,7 @@ hipError_t launch_fused_group_h(const RowPassArgs& a, hipStream_t s, bool prepar
         a.f2.radix[0] == r1 && (a.f2.nstages > 1 ? a.f2.radix[1] : 1) == r2 &&                      \
         (a.f2.nstages > 2 ? a.f2.radix[2] : 1) == r3) {                                             \
         constexpr size_t lds = fused_lds_bytes_of(m1, nb, rpw, r1, r2, r3);                         \
+        constexpr size_t lds_k = fused_launch_lds(m1, nb, rpw, r1, r2, r3);                         \
         auto* k = &k_rowfused<m1, nb, rpw, nt, r1, r2, r3, false>;                                  \
         if (a.direct) {                                                                             \
             if constexpr (nb == 1) k = &k_rowfused<m1, nb, rpw, nt, r1, r2, r3, nb == 1>;           \
@@ -638,7 +692,7 @@ hipError_t launch_fused_group_h(const RowPassArgs& a, hipStream_t s, bool prepar
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
             if (e0 != hipSuccess) return e0;                                                        \
             return hipFuncSetAttribute(reinterpret_cast<const void*>(k),                            \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);       \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_k);     \
         }                                                                                           \
         if (batch) {                                                                                \
             auto* kb = &k_rowfused_batch<m1, nb, rpw, nt, r1, r2, r3, false>;                       \
@@ -656,14 +710,14 @@ hipError_t launch_fused_group_h(const RowPassArgs& a, hipStream_t s, bool prepar
             int o = occ[a.direct ? 1 : 0].load(std::memory_order_relaxed);                          \
             if (!o) {                                                                               \
                 int n = 0;                                                                          \
-                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, nt, lds) != hipSuccess)     \
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, nt, lds_k) != hipSuccess)   \
                     n = 1;                                                                          \
                 o = std::max(n, 1);                                                                 \
                 occ[a.direct ? 1 : 0].store(o, std::memory_order_relaxed);                          \
             }                                                                                       \
             grid = std::min(nblk, a.fused_cus * o);                                                 \
         }                                                                                           \
-        hipLaunchKernelGGL(k, dim3(grid), dim3(nt), lds, s, a);                                     \
+        hipLaunchKernelGGL(k, dim3(grid), dim3(nt), lds_k, s, a);                                   \
         return hipGetLastError();                                                                   \
     }
 

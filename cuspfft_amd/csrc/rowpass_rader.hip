@@ -74,6 +74,21 @@ constexpr size_t rader_launch_lds(int m1, int nt, int r1, int r2, int r3) {
     return rader_lds_bytes_of(m1, nt, r1, r2, r3) + (rader_pf(m1, nt, r1, r2, r3) ? rader_pf_bytes(nt) : 0);
 }
 
+// The L2 prefetch (kRaderAhead) of pass-1 row bx + kRaderAhead of the tile.
+template <int NW>
+__device__ __forceinline__ void rader_prefetch(const RowPassArgs& a, int bx, int wave, int lane, float* scr) {
+    const int nb = bx + kRaderAhead;
+    if (nb >= a.tile_rows) return;
+    const char* rowp = reinterpret_cast<const char*>(a.X0 + (size_t)nb * a.ldx);
+    const int lines = (a.C * 8 + 127) >> 7;
+    for (int i = __builtin_amdgcn_readfirstlane(wave); i * 64 < lines; i += NW) {
+        const int line = i * 64 + lane;
+        if (line < lines)
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(rowp + (size_t)line * 128),
+                                             (__attribute__((address_space(3))) void*)scr, 4, 0, 0);
+    }
+}
+
 template <int M1, int NT, int R1, int R2, int R3, bool PF>
 __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx) {
     constexpr int M2 = R1 * R2 * R3;
@@ -121,7 +136,10 @@ __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx
         for (int i = t; i < NA; i += NT) s_ta[i] = a.tw2[(i / nsa + 1) * (i % nsa)];
         for (int i = t; i < NBT; i += NT) s_tb[i] = a.tw2[(i / nsb + 1) * (i % nsb)];
     }
-    __syncthreads();
+    lds_barrier();
+    // after the row's own loads (whose waits it does not join); the barriers below are
+    // LDS-only, so it stays in flight until the pivot stage's loads of B^ wait for it
+    if constexpr (PF) rader_prefetch<NW>(a, bx, wave, lane, reinterpret_cast<float*>(s_red + 2 * NW));
     if (live) RegFft<M1>::run(v, false);
     {
         // sum_q a_s[q] = sum over the columns of the codelet's output 0: per wave and row
@@ -148,7 +166,7 @@ __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx
 #pragma unroll
         for (int k = 0; k < KB; k++) col[k * LP] = v[k];
     }
-    __syncthreads();
+    lds_barrier();
     fused_conv<M2, G, KB, NT, R1, R2, R3, P, TP>(r_bh, buf, s_tw, 0, s_ta, s_tb);
 
     // phase C: the inverse length-M1 codelet gives c_s[m], m = n2 + M2 n1
@@ -162,7 +180,7 @@ __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx
 #pragma unroll
         for (int n1 = 0; n1 < M1; n1++) kk[n1] = a.rader_k[n2 + M2 * n1];
     }
-    __syncthreads();  // the rows are read: buf now takes E_0 | E_1 in natural order
+    lds_barrier();  // the rows are read: buf now takes E_0 | E_1 in natural order
     if (live) {
         float2* e = buf + s * p;
 #pragma unroll
@@ -173,26 +191,7 @@ __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx
             e[0] = sum;
         }
     }
-    __syncthreads();
-    if constexpr (PF) {
-        // after the row's last global load and last barrier: no later wait but the final one
-        // includes these
-        const int nb = bx + kRaderAhead;
-        if (nb < a.tile_rows) {
-            const char* rowp = reinterpret_cast<const char*>(a.X0 + (size_t)nb * a.ldx);
-            const int lines = (a.C * 8 + 127) >> 7;
-            const int wv = __builtin_amdgcn_readfirstlane(wave);
-            float* scr = reinterpret_cast<float*>(s_red + 2 * NW);
-            for (int i = wv; i * 64 < lines; i += NW) {
-                const int line = i * 64 + lane;
-                if (line < lines)
-                    __builtin_amdgcn_global_load_lds(
-                        (const __attribute__((address_space(1))) void*)(rowp + (size_t)line * 128),
-                        (__attribute__((address_space(3))) void*)scr, 4, 0, 0);
-            }
-        }
-    }
-
+    lds_barrier();
     // the radix-2 combination, row u and its mirror row R - u
     typedef float f32x2 __attribute__((ext_vector_type(2)));
     const int u = a.u_base + bx;
@@ -336,7 +335,7 @@ __device__ __forceinline__ void dft32_mfma(const DftFrags& f, const ColFrags& x,
 }
 #undef SPFFT_MFMA3
 
-template <int M1, int NT, int R1, int R2, int R3>
+template <int M1, int NT, int R1, int R2, int R3, bool PF>
 __device__ __forceinline__ void rowrader_mfma_body(const RowPassArgs& a, const int bx) {
     constexpr int M2 = R1 * R2 * R3;
     constexpr int L = M1 * M2;  // p - 1
@@ -393,7 +392,7 @@ __device__ __forceinline__ void rowrader_mfma_body(const RowPassArgs& a, const i
     }
     DftFrags f;
     load_dft_frags(frag_tab, lane, f);
-    __syncthreads();
+    lds_barrier();
     float2 psum0 = make_float2(0.f, 0.f), psum1 = make_float2(0.f, 0.f);
 #pragma unroll
     for (int k = 0; k < TPW; k++) {
@@ -443,7 +442,7 @@ __device__ __forceinline__ void rowrader_mfma_body(const RowPassArgs& a, const i
         s_red[2 * wave] = psum0;
         s_red[2 * wave + 1] = psum1;
     }
-    __syncthreads();
+    lds_barrier();
 
     // phase B: the convolution of both row groups with B^
     fused_conv<M2, G, KB, NT, R1, R2, R3, P, TP>(r_bh, buf, s_tw, 0, s_ta, s_tb);
@@ -485,7 +484,7 @@ __device__ __forceinline__ void rowrader_mfma_body(const RowPassArgs& a, const i
         split_col(xc, xcs[k]);
         __builtin_amdgcn_sched_barrier(0);  // one tile's loads in flight at a time
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int k = 0; k < TPW; k++) {
         const int tile = wave + k * NW;
@@ -511,7 +510,9 @@ __device__ __forceinline__ void rowrader_mfma_body(const RowPassArgs& a, const i
             e[0] = sum;
         }
     }
-    __syncthreads();
+    lds_barrier();
+
+    if constexpr (PF) rader_prefetch<NW>(a, bx, wave, lane, reinterpret_cast<float*>(s_red + 2 * NW));
 
     // the radix-2 combination, row u and its mirror row R - u
     typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -534,18 +535,19 @@ __device__ __forceinline__ void rowrader_mfma_body(const RowPassArgs& a, const i
             }
         }
     }
+    if constexpr (PF) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 template <int M1, int NT, int R1, int R2, int R3>
 __global__ __launch_bounds__(NT) void k_rowrader_mfma(RowPassArgs a) {
-    rowrader_mfma_body<M1, NT, R1, R2, R3>(a, blockIdx.x);
+    rowrader_mfma_body<M1, NT, R1, R2, R3, rader_pf(M1, NT, R1, R2, R3)>(a, blockIdx.x);
 }
 
 template <int M1, int NT, int R1, int R2, int R3>
 __global__ __launch_bounds__(NT) void k_rowrader_mfma_batch(const RowPassArgs* __restrict__ descs,
                                                             const int2* __restrict__ work) {
     const int2 w = work[blockIdx.x];
-    rowrader_mfma_body<M1, NT, R1, R2, R3>(descs[w.x], w.y);
+    rowrader_mfma_body<M1, NT, R1, R2, R3, false>(descs[w.x], w.y);
 }
 
 }  // namespace
@@ -607,15 +609,16 @@ hipError_t launch_rader(const RowPassArgs& a, hipStream_t s, bool prepare_only) 
     if (a.precision == 1 && a.M1 == m1 && a.fused_nt == nt && a.f2.radix[0] == r1 &&                     \
         (a.f2.nstages > 1 ? a.f2.radix[1] : 1) == r2 && (a.f2.nstages > 2 ? a.f2.radix[2] : 1) == r3) {   \
         constexpr size_t lds = rader_lds_bytes_of(m1, nt, r1, r2, r3);                                   \
+        constexpr size_t lds_pf = rader_launch_lds(m1, nt, r1, r2, r3);                                  \
         const void* k = reinterpret_cast<const void*>(&k_rowrader_mfma<m1, nt, r1, r2, r3>);              \
         if (prepare_only) {                                                                              \
             const hipError_t e0 = hipFuncSetAttribute(                                                   \
                 reinterpret_cast<const void*>(&k_rowrader_mfma_batch<m1, nt, r1, r2, r3>),               \
                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                   \
             if (e0 != hipSuccess) return e0;                                                             \
-            return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);         \
+            return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pf);      \
         }                                                                                                \
-        hipLaunchKernelGGL((k_rowrader_mfma<m1, nt, r1, r2, r3>), dim3(a.tile_rows), dim3(nt), lds, s, a); \
+        hipLaunchKernelGGL((k_rowrader_mfma<m1, nt, r1, r2, r3>), dim3(a.tile_rows), dim3(nt), lds_pf, s, a); \
         return hipGetLastError();                                                                        \
     }
     SPFFT_RADER_MFMA_LIST(SPFFT_CASE)
