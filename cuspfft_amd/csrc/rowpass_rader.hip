@@ -1,21 +1,29 @@
 // rowpass_rader.hip -- k_rowrader: the row pass of an even width C = 2p, p an odd
 // prime, by Rader's algorithm instead of a Bluestein convolution.
 //
-// The row DFT splits by column parity (x_s[j] = X0[u][2j + s], s = 0, 1):
-//   Y[v] = E_0[v mod p] + W_C^v E_1[v mod p],   E_s = DFT_p(x_s),
-// and each length-p DFT of a prime p is, with a primitive root g of p, a cyclic
-// convolution of length L = p - 1 (Rader):
+// The row DFT of length 2p is a 2 x p Good-Thomas (prime-factor) transform: with
+// x_s[j] = X0[u][(p s + 2 j) mod 2p] (s = 0, 1; the even columns, then the odd ones in
+// the order p, p + 2, ...), and E_s = DFT_p(x_s),
+//   Y[k] = E_0[k mod p] + (-1)^k E_1[k mod p],
+// with no twiddle factors.  Each length-p DFT of a prime p is, with a primitive root
+// g of p, a cyclic convolution of length L = p - 1 (Rader):
 //   a_s[q] = x_s[g^q],  b[q] = W_p^{g^-q}   (q < L)
 //   E_s[g^-m] = x_s[0] + (a_s (*) b)[m],    E_s[0] = x_s[0] + sum_q a_s[q].
-// The column pass writes X0 in Rader order (position s L + q holds column
-// 2 g^q + s, positions 2L and 2L + 1 hold columns 0 and 1), so a_s is one
-// contiguous run of the pass-1 row.  One workgroup per output row u runs both
-// convolutions on chip as the two row groups of k_rowfused's four-step split
-// L = M1 M2 (phase A: length-M1 register codelet and W_L twiddles; phase B: the
-// Stockham stages through LDS with B^ = DFT_L(b) / L at the pivot; phase C: the
-// inverse), then scatters E_s to natural order in LDS and writes
-//   row u:      X[u][k] = E_0[k] + W_C^k E_1[k]      (k < p),  X[u][p] = E_0[0] - E_1[0]
-//   row R - u:  X[R-u][p - k] = conj(E_0[k] - W_C^k E_1[k]),   X[R-u][0] = conj(X[u][0])
+// The convolution runs as DFT_L, a product with B^ = DFT_L(b) / L, and the inverse
+// DFT_L, each again a prime-factor transform L = M1 x M2 (gcd(M1, M2) = 1): entry
+// (n1, n2) is a_s[(M2 n1 + M1 n2) mod L] and frequency (k1, k2) is
+// (M2 (M2^-1 mod M1) k1 + M1 (M1^-1 mod M2) k2) mod L, so that
+//   DFT_L = DFT_M1 over n1 (x) DFT_M2 over n2
+// with no twiddles between the two.  The column pass writes X0 in this order
+// (position s L + n2 + M2 n1 holds the column of a_s at (n1, n2); positions 2L and
+// 2L + 1 hold columns 0 and p), the host stores B^ and the scatter indices g^-m in
+// it, so the kernel only sees contiguous runs.  One workgroup per output row u runs
+// both convolutions on chip as the two row groups of k_rowfused's split (phase A:
+// length-M1 register codelet; phase B: the Stockham stages of length M2 through LDS
+// with B^ at the pivot; phase C: the inverse codelet), then scatters E_s to natural
+// order in LDS and writes
+//   row u:      X[u][k] = E_0[k] + (-1)^k E_1[k]      (k < p),  X[u][p] = E_0[0] - E_1[0]
+//   row R - u:  X[R-u][p - k] = conj(E_0[k] - (-1)^k E_1[k]),   X[R-u][0] = conj(X[u][0])
 // For c3's width 16706 = 2 * 8353 the convolutions have length 8352 = 29 * 288
 // (two of them per row) against one of length 34992 for Bluestein.
 //
@@ -30,15 +38,14 @@ namespace {
 // LDS entries (float2) of the row buffer: the two row groups of the convolution,
 // or afterwards E_0 and E_1 in natural order (2p = 2L + 2 entries)
 constexpr int rader_buf(int m1, int m2, int lp) { return 2 * m1 * lp > 2 * m1 * m2 + 2 ? 2 * m1 * lp : 2 * m1 * m2 + 2; }
-// two-level tables of W_n^e, e < n: 256 low entries and ceil(n / 256) high ones
-constexpr int rader_w2(int n) { return 256 + (n + 255) / 256; }
+constexpr int rader_gcd(int a, int b) { return b ? rader_gcd(b, a % b) : a; }
 constexpr size_t rader_lds_bytes(int m1, int nt, int r1, int r2, int r3, int pad, bool tp) {
     const int m2 = r1 * r2 * r3, L = m1 * m2;
     const int lp = pad == 2 ? m2 + m2 / 8 : pad ? m2 + m2 / 32 : m2;
     const int tabs = tp ? fused_tp_a(r1, r2, r3) + fused_tp_b(r1, r2, r3) : 0;
-    // W_M2, W_L, the r-major tables, the row buffer, W_C (C = 2p = 2L + 2), the partial sums
-    return (size_t)(m2 + rader_w2(L) + tabs + rader_buf(m1, m2, lp) + rader_w2(L + 1) + 2 * (nt / 64)) *
-           sizeof(float2);
+    (void)L;
+    // W_M2, the r-major tables, the row buffer, the partial sums
+    return (size_t)(m2 + tabs + rader_buf(m1, m2, lp) + 2 * (nt / 64)) * sizeof(float2);
 }
 // Row padding: 1 (entry i at i + i/32) where every stage keeps its compile-time
 // offsets, else 2 (i + i/8: radix-8 stages write at a stride of 9 entries instead of
@@ -89,6 +96,34 @@ __device__ __forceinline__ void rader_prefetch(const RowPassArgs& a, int bx, int
     }
 }
 
+// The 2 x p combination from E_0 | E_1 (natural order in buf): row u and its
+// mirror row R - u, with the sign (-1)^k of the prime-factor split.
+template <int NT>
+__device__ __forceinline__ void rader_combine(const RowPassArgs& a, const int bx, const float2* buf) {
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    const int p = a.rader_p;
+    const int u = a.u_base + bx;
+    const bool mirror = u > 0 && a.R - u > u;
+    f32x2* o = reinterpret_cast<f32x2*>(a.out + (size_t)u * a.P);
+    f32x2* om = reinterpret_cast<f32x2*>(mirror ? a.out_m + (size_t)(a.R - u) * a.P : a.out);
+    for (int k = threadIdx.x; k < p; k += NT) {
+        const float2 e0 = buf[k], f1 = buf[p + k];
+        const float2 e1 = (k & 1) ? make_float2(-f1.x, -f1.y) : f1;
+        const f32x2 y0 = {e0.x + e1.x, e0.y + e1.y};
+        const f32x2 y1 = {e0.x - e1.x, e0.y - e1.y};
+        __builtin_nontemporal_store(y0, o + k);
+        if (k == 0) __builtin_nontemporal_store(y1, o + p);
+        if (mirror) {
+            const f32x2 c1 = {y1[0], -y1[1]};
+            __builtin_nontemporal_store(c1, om + (p - k));
+            if (k == 0) {
+                const f32x2 c0 = {y0[0], -y0[1]};
+                __builtin_nontemporal_store(c0, om);
+            }
+        }
+    }
+}
+
 template <int M1, int NT, int R1, int R2, int R3, bool PF>
 __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx) {
     constexpr int M2 = R1 * R2 * R3;
@@ -98,6 +133,7 @@ __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx
     constexpr int NW = NT / 64;
     static_assert(NT % 64 == 0 && 2 * M2 <= NT, "one column n2 of one row group per thread");
     static_assert(R1 > 1 && (R2 > 1 || R3 == 1), "stage radices first");
+    static_assert(rader_gcd(M1, M2) == 1, "the prime-factor split L = M1 x M2 needs coprime factors");
     constexpr int P = rader_pad(M1, NT, R1, R2, R3);
     constexpr int LP = kRowPitch<M2, P>;
     constexpr int TP = rader_tp(M1, NT, R1, R2, R3) ? 1 : 0;
@@ -106,12 +142,10 @@ __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx
     constexpr int NBUF = rader_buf(M1, M2, LP);
     extern __shared__ float2 smem[];
     float2* s_tw = smem;                    // W_M2^i, i < M2
-    float2* s_wm = s_tw + M2;               // W_L^e: 256 low + (L + 255) / 256 high entries
-    float2* s_ta = s_wm + rader_w2(L);      // TP: r-major tables of the pivot stage ...
+    float2* s_ta = s_tw + M2;               // TP: r-major tables of the pivot stage ...
     float2* s_tb = NBT ? s_ta + NA : s_ta;  // ... and of the last inverse stage
     float2* buf = s_ta + NA + NBT;          // [s][k1][n2] rows, then E_0 | E_1
-    float2* s_wc = buf + NBUF;              // W_C^e: 256 low + (p + 255) / 256 high entries
-    float2* s_red = s_wc + rader_w2(L + 1); // [wave][s]: partial sums of a_s
+    float2* s_red = buf + NBUF;             // [wave][s]: partial sums of a_s
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const int s = t / M2, n2 = t - s * M2;  // row group and column (t < 2 M2)
@@ -119,18 +153,14 @@ __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx
     const int p = a.rader_p;
     const bio::rsrc_t r_bh = bio::make_rsrc(a.bhat, (uint32_t)L * 8u);
 
-    // phase A: a_s[n2 + M2 n1] (n1 < M1) and x_s[0]; the tables while they load
+    // phase A: the entries (n1, n2), n1 < M1, of a_s; the tables while they load
     const bio::rsrc_t r_x = bio::make_rsrc(a.X0 + (size_t)bx * a.ldx, (uint32_t)a.C * 8u);
     float2 v[M1];
-    float2 x0 = make_float2(0.f, 0.f);
     if (live) {
 #pragma unroll
         for (int n1 = 0; n1 < M1; n1++) v[n1] = bio::load2_stream(r_x, (s * L + n2) * 8, M2 * n1 * 8);
-        x0 = bio::load2_stream(r_x, (2 * L + s) * 8, 0);
     }
     for (int i = t; i < M2; i += NT) s_tw[i] = a.tw2[i];
-    for (int i = t; i < 256 + (L + 255) / 256; i += NT) s_wm[i] = a.wm2[i];
-    for (int i = t; i < 256 + (p + 255) / 256; i += NT) s_wc[i] = a.wc2[i];
     if constexpr (TP) {
         constexpr int nsa = NA ? M2 / RA : 1, nsb = NBT ? M2 / R1 : 1;
         for (int i = t; i < NA; i += NT) s_ta[i] = a.tw2[(i / nsa + 1) * (i % nsa)];
@@ -158,9 +188,8 @@ __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx
             s_red[2 * wave + 1] = p1;
         }
     }
-    if (live) fused_twiddle<M1, false>(v, n2, s_wm);
 
-    // phase B: both row groups' rows k1 into LDS, the convolution with B^
+    // phase B (no twiddles between the length-M1 and length-M2 DFTs: prime-factor split): both row groups' rows k1 into LDS, the convolution with B^
     if (live) {
         float2* col = buf + s * KB * LP + lds_idx<P>(n2);
 #pragma unroll
@@ -169,13 +198,16 @@ __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx
     lds_barrier();
     fused_conv<M2, G, KB, NT, R1, R2, R3, P, TP>(r_bh, buf, s_tw, 0, s_ta, s_tb);
 
-    // phase C: the inverse length-M1 codelet gives c_s[m], m = n2 + M2 n1
+    // phase C: the inverse length-M1 codelet gives x_s[0] + c_s[m], m = (M2 n1 + M1 n2)
+    // mod L (x_s[0] added to the codelet's input 0 reaches each of its outputs)
     int kk[M1];  // g^-m mod p
+    float2 x0 = make_float2(0.f, 0.f);
     if (live) {
+        x0 = bio::load2(r_x, (2 * L + s) * 8, 0);
         const float2* col = buf + s * KB * LP + lds_idx<P>(n2);
 #pragma unroll
         for (int k = 0; k < KB; k++) v[k] = col[k * LP];
-        fused_twiddle<M1, true>(v, n2, s_wm);
+        v[0] = cadd(v[0], x0);
         RegFft<M1>::run(v, true);
 #pragma unroll
         for (int n1 = 0; n1 < M1; n1++) kk[n1] = a.rader_k[n2 + M2 * n1];
@@ -184,7 +216,7 @@ __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx
     if (live) {
         float2* e = buf + s * p;
 #pragma unroll
-        for (int n1 = 0; n1 < M1; n1++) e[kk[n1]] = cadd(x0, v[n1]);
+        for (int n1 = 0; n1 < M1; n1++) e[kk[n1]] = v[n1];
         if (n2 == 0) {
             float2 sum = x0;
             for (int w = 0; w < NW; w++) sum = cadd(sum, s_red[2 * w + s]);
@@ -192,27 +224,7 @@ __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx
         }
     }
     lds_barrier();
-    // the radix-2 combination, row u and its mirror row R - u
-    typedef float f32x2 __attribute__((ext_vector_type(2)));
-    const int u = a.u_base + bx;
-    const bool mirror = u > 0 && a.R - u > u;
-    f32x2* o = reinterpret_cast<f32x2*>(a.out + (size_t)u * a.P);
-    f32x2* om = reinterpret_cast<f32x2*>(mirror ? a.out_m + (size_t)(a.R - u) * a.P : a.out);
-    for (int k = t; k < p; k += NT) {
-        const float2 e0 = buf[k], e1 = cmul(buf[p + k], cmul(s_wc[256 + (k >> 8)], s_wc[k & 255]));
-        const f32x2 y0 = {e0.x + e1.x, e0.y + e1.y};
-        const f32x2 y1 = {e0.x - e1.x, e0.y - e1.y};
-        __builtin_nontemporal_store(y0, o + k);
-        if (k == 0) __builtin_nontemporal_store(y1, o + p);
-        if (mirror) {
-            const f32x2 c1 = {y1[0], -y1[1]};
-            __builtin_nontemporal_store(c1, om + (p - k));
-            if (k == 0) {
-                const f32x2 c0 = {y0[0], -y0[1]};
-                __builtin_nontemporal_store(c0, om);
-            }
-        }
-    }
+    rader_combine<NT>(a, bx, buf);
     // the prefetch writes this workgroup's LDS: it lands before the workgroup ends
     if constexpr (PF) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
@@ -347,6 +359,7 @@ __device__ __forceinline__ void rowrader_mfma_body(const RowPassArgs& a, const i
     constexpr int TPW = (NTILE + NW - 1) / NW;  // tiles per wave (at most)
     static_assert(M1 <= 32 && M1 > 16, "one 32-point DFT matrix, two k-steps");
     static_assert(NT % 64 == 0, "whole waves");
+    static_assert(rader_gcd(M1, M2) == 1, "the prime-factor split L = M1 x M2 needs coprime factors");
     constexpr int P = rader_pad(M1, NT, R1, R2, R3);
     constexpr int LP = kRowPitch<M2, P>;
     constexpr int TP = rader_tp(M1, NT, R1, R2, R3) ? 1 : 0;
@@ -355,12 +368,10 @@ __device__ __forceinline__ void rowrader_mfma_body(const RowPassArgs& a, const i
     constexpr int NBUF = rader_buf(M1, M2, LP);
     extern __shared__ float2 smem[];
     float2* s_tw = smem;
-    float2* s_wm = s_tw + M2;
-    float2* s_ta = s_wm + rader_w2(L);
+    float2* s_ta = s_tw + M2;
     float2* s_tb = NBT ? s_ta + NA : s_ta;
     float2* buf = s_ta + NA + NBT;
-    float2* s_wc = buf + NBUF;
-    float2* s_red = s_wc + rader_w2(L + 1);
+    float2* s_red = buf + NBUF;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6, h = lane >> 5, jl = lane & 31;
     const int p = a.rader_p;
@@ -383,8 +394,6 @@ __device__ __forceinline__ void rowrader_mfma_body(const RowPassArgs& a, const i
         }
     }
     for (int i = t; i < M2; i += NT) s_tw[i] = a.tw2[i];
-    for (int i = t; i < 256 + (L + 255) / 256; i += NT) s_wm[i] = a.wm2[i];
-    for (int i = t; i < 256 + (p + 255) / 256; i += NT) s_wc[i] = a.wc2[i];
     if constexpr (TP) {
         constexpr int nsa = NA ? M2 / RA : 1, nsb = NBT ? M2 / R1 : 1;
         for (int i = t; i < NA; i += NT) s_ta[i] = a.tw2[(i / nsa + 1) * (i % nsa)];
@@ -411,23 +420,14 @@ __device__ __forceinline__ void rowrader_mfma_body(const RowPassArgs& a, const i
             if (s == 0) psum0 = cadd(psum0, y0);
             else psum1 = cadd(psum1, y0);
         }
-        // W_L^{k1 n2} for k1 = 4 (2 g + h) + i: a lookup per i and per g, a product each
-        float2 wi[4], wg[4];
-        wi[0] = make_float2(1.f, 0.f);
-#pragma unroll
-        for (int i = 1; i < 4; i++) wi[i] = wm_pow(s_wm, i * n2);
-#pragma unroll
-        for (int g = 0; g < 4; g++) wg[g] = wm_pow(s_wm, 4 * (2 * g + h) * n2);
+        // rows k1 = 4 (2 g + h) + i (no twiddles: prime-factor split)
         float2* col = buf + s * KB * LP + lds_idx<P>(n2);
 #pragma unroll
         for (int g = 0; g < 4; g++)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int k1 = 8 * g + 4 * h + i;
-                if (lv && k1 < M1) {
-                    const float2 w = i == 0 ? wg[g] : cmul(wg[g], wi[i]);
-                    col[k1 * LP] = cmul(make_float2(yr[4 * g + i], yi[4 * g + i]), w);
-                }
+                if (lv && k1 < M1) col[k1 * LP] = make_float2(yr[4 * g + i], yi[4 * g + i]);
             }
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -447,8 +447,8 @@ __device__ __forceinline__ void rowrader_mfma_body(const RowPassArgs& a, const i
     // phase B: the convolution of both row groups with B^
     fused_conv<M2, G, KB, NT, R1, R2, R3, P, TP>(r_bh, buf, s_tw, 0, s_ta, s_tb);
 
-    // phase C: the wave's columns back from LDS, times W_L^{-k1 n2}, split into BF16
-    // parts (held across the barrier: buf then takes E_0 | E_1 in natural order)
+    // phase C: the wave's columns back from LDS, split into BF16 parts (held across
+    // the barrier: buf then takes E_0 | E_1 in natural order)
     ColFrags xcs[TPW];
 #pragma unroll
     for (int k = 0; k < TPW; k++) {
@@ -458,24 +458,13 @@ __device__ __forceinline__ void rowrader_mfma_body(const RowPassArgs& a, const i
         const int s = q >= M2 ? 1 : 0, n2 = q - s * M2;
         float2 xc[16];
         if (tile < NTILE) {
-            // rows k1 = 8 (2 ks + h) + j: W_L^{k1 n2} = W^{8 (2 ks + h) n2} W^{j n2}
-            float2 wj[8], wk[2];
-            wj[0] = make_float2(1.f, 0.f);
-#pragma unroll
-            for (int j = 1; j < 8; j++) wj[j] = wm_pow(s_wm, j * n2);
-#pragma unroll
-            for (int ks = 0; ks < 2; ks++) wk[ks] = wm_pow(s_wm, 8 * (2 * ks + h) * n2);
+            // rows k1 = 8 (2 ks + h) + j
             const float2* col = buf + s * KB * LP + lds_idx<P>(n2);
 #pragma unroll
             for (int e = 0; e < 16; e++) {
                 const int ks = e >> 3, j = e & 7;
                 const int k1 = 16 * ks + 8 * h + j;
-                float2 v = make_float2(0.f, 0.f);
-                if (lv && k1 < M1) {
-                    const float2 w = j == 0 ? wk[ks] : cmul(wk[ks], wj[j]);
-                    v = cmulc(col[k1 * LP], w);
-                }
-                xc[e] = v;
+                xc[e] = lv && k1 < M1 ? col[k1 * LP] : make_float2(0.f, 0.f);
             }
         } else {
 #pragma unroll
@@ -495,7 +484,7 @@ __device__ __forceinline__ void rowrader_mfma_body(const RowPassArgs& a, const i
         const float2 x0 = bio::load2(r_x, lv ? (2 * L + s) * 8 : bio::kOob, 0);
         floatx16 yr, yi;
         dft32_mfma<true>(f, xcs[k], yr, yi);
-        // c_s[m], m = n2 + M2 m1 (m1 = 8 g + 4 h + i): E_s[g^-m] = x_s[0] + c_s[m]
+        // c_s[m], m = (M2 m1 + M1 n2) mod L (m1 = 8 g + 4 h + i): E_s[g^-m] = x_s[0] + c_s[m]
         float2* e = buf + s * p;
 #pragma unroll
         for (int g = 0; g < 4; g++)
@@ -514,27 +503,7 @@ __device__ __forceinline__ void rowrader_mfma_body(const RowPassArgs& a, const i
 
     if constexpr (PF) rader_prefetch<NW>(a, bx, wave, lane, reinterpret_cast<float*>(s_red + 2 * NW));
 
-    // the radix-2 combination, row u and its mirror row R - u
-    typedef float f32x2 __attribute__((ext_vector_type(2)));
-    const int u = a.u_base + bx;
-    const bool mirror = u > 0 && a.R - u > u;
-    f32x2* o = reinterpret_cast<f32x2*>(a.out + (size_t)u * a.P);
-    f32x2* om = reinterpret_cast<f32x2*>(mirror ? a.out_m + (size_t)(a.R - u) * a.P : a.out);
-    for (int k = t; k < p; k += NT) {
-        const float2 e0 = buf[k], e1 = cmul(buf[p + k], cmul(s_wc[256 + (k >> 8)], s_wc[k & 255]));
-        const f32x2 y0 = {e0.x + e1.x, e0.y + e1.y};
-        const f32x2 y1 = {e0.x - e1.x, e0.y - e1.y};
-        __builtin_nontemporal_store(y0, o + k);
-        if (k == 0) __builtin_nontemporal_store(y1, o + p);
-        if (mirror) {
-            const f32x2 c1 = {y1[0], -y1[1]};
-            __builtin_nontemporal_store(c1, om + (p - k));
-            if (k == 0) {
-                const f32x2 c0 = {y0[0], -y0[1]};
-                __builtin_nontemporal_store(c0, om);
-            }
-        }
-    }
+    rader_combine<NT>(a, bx, buf);
     if constexpr (PF) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 

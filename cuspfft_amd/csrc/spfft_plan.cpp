@@ -17,6 +17,7 @@
 #include <mutex>
 #include <memory>
 #include <new>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -196,6 +197,16 @@ long long pow_mod(long long b, long long e, long long m) {
     for (; e > 0; e >>= 1, b = b * b % m)
         if (e & 1) r = r * b % m;
     return r;
+}
+
+// a^-1 mod m for gcd(a, m) = 1 (extended Euclid); 0 for m = 1.
+long long inv_mod(long long a, long long m) {
+    long long r0 = m, r1 = ((a % m) + m) % m, s0 = 0, s1 = 1;
+    while (r1 != 0) {
+        const long long q = r0 / r1, r2 = r0 - q * r1, s2 = s0 - q * s1;
+        r0 = r1; r1 = r2; s0 = s1; s1 = s2;
+    }
+    return m == 1 ? 0 : ((s0 % m) + m) % m;
 }
 
 // The smallest primitive root g of the prime p: g^((p-1)/q) != 1 for every prime q | p - 1.
@@ -959,37 +970,46 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
         for (int n2 = 0; n2 < p->M2; n2++)
             tm[(size_t)k1 * p->M2 + n2] = to_f2(unit_root((long long)k1 * n2, M, -1));
     if (a.rader) {
-        // g a primitive root of p, L = p - 1 = M: kt[m] = g^-m mod p; b[q] = W_p^{g^-q};
-        // X0 position s L + q holds column 2 g^q + s, positions 2L, 2L + 1 columns 0, 1;
-        // W_C two-level table for the combination of the halves
+        // g a primitive root of p, L = p - 1 = M = M1 M2 (coprime): the prime-factor
+        // orders of the kernel (rowpass_rader.hip).  Entry (n1, n2) of a convolution
+        // operand, X0 position s L + n2 + M2 n1, is a_s[q], q = (M2 n1 + M1 n2) mod L,
+        // i.e. column (p s + 2 g^q) mod 2p; positions 2L, 2L + 1 hold columns 0 and p.
+        // kt[n2 + M2 n1] = g^-q mod p for the same q; b[q] = W_p^{g^-q}; frequency
+        // (k1, k2) of B^ is (c1 k1 + c2 k2) mod L, c1 = M2 (M2^-1 mod M1),
+        // c2 = M1 (M1^-1 mod M2).
         const long long pr = a.rader_p, g = primitive_root(pr), ginv = pow_mod(g, pr - 2, pr);
-        std::vector<int> kt((size_t)M), perm((size_t)cols);
+        const long long m1 = p->M1, m2 = p->M2;
+        if (std::gcd(m1, m2) != 1) return fail(SPFFT_ERR_UNSUPPORTED, "Rader shape with non-coprime M1, M2");
+        std::vector<int> kt((size_t)M), perm((size_t)cols), gpow((size_t)M), gneg((size_t)M);
         std::vector<cd> b((size_t)M);
         long long gm = 1, gq = 1;
-        for (int m = 0; m < M; m++) {
-            kt[m] = (int)gm;
-            b[m] = unit_root(gm, pr, -1);
-            perm[m] = (int)(2 * gq);
-            perm[(size_t)M + m] = (int)(2 * gq + 1);
+        for (int q = 0; q < M; q++) {
+            gneg[q] = (int)gm;
+            gpow[q] = (int)gq;
+            b[q] = unit_root(gm, pr, -1);
             gm = gm * ginv % pr;
             gq = gq * g % pr;
         }
+        for (long long n1 = 0; n1 < m1; n1++)
+            for (long long n2 = 0; n2 < m2; n2++) {
+                const size_t pos = (size_t)(n2 + m2 * n1);
+                const long long q = (m2 * n1 + m1 * n2) % M;
+                kt[pos] = gneg[q];
+                perm[pos] = (int)(2LL * gpow[q]);
+                perm[(size_t)M + pos] = (int)((pr + 2LL * gpow[q]) % (2 * pr));
+            }
         perm[2 * (size_t)M] = 0;
-        perm[2 * (size_t)M + 1] = 1;
+        perm[2 * (size_t)M + 1] = (int)pr;
         fft64(b, -1);
+        const long long c1 = m2 * inv_mod(m2, m1) % M, c2 = m1 * inv_mod(m1, m2) % M;
         const double inv_m = 1.0 / (double)M;
-        for (int k1 = 0; k1 < p->M1; k1++)
-            for (int k2 = 0; k2 < p->M2; k2++)
-                bh[(size_t)k1 * p->M2 + k2] = to_f2(b[(size_t)k1 + (size_t)p->M1 * k2] * inv_m);
-        std::vector<float2> wc(256 + (pr + 255) / 256);
-        for (int e = 0; e < 256; e++) wc[e] = to_f2(unit_root(e, cols, -1));
-        for (int i = 0; i < (int)(pr + 255) / 256; i++) wc[256 + i] = to_f2(unit_root(256LL * i, cols, -1));
+        for (long long k1 = 0; k1 < m1; k1++)
+            for (long long k2 = 0; k2 < m2; k2++)
+                bh[(size_t)(k1 * m2 + k2)] = to_f2(b[(size_t)((c1 * k1 + c2 * k2) % M)] * inv_m);
         SPFFT_HIP(upload(p->rader_k, kt), "upload Rader indices");
         SPFFT_HIP(upload(p->colperm, perm), "upload Rader column order");
-        SPFFT_HIP(upload(p->wc2, wc), "upload twiddles");
         a.rader_k = p->rader_k.as<int>();
         a.colperm = p->colperm.as<int>();
-        a.wc2 = p->wc2.as<float2>();
     } else if (!a.direct) {
         // b[0] = 1, b[k] = b[M-k] = exp(+i pi k^2 / C) for 1 <= k < C
         std::vector<cd> b((size_t)M, cd(0, 0));
@@ -1100,7 +1120,7 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
         SPFFT_HIP(a.rader ? spfft::launch_rader(a, nullptr, true) : spfft::launch_fused(a, nullptr, true),
                   "fused row kernel LDS limit");
     p->table_bytes = p->h.bytes + p->wr_lo.bytes + p->wr_hi.bytes + p->wr_full.bytes + p->tw2.bytes + p->twM.bytes +
-                     p->wm2.bytes + p->rader_k.bytes + p->colperm.bytes + p->wc2.bytes +
+                     p->wm2.bytes + p->rader_k.bytes + p->colperm.bytes +
                      p->bhat.bytes + p->mfma_dft.bytes + p->mfma_frag.bytes + p->T.bytes + p->X0.bytes;
     *out = guard.release();
     return SPFFT_OK;

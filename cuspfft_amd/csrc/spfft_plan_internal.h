@@ -82,7 +82,7 @@ struct spfft_plan_s {
     bool colpass_full = false;
     spfft::RowPassArgs base{};  // kernel argument template (pointers to the tables filled in)
     DevBuf h, wr_hi, wr_lo, wr_full, tw2, twM, wm2, bhat, mfma_dft, mfma_frag, X0, T;
-    DevBuf rader_k, colperm, wc2;  // Rader plans: g^-m mod p, X0 column order, W_C table
+    DevBuf rader_k, colperm;  // Rader plans: g^-m mod p, X0 column order
     // CSC workspace
     int nnz_cap = -1;
     int last_nnz = 0;

@@ -67,8 +67,8 @@ struct RowPassArgs {
                                //   workgroups loop over row groups); 0: one workgroup per group
     int rader;                 // 1: k_rowrader (C = 2p, p an odd prime, M = p - 1; fused = 1)
     int rader_p;               // k_rowrader: p
-    const int* rader_k;        // k_rowrader: [m] = g^-m mod p, m < p - 1 (g a primitive root)
-    const float2* wc2;         // k_rowrader: W_C^e = wc2[256 + (e >> 8)] * wc2[e & 255], e < p
+    const int* rader_k;        // k_rowrader: [n2 + M2 n1] = g^-q mod p, q = (M2 n1 + M1 n2) mod (p - 1)
+                               //   (g a primitive root; the prime-factor order of the kernel)
     const int* colperm;        // column pass: X0 column (position) i holds matrix column
                                //   colperm[i] (NULL: the identity; k_rowrader's order)
     float2* X0;                // tile pass-1 output [u_local][ldx]
