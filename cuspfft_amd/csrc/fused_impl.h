@@ -188,15 +188,19 @@ constexpr size_t fused_lds_bytes_of(int m1, int nb, int rpw, int r1, int r2, int
                 : 0);
 }
 
-// L2 prefetch of the pass-1 rows of the row group 256 workgroups later (it runs on the
+// L2 prefetch of the pass-1 rows of the row group 128 workgroups later (it runs on the
 // same XCD: workgroups go to the 8 XCDs in turn), by direct-to-LDS loads of one 4-byte
 // word per 128-byte line into a 256-byte scratch past the shape's LDS that all waves
 // share (never read).  For one-batch shapes launched one workgroup per row group, when
 // the scratch fits without lowering the number of resident workgroups per CU.  Measured:
-// c2's row kernel (17496 = 24 * 27 * 27) 0.383 -> 0.368 ms; the two-batch c3 Bluestein
+// c2's row kernel (17496 = 24 * 27 * 27) 0.383 -> 0.368 ms (0.358 at a distance of 128
+// workgroups instead of 256, 0.362 at 64); the two-batch c3 Bluestein
 // shape (34992) 1.73 -> 1.86 ms and the persistent direct p14 kernel 0.84 -> 0.89 ms, so
 // those have none: their loads of B^ (or next row group) soon after wait for it.
-constexpr int kFusedAhead = 256;
+#ifndef SPFFT_FUSED_AHEAD
+#define SPFFT_FUSED_AHEAD 128
+#endif
+constexpr int kFusedAhead = SPFFT_FUSED_AHEAD;
 constexpr size_t kPrefetchScratch = 64 * sizeof(float);
 constexpr bool fused_pf(int m1, int nb, int rpw, int r1, int r2, int r3) {
     return nb == 1 && fused_lds_bytes_of(m1, nb, rpw, r1, r2, r3) + kPrefetchScratch <= 160 * 1024 &&

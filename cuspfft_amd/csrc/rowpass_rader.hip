@@ -72,7 +72,18 @@ constexpr size_t rader_lds_bytes_of(int m1, int nt, int r1, int r2, int r3) {
 // CU can keep lines in flight (about 17 GB/s per CU from HBM); from L2 they arrive
 // sooner.  No registers wait for these loads, and they are issued after the row's last
 // global load and barrier, so only the final wait of the workgroup includes them.
-constexpr int kRaderAhead = 256;  // c3 row kernel: 128 1.113 ms, 256 1.046, 512 1.069, 768 1.071
+// The distance keeps the prefetched rows of an XCD (kRaderAhead / 8 rows of 134 KB on
+// c3) well inside its 4 MB L2: c3 row kernel 32: 0.976 ms, 48: 0.908, 64: 0.911,
+// 80: 0.933, 128: 0.932, 256: 0.946 (at 256 a fifth more lines are fetched twice).
+// k_rowrader_mfma issues its prefetch late, after phase C (kRaderAheadMfma).
+#ifndef SPFFT_RADER_AHEAD
+#define SPFFT_RADER_AHEAD 56
+#endif
+#ifndef SPFFT_RADER_AHEAD_MFMA
+#define SPFFT_RADER_AHEAD_MFMA 256
+#endif
+constexpr int kRaderAhead = SPFFT_RADER_AHEAD;
+constexpr int kRaderAheadMfma = SPFFT_RADER_AHEAD_MFMA;
 constexpr size_t rader_pf_bytes(int) { return 64 * sizeof(float); }
 constexpr bool rader_pf(int m1, int nt, int r1, int r2, int r3) {
     return rader_lds_bytes_of(m1, nt, r1, r2, r3) + rader_pf_bytes(nt) <= 160 * 1024;
@@ -81,10 +92,10 @@ constexpr size_t rader_launch_lds(int m1, int nt, int r1, int r2, int r3) {
     return rader_lds_bytes_of(m1, nt, r1, r2, r3) + (rader_pf(m1, nt, r1, r2, r3) ? rader_pf_bytes(nt) : 0);
 }
 
-// The L2 prefetch (kRaderAhead) of pass-1 row bx + kRaderAhead of the tile.
-template <int NW>
+// The L2 prefetch of pass-1 row bx + AHEAD of the tile.
+template <int NW, int AHEAD>
 __device__ __forceinline__ void rader_prefetch(const RowPassArgs& a, int bx, int wave, int lane, float* scr) {
-    const int nb = bx + kRaderAhead;
+    const int nb = bx + AHEAD;
     if (nb >= a.tile_rows) return;
     const char* rowp = reinterpret_cast<const char*>(a.X0 + (size_t)nb * a.ldx);
     const int lines = (a.C * 8 + 127) >> 7;
@@ -169,7 +180,7 @@ __device__ __forceinline__ void rowrader_body(const RowPassArgs& a, const int bx
     lds_barrier();
     // after the row's own loads (whose waits it does not join); the barriers below are
     // LDS-only, so it stays in flight until the pivot stage's loads of B^ wait for it
-    if constexpr (PF) rader_prefetch<NW>(a, bx, wave, lane, reinterpret_cast<float*>(s_red + 2 * NW));
+    if constexpr (PF) rader_prefetch<NW, kRaderAhead>(a, bx, wave, lane, reinterpret_cast<float*>(s_red + 2 * NW));
     if (live) RegFft<M1>::run(v, false);
     {
         // sum_q a_s[q] = sum over the columns of the codelet's output 0: per wave and row
@@ -501,7 +512,7 @@ __device__ __forceinline__ void rowrader_mfma_body(const RowPassArgs& a, const i
     }
     lds_barrier();
 
-    if constexpr (PF) rader_prefetch<NW>(a, bx, wave, lane, reinterpret_cast<float*>(s_red + 2 * NW));
+    if constexpr (PF) rader_prefetch<NW, kRaderAheadMfma>(a, bx, wave, lane, reinterpret_cast<float*>(s_red + 2 * NW));
 
     rader_combine<NT>(a, bx, buf);
     if constexpr (PF) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
