@@ -294,7 +294,8 @@ def main():
         plan_source = "selector"
     plan = sp.Plan(m.rows, m.cols, u_tile=args.tile, **opts)
     info = plan.info
-    transform = "direct" if info["direct"] else "rader" if info["rader"] else "bluestein"
+    transform = ("direct" if info["direct"] else "rader" if info["rader"]
+                 else "split" if info["row_transform"] == sp.TRANSFORM_SPLIT else "bluestein")
     if plan_source == "planner":
         plan_source = f"planner ({transform}; selector not used)"
     d_row = torch.as_tensor(m.row_idx).to(dev)
@@ -323,7 +324,12 @@ def main():
     prof = plan.profile_read(stream)
     plan.profile(False)
     plan.check_input()
-    if plan.info["row_kernel"] == sp.ROW_FUSED:
+    if plan.info["row_transform"] == sp.TRANSFORM_SPLIT:
+        # split transform: the width-m2 row kernel (timed as the fft1 stage) and
+        # k_splitcols (timed as ifft1_finalize)
+        names = {"fft1": "split_rows", "ifft1_finalize": "split_cols"}
+        prof = {names.get(k, k): v for k, v in prof.items() if k != "conv"}
+    elif plan.info["row_kernel"] == sp.ROW_FUSED:
         # one kernel (k_rowfused / k_rowrader) does fft1 + conv + ifft1: reported as row_fused
         prof = {("row_fused" if k == "conv" else k): v for k, v in prof.items()
                 if k not in ("fft1", "ifft1_finalize")}
@@ -374,7 +380,8 @@ def main():
     def plan_desc(i):
         return {"fft_len": i["fft_len"], "m1": i["m1"], "m2": i["m2"],
                 "row_kernel": {1: "four-step", 2: "fused"}[i["row_kernel"]],
-                "transform": "direct" if i["direct"] else "rader" if i["rader"] else "bluestein",
+                "transform": ("direct" if i["direct"] else "rader" if i["rader"]
+                              else "split" if i["row_transform"] == sp.TRANSFORM_SPLIT else "bluestein"),
                 "precision": "bf16x3-mfma" if i["precision"] else "fp32"}
 
     forms = None

@@ -27,9 +27,10 @@ PREC_BF16X3 = 1
 ROW_AUTO, ROW_FOUR_STEP, ROW_FUSED = 0, 1, 2
 # column-pass half-period row pairing (spfft_opts.colpass_pair)
 PAIR_AUTO, PAIR_OFF = 0, 1
-# row transform (spfft_opts.row_transform): Bluestein, a direct length-cols FFT, or
-# Rader's convolutions for cols = 2p (p an odd prime)
-TRANSFORM_AUTO, TRANSFORM_BLUESTEIN, TRANSFORM_DIRECT, TRANSFORM_RADER = 0, 1, 2, 3
+# row transform (spfft_opts.row_transform): Bluestein, a direct length-cols FFT,
+# Rader's convolutions for cols = 2p (p an odd prime), or the split cols = n1 * n2 (an
+# on-chip row pass of width n2, then length-n1 DFTs across)
+TRANSFORM_AUTO, TRANSFORM_BLUESTEIN, TRANSFORM_DIRECT, TRANSFORM_RADER, TRANSFORM_SPLIT = 0, 1, 2, 3, 4
 # batch execution (spfft_batch_set_mode): shared descriptor-driven launches, or per matrix
 BATCH_AUTO, BATCH_PER_MATRIX = 0, 1
 

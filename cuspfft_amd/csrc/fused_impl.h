@@ -210,12 +210,21 @@ constexpr bool fused_pf(int m1, int nb, int rpw, int r1, int r2, int r3) {
 constexpr size_t fused_launch_lds(int m1, int nb, int rpw, int r1, int r2, int r3) {
     return fused_lds_bytes_of(m1, nb, rpw, r1, r2, r3) + (fused_pf(m1, nb, rpw, r1, r2, r3) ? kPrefetchScratch : 0);
 }
+// X0 offset (entries) of tile row r: r ldx, or (vrow_n1 > 0: the split transform's runs)
+// run r % vrow_n1 of X0 row r / vrow_n1
+__device__ __forceinline__ size_t x0_row(const RowPassArgs& a, int r) {
+    if (a.vrow_n1 == 0) return (size_t)r * a.ldx;
+    const int q = r / a.vrow_n1;
+    return (size_t)q * a.vrow_ld + (size_t)(r - q * a.vrow_n1) * a.C;
+}
+
 // rows [row0, row0 + nrows) of the tile's pass-1 buffer into L2; nw waves share the lines
 __device__ __forceinline__ void prefetch_rows(const RowPassArgs& a, int row0, int nrows, int nw, float* scr) {
     if (row0 >= a.tile_rows) return;
     nrows = min(nrows, a.tile_rows - row0);
-    const char* p = reinterpret_cast<const char*>(a.X0 + (size_t)row0 * a.ldx);
-    const int lines = (int)(((size_t)(nrows - 1) * a.ldx * 8 + (size_t)a.C * 8 + 127) >> 7);
+    const size_t base = x0_row(a, row0);
+    const char* p = reinterpret_cast<const char*>(a.X0 + base);
+    const int lines = (int)(((x0_row(a, row0 + nrows - 1) - base) * 8 + (size_t)a.C * 8 + 127) >> 7);
     const int lane = threadIdx.x & 63;
     for (int i = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); i * 64 < lines; i += nw) {
         const int line = i * 64 + lane;
@@ -421,12 +430,13 @@ __device__ __forceinline__ void rowfused_body(const RowPassArgs& a, const int bx
         const int nrows = min(RPW, a.tile_rows - row0);
         // RPW == 1: the row as a buffer of C entries, so reads past it return 0 (the
         // zero padding) with the stride M2 n1 as the scalar offset
-        const bio::rsrc_t r_x = bio::make_rsrc(a.X0 + (size_t)row0 * a.ldx,
-                                                   (uint32_t)(RPW == 1 ? a.C : nrows * a.ldx) * 8u);
+        const size_t xb = x0_row(a, row0);
+        const bio::rsrc_t r_x = bio::make_rsrc(
+            a.X0 + xb, (uint32_t)(RPW == 1 ? a.C : x0_row(a, row0 + nrows - 1) - xb + a.C) * 8u);
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
             if (live[c]) {
-                const int rb = g[c] * a.ldx;
+                const int rb = (int)(x0_row(a, row0 + g[c]) - xb);
 #pragma unroll
                 for (int n1 = 0; n1 < M1; n1++) {
                     if (n1 >= NZ) {

@@ -287,7 +287,8 @@ __device__ __forceinline__ void colpass_body(const RowPassArgs& a, const int bx,
         }
     }
     __syncthreads();
-    // whole TC-column row segments of X0 (ldx is a multiple of 64)
+    // whole TC-column row segments of X0 (ldx is a multiple of 64, or C for the split
+    // transform: positions past the row are not written)
     const int half = a.R >> 1, qmax = half >> 1;  // PAIR: rows u <= qmax and H - u
     for (int e = threadIdx.x; e < NTILE * 64 * U * TC; e += kColThreads) {
         const int row = e / TC, col = e % TC;
@@ -302,7 +303,7 @@ __device__ __forceinline__ void colpass_body(const RowPassArgs& a, const int bx,
             ul = u0l + row;
             ok = ul < a.tile_rows;
         }
-        if (ok) {  // X0 is read once, by the row pass: non-temporal
+        if (ok && c0 + col < a.ldx) {  // X0 is read once, by the row pass: non-temporal
             typedef float f32x2 __attribute__((ext_vector_type(2)));
             const f32x2 x = {t_re[row * TS + col], t_im[row * TS + col]};
             __builtin_nontemporal_store(x, reinterpret_cast<f32x2*>(&a.X0[(size_t)ul * a.ldx + c0 + col]));
@@ -468,7 +469,7 @@ __device__ __forceinline__ void colpass_mx_body(const RowPassArgs& a, const int 
                 ul = u;
                 ok = u < a.tile_rows;
             }
-            if (ok) {  // X0 is read once, by the row pass: non-temporal
+            if (ok && c0 + col < a.ldx) {  // X0 is read once, by the row pass: non-temporal
                 typedef float f32x2 __attribute__((ext_vector_type(2)));
                 const f32x2 x = {t_re[row * TS + col], t_im[row * TS + col]};
                 __builtin_nontemporal_store(x, reinterpret_cast<f32x2*>(&a.X0[(size_t)ul * a.ldx + c0 + col]));
@@ -695,10 +696,8 @@ static hipError_t launch_fft1(const RowPassArgs& a, dim3 grid, hipStream_t s, bo
     return e;
 }
 
-hipError_t launch_rowpass_tile(const RowPassArgs& a, bool packed, hipStream_t s, hipEvent_t* ev) {
+hipError_t launch_colpass_tile(const RowPassArgs& a, bool packed, hipStream_t s) {
     if (a.tile_rows <= 0) return hipSuccess;
-    hipError_t e;
-    if (ev && (e = hipEventRecord(ev[0], s)) != hipSuccess) return e;
     if (packed) {
         if (a.wr_full) launch_colpass_u<true, true>(a, s);
         else launch_colpass_u<true, false>(a, s);
@@ -706,7 +705,14 @@ hipError_t launch_rowpass_tile(const RowPassArgs& a, bool packed, hipStream_t s,
         if (a.wr_full) launch_colpass_u<false, true>(a, s);
         else launch_colpass_u<false, false>(a, s);
     }
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+hipError_t launch_rowpass_tile(const RowPassArgs& a, bool packed, hipStream_t s, hipEvent_t* ev) {
+    if (a.tile_rows <= 0) return hipSuccess;
+    hipError_t e;
+    if (ev && (e = hipEventRecord(ev[0], s)) != hipSuccess) return e;
+    if ((e = launch_colpass_tile(a, packed, s)) != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev[1], s)) != hipSuccess) return e;
     if (a.fused) {
         // one kernel for the whole row pass; the fft1 and ifft1 stages time as empty

@@ -1,0 +1,16 @@
+// rowpass_split_b.hip -- k_splitcols instantiations (see split_impl.h).
+#include "spfft_kernels.h"
+#include "split_impl.h"
+
+namespace spfft {
+
+hipError_t launch_split_cols_b(const RowPassArgs& a, dim3 grid, size_t lds, hipStream_t s) {
+#define SPFFT_SPLIT_LAUNCH_REG(n) SPFFT_SPLIT_LAUNCH_CASE(n, false)
+#define SPFFT_SPLIT_LAUNCH_DIR(n) SPFFT_SPLIT_LAUNCH_CASE(n, true)
+    SPFFT_SPLIT_REG_LIST_B(SPFFT_SPLIT_LAUNCH_REG)
+#undef SPFFT_SPLIT_LAUNCH_REG
+#undef SPFFT_SPLIT_LAUNCH_DIR
+    return hipErrorInvalidValue;
+}
+
+}  // namespace spfft

@@ -93,5 +93,12 @@ hipError_t launch_rader(const RowPassArgs& a, hipStream_t s, bool prepare_only);
 // ev: NULL, or 5 events recorded before k_colpass and after each of the 4 kernels
 // (with the fused row kernel: after k_colpass, twice, after k_rowfused, twice).
 hipError_t launch_rowpass_tile(const RowPassArgs& a, bool packed, hipStream_t s, hipEvent_t* ev);
+// The column pass of one tile alone (k_colpass or k_colpass_mx as a.colpass_rows picks).
+hipError_t launch_colpass_tile(const RowPassArgs& a, bool packed, hipStream_t s);
+// Split row transform (rowpass_split*.hip): pass B, k_splitcols<a.split_n1> over the
+// tile buffer a.T (C entries per row) into the output rows of the tile.
+bool split_supports_n1(int n1, bool* direct);
+size_t split_lds_bytes(int cols);
+hipError_t launch_split_cols(const RowPassArgs& a, hipStream_t s);
 
 }  // namespace spfft

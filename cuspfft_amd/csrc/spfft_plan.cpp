@@ -501,7 +501,34 @@ static spfft_status rowpass_tile(spfft_plan_s& p, RowPassArgs& a, int u0, int n,
         if (!ev) return fail(SPFFT_ERR_HIP, "hipEventCreate failed");
         p.ev_groups.push_back({1, at});
     }
-    SPFFT_HIP(spfft::launch_rowpass_tile(a, p.packed, s, ev), "row pass launch");
+    if (!a.split_n1) {
+        SPFFT_HIP(spfft::launch_rowpass_tile(a, p.packed, s, ev), "row pass launch");
+        return SPFFT_OK;
+    }
+    // split C = N1 N2: the column pass into X0 (each row N1 runs of N2 entries), pass A
+    // (the width-N2 plan's row kernel over the tile's n N1 runs as rows u N1 + n1, full
+    // spectra into T at a pitch of split_p2, no mirror rows: R = 0), pass B
+    // (k_splitcols).  Events: column pass, pass A (as the "fft1" stage), nothing, pass B
+    // (as "ifft1_finalize").
+    RowPassArgs b = p.split_sub->base;
+    b.X0 = a.X0;
+    b.vrow_n1 = a.split_n1;  // tile row u N1 + n1 is run n1 of X0 row u
+    b.vrow_ld = a.ldx;
+    b.out = b.out_m = a.T;
+    b.P = a.split_p2;
+    b.V = a.split_n2;
+    b.R = 0;
+    b.u_base = 0;
+    b.tile_rows = n * a.split_n1;
+    b.sub_rows = 0;
+    if (ev) SPFFT_HIP(hipEventRecord(ev[0], s), "hipEventRecord");
+    SPFFT_HIP(spfft::launch_colpass_tile(a, p.packed, s), "column pass launch");
+    if (ev) SPFFT_HIP(hipEventRecord(ev[1], s), "hipEventRecord");
+    SPFFT_HIP(spfft::launch_fused(b, s, false), "split row kernel launch");
+    if (ev) SPFFT_HIP(hipEventRecord(ev[2], s), "hipEventRecord");
+    if (ev) SPFFT_HIP(hipEventRecord(ev[3], s), "hipEventRecord");
+    SPFFT_HIP(spfft::launch_split_cols(a, s), "split column kernel launch");
+    if (ev) SPFFT_HIP(hipEventRecord(ev[4], s), "hipEventRecord");
     return SPFFT_OK;
 }
 
@@ -690,6 +717,44 @@ int find_rader(int L, int m1_req, size_t lds_limit, bool mfma, spfft::FusedShape
     return -1;
 }
 
+// The width-n2 row pass of a split plan, on chip: the direct transform when n2 has a
+// one-batch fused shape, else Bluestein at the smallest fused length M >= 2 n2 - 1 within
+// 2.5 n2 (Rader is not used there: its row kernel writes half spectra).  *len: its length.
+bool split_sub_opts(int n2, size_t lds_limit, spfft_opts* so, int* len) {
+    spfft_opts_default(so);
+    spfft::FusedShape f;
+    if (find_fused(n2, 0, lds_limit, &f, true) >= 0) {
+        so->row_transform = SPFFT_TRANSFORM_DIRECT;
+        *len = n2;
+        return true;
+    }
+    int best = 0;
+    for (int i = 0; spfft::fused_shape(i, &f); i++) {
+        const int m = f.m1 * f.r[0] * f.r[1] * f.r[2];
+        if (f.lds_bytes <= lds_limit && is_7_smooth(m) && m >= 2 * n2 - 1 && (best == 0 || m < best)) best = m;
+    }
+    if (best == 0 || best > 2.5 * n2 + 64) return false;
+    so->row_transform = SPFFT_TRANSFORM_BLUESTEIN;
+    so->fft_len = best;
+    *len = best;
+    return true;
+}
+
+// The split C = n1 * n2 with the smallest n1 that has a column kernel and leaves an
+// on-chip row pass of width n2, or 0.
+int find_split(int cols, int tile_rows, size_t lds_limit, spfft_opts* so, int* n2_out, int* len) {
+    for (int n1 = 2; n1 <= 127 && n1 <= cols / 2; n1++) {
+        bool direct = false;
+        if (cols % n1 || !spfft::split_supports_n1(n1, &direct)) continue;
+        if ((long long)tile_rows * n1 > (1 << 22)) continue;  // rows of the width-n2 plan
+        if (split_sub_opts(cols / n1, lds_limit, so, len)) {
+            *n2_out = cols / n1;
+            return n1;
+        }
+    }
+    return 0;
+}
+
 spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t budget, size_t lds_limit,
                         spfft_plan_s* p) {
     spfft_opts o;
@@ -740,8 +805,8 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     }
     if (o.row_persist != 0 && o.row_persist != 1) return fail(SPFFT_ERR_INVALID_ARG, "row_persist must be 0 or 1");
     // the direct length-cols FFT: a single-batch fused shape of length cols
-    if (o.row_transform < SPFFT_TRANSFORM_AUTO || o.row_transform > SPFFT_TRANSFORM_RADER)
-        return fail(SPFFT_ERR_INVALID_ARG, "row_transform must be 0, 1, 2 or 3");
+    if (o.row_transform < SPFFT_TRANSFORM_AUTO || o.row_transform > SPFFT_TRANSFORM_SPLIT)
+        return fail(SPFFT_ERR_INVALID_ARG, "row_transform must be 0, 1, 2, 3 or 4");
     spfft::FusedShape fd{};
     const bool chip_ok = !mfma && o.fft_len == 0 && o.conv_rows == 0 && o.row_kernel != SPFFT_ROW_FOUR_STEP;
     if (o.row_threads < 0 || o.row_threads % 64) return fail(SPFFT_ERR_INVALID_ARG, "row_threads must be 0 or a multiple of 64");
@@ -778,14 +843,40 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     spfft::FusedShape fs = fd;
     const bool fused_ok = direct || rader ||
                           (!mfma && o.conv_rows == 0 && o.row_kernel != SPFFT_ROW_FOUR_STEP &&
+                           o.row_transform != SPFFT_TRANSFORM_SPLIT &&
                            find_fused(M, o.m1, lds_limit, &fs, false, o.row_threads) >= 0);
-    if (o.row_kernel == SPFFT_ROW_FUSED && !fused_ok)
+    // split (C = N1 N2): when asked for, or (AUTO) when no on-chip row pass exists for C
+    int split_n1 = 0, split_n2 = 0;
+    {
+        const bool split_opts_ok = !mfma && o.fft_len == 0 && o.conv_rows == 0 && o.m1 == 0 && o.t_rows == 0 &&
+                                   o.row_threads == 0 && o.row_kernel != SPFFT_ROW_FOUR_STEP;
+        const bool want = o.row_transform == SPFFT_TRANSFORM_SPLIT ||
+                          (o.row_transform == SPFFT_TRANSFORM_AUTO && !fused_ok);
+        const int tile_rows = std::min(o.u_tile > 0 ? o.u_tile : rows / 2 + 1, rows / 2 + 1);
+        if (split_opts_ok && want)
+            split_n1 = find_split(cols, tile_rows, lds_limit, &p->split_opts, &split_n2, &p->split_len);
+        if (o.row_transform == SPFFT_TRANSFORM_SPLIT && !split_n1)
+            return fail(SPFFT_ERR_UNSUPPORTED, "no split transform for cols = " + std::to_string(cols) +
+                                                   " with these options");
+    }
+    if (o.row_kernel == SPFFT_ROW_FUSED && !fused_ok && !split_n1)
         return fail(SPFFT_ERR_UNSUPPORTED, "no fused row-kernel shape for M = " + std::to_string(M) +
                                                (mfma ? " in the BF16x3 form" : ""));
     RowPassArgs& a = p->base;
     spfft_status st = SPFFT_OK;
     p->opts.u_tile = o.u_tile;
-    if (fused_ok) {
+    if (split_n1) {
+        // the row pass is the width-N2 plan's kernel plus k_splitcols: no M of its own
+        // (the tables sized by M, M1, M2 are one entry)
+        M = 1;
+        p->M = p->M1 = p->M2 = 1;
+        a.fused = 0;
+        a.conv_kind = 0;
+        a.f2.L = 1;
+        a.f2.nstages = 0;
+        a.split_n1 = split_n1;
+        a.split_n2 = split_n2;
+    } else if (fused_ok) {
         p->M1 = fs.m1;
         p->M2 = fs.r[0] * fs.r[1] * fs.r[2];
         a.fused = 1;
@@ -808,7 +899,7 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
         a.fused = 0;
         st = choose_split(*p, o.m1, budget, mfma);
     }
-    if (st != SPFFT_OK && o.fft_len == 0 && o.m1 == 0) {
+    if (st != SPFFT_OK && !split_n1 && o.fft_len == 0 && o.m1 == 0) {
         // a rule-derived M without a supported split (e.g. 5^7): the next smooth length
         auto next = [&](int n) {
             return o.length_rule == SPFFT_SMOOTH_3 ? spfft_next_3_smooth(n) : spfft_next_7_smooth(n);
@@ -827,7 +918,7 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     // four-step kernels: T holds t_rows rows at a time (0: the whole tile)
     if (o.t_rows < 0) return fail(SPFFT_ERR_INVALID_ARG, "t_rows must be >= 0");
     p->t_rows = 0;
-    if (!a.fused) {
+    if (!a.fused && !split_n1) {
         const int tile = std::min(p->opts.u_tile, rows / 2 + 1);
         p->t_rows = o.t_rows > 0 ? std::min(o.t_rows, tile) : tile;
     }
@@ -839,6 +930,7 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     a.M1 = p->M1;
     a.M2 = p->M2;
     a.ldx = (cols + 63) / 64 * 64;
+    a.split_p2 = split_n1 ? (split_n2 + 15) / 16 * 16 : 0;
     int hs = 8;
     while ((1LL << (2 * hs)) < rows) hs++;
     a.hi_shift = hs;
@@ -863,7 +955,7 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     // two-level table's extra product costs less than the lost occupancy (R = 6000: column
     // pass 0.104 ms whole, 0.041 ms two-level; R = 8000: 0.184 / 0.067 ms)
     p->colpass_full = o.colpass_table == 2 || (o.colpass_table == 0 && full_fits && rows <= kAutoFullTableRows);
-    if (a.fused) {
+    if (a.fused || split_n1) {
         a.G2 = 0;
         return SPFFT_OK;
     }
@@ -896,9 +988,17 @@ void fill_info(const spfft_plan_s* p, spfft_plan_info* info) {
     info->rader = p->base.rader;
     info->row_transform = p->base.direct ? SPFFT_TRANSFORM_DIRECT
                           : p->base.rader ? SPFFT_TRANSFORM_RADER
+                          : p->base.split_n1 ? SPFFT_TRANSFORM_SPLIT
                                           : SPFFT_TRANSFORM_BLUESTEIN;
     info->t_rows = p->t_rows;
     info->row_threads = p->base.fused ? p->base.fused_nt : 0;
+    if (p->base.split_n1) {
+        info->fft_len = p->split_len;
+        info->m1 = p->base.split_n1;
+        info->m2 = p->base.split_n2;
+        info->row_kernel = SPFFT_ROW_FUSED;
+        info->row_threads = p->split_sub ? p->split_sub->base.fused_nt : 0;
+    }
 }
 
 }  // namespace
@@ -928,7 +1028,9 @@ namespace spfft_internal {
 
 void workspace_bytes(const spfft_plan_s& p, size_t* t_bytes, size_t* x0_bytes) {
     const size_t tile_rows = (size_t)std::min(p.opts.u_tile, p.R / 2 + 1);
-    *t_bytes = p.base.fused ? 0 : (size_t)p.t_rows * (size_t)p.M * sizeof(float2);
+    *t_bytes = p.base.split_n1 ? tile_rows * (size_t)p.base.split_n1 * p.base.split_p2 * sizeof(float2)
+               : p.base.fused ? 0
+                              : (size_t)p.t_rows * (size_t)p.M * sizeof(float2);
     *x0_bytes = tile_rows * (size_t)p.base.ldx * sizeof(float2);
 }
 
@@ -962,6 +1064,52 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
         const long long k2 = (long long)k * k % (2LL * cols);
         // exp(-i pi k^2 / C); the direct and Rader transforms have no chirp (h = 1)
         h[k] = a.direct || a.rader ? make_float2(1.f, 0.f) : to_f2(unit_root(k2, 2LL * cols, -1));
+    }
+    if (a.split_n1) {
+        // split C = N1 N2: the width-N2 plan (its row kernel takes the runs of N2 entries
+        // of each X0 row as its rows: X0 position n1 N2 + n2 holds column n1 + N1 n2); the column pass
+        // applies that plan's input chirp, h[n1 N2 + n2] = exp(-i pi n2^2 / N2) (1 for
+        // its direct transform); W_C and, for an odd prime N1 > 31, the DFT_N1 table of
+        // k_splitcols
+        const int n1s = a.split_n1, n2s = a.split_n2;
+        const int tile_rows = std::min(p->opts.u_tile, rows / 2 + 1);
+        spfft_plan_s* sub = nullptr;
+        st = plan_create_ex(tile_rows * n1s, n2s, &p->split_opts, false, &sub);
+        if (st != SPFFT_OK) return st;
+        p->split_sub.reset(sub);
+        if (!sub->base.fused || sub->base.rader)
+            return fail(SPFFT_ERR_UNSUPPORTED, "split transform: the width-" + std::to_string(n2s) +
+                                                   " row pass is not on chip");
+        p->split_len = sub->M;
+        std::vector<int> perm((size_t)cols);
+        for (int i1 = 0; i1 < n1s; i1++)
+            for (int i2 = 0; i2 < n2s; i2++) {
+                const size_t pos = (size_t)i1 * n2s + i2;
+                perm[pos] = i1 + n1s * i2;
+                const long long q = (long long)i2 * i2 % (2LL * n2s);
+                h[pos] = sub->base.direct ? make_float2(1.f, 0.f) : to_f2(unit_root(q, 2LL * n2s, -1));
+            }
+        std::vector<float2> w((size_t)256 + (cols + 255) / 256);
+        for (int e = 0; e < 256; e++) w[e] = to_f2(unit_root(e, cols, -1));
+        for (int i = 0; i < (cols + 255) / 256; i++) w[256 + i] = to_f2(unit_root(256LL * i, cols, -1));
+        SPFFT_HIP(upload(p->colperm, perm), "upload split column order");
+        SPFFT_HIP(upload(p->split_w, w), "upload twiddles");
+        a.colperm = p->colperm.as<int>();
+        a.split_w = p->split_w.as<float2>();
+        bool dir = false;
+        spfft::split_supports_n1(n1s, &dir);
+        if (dir) {
+            const int H = (n1s - 1) / 2;
+            std::vector<float> t((size_t)2 * H * H);
+            for (int pp = 1; pp <= H; pp++)
+                for (int q = 1; q <= H; q++) {
+                    const cd z = unit_root((long long)pp * q, n1s, +1);  // cos + i sin
+                    t[2 * ((size_t)(pp - 1) * H + q - 1)] = (float)z.real();
+                    t[2 * ((size_t)(pp - 1) * H + q - 1) + 1] = (float)z.imag();
+                }
+            SPFFT_HIP(upload(p->split_dft, t), "upload DFT table");
+            a.split_dft = p->split_dft.as<float2>();
+        }
     }
     for (int l = 0; l < (1 << hs); l++) lo[l] = to_f2(unit_root(l, rows, -1));
     for (int j = 0; j < a.n_hi; j++) hi[j] = to_f2(unit_root((long long)j << hs, rows, -1));
@@ -1010,7 +1158,7 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
         SPFFT_HIP(upload(p->colperm, perm), "upload Rader column order");
         a.rader_k = p->rader_k.as<int>();
         a.colperm = p->colperm.as<int>();
-    } else if (!a.direct) {
+    } else if (!a.direct && !a.split_n1) {
         // b[0] = 1, b[k] = b[M-k] = exp(+i pi k^2 / C) for 1 <= k < C
         std::vector<cd> b((size_t)M, cd(0, 0));
         b[0] = cd(1, 0);
@@ -1088,7 +1236,10 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
     }
     const int tile_rows = std::min(o.u_tile, rows / 2 + 1);
     if (own_workspace) {
-        if (!a.fused)
+        if (a.split_n1)
+            SPFFT_HIP(p->T.alloc((size_t)tile_rows * a.split_n1 * a.split_p2 * sizeof(float2)),
+                      "hipMalloc(tile buffer)");
+        else if (!a.fused)
             SPFFT_HIP(p->T.alloc((size_t)p->t_rows * M * sizeof(float2)), "hipMalloc(tile buffer)");
         SPFFT_HIP(p->X0.alloc((size_t)tile_rows * a.ldx * sizeof(float2)), "hipMalloc(pass-1 buffer)");
     }
@@ -1121,7 +1272,8 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
                   "fused row kernel LDS limit");
     p->table_bytes = p->h.bytes + p->wr_lo.bytes + p->wr_hi.bytes + p->wr_full.bytes + p->tw2.bytes + p->twM.bytes +
                      p->wm2.bytes + p->rader_k.bytes + p->colperm.bytes +
-                     p->bhat.bytes + p->mfma_dft.bytes + p->mfma_frag.bytes + p->T.bytes + p->X0.bytes;
+                     p->bhat.bytes + p->mfma_dft.bytes + p->mfma_frag.bytes + p->T.bytes + p->X0.bytes +
+                     p->split_w.bytes + p->split_dft.bytes + (p->split_sub ? p->split_sub->table_bytes : 0);
     *out = guard.release();
     return SPFFT_OK;
 }

@@ -5,6 +5,7 @@
 
 #include <string>
 #include <utility>
+#include <memory>
 #include <vector>
 
 #include "../../include/spfft_c.h"
@@ -82,7 +83,13 @@ struct spfft_plan_s {
     bool colpass_full = false;
     spfft::RowPassArgs base{};  // kernel argument template (pointers to the tables filled in)
     DevBuf h, wr_hi, wr_lo, wr_full, tw2, twM, wm2, bhat, mfma_dft, mfma_frag, X0, T;
-    DevBuf rader_k, colperm;  // Rader plans: g^-m mod p, X0 column order
+    DevBuf rader_k, colperm;  // Rader and split plans: g^-m mod p, X0 column order
+    // split plans (C = N1 N2): the plan of the width-N2 row pass over the N1 runs of each
+    // X0 row (its tables and shape; its own workspace is unused), W_C and DFT_N1 tables
+    std::unique_ptr<spfft_plan_s> split_sub;
+    spfft_opts split_opts{};  // the options of that plan
+    int split_len = 0;        // its row-pass length (Bluestein M, or N2 when direct)
+    DevBuf split_w, split_dft;
     // CSC workspace
     int nnz_cap = -1;
     int last_nnz = 0;

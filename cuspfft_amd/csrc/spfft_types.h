@@ -69,6 +69,17 @@ struct RowPassArgs {
     int rader_p;               // k_rowrader: p
     const int* rader_k;        // k_rowrader: [n2 + M2 n1] = g^-q mod p, q = (M2 n1 + M1 n2) mod (p - 1)
                                //   (g a primitive root; the prime-factor order of the kernel)
+    int split_n1, split_n2;    // split row transform (C = N1 N2, split_n1 > 0): pass A a row
+                               //   kernel of width N2 over the N1 runs of each X0 row, pass B
+                               //   k_splitcols<N1> over the tile buffer T
+    int split_p2;              // T: run n1 of row u at T + (u N1 + n1) split_p2 (N2 rounded up to
+                               //   16 entries: whole 128-byte lines)
+    int vrow_n1, vrow_ld;      // row kernels, vrow_n1 > 0 (pass A of the split transform): tile row
+                               //   r is run r % vrow_n1 of X0 row r / vrow_n1 (X0 rows of vrow_ld
+                               //   entries, runs of C); 0: tile row r is X0 row r (ldx entries)
+    const float2* split_w;     // k_splitcols: W_C^e = split_w[256 + (e >> 8)] * split_w[e & 255]
+    const float2* split_dft;   // k_splitcols, odd prime N1 > 31: (cos, sin)(2 pi p q / N1) at
+                               //   [(p - 1) H + q - 1], p, q = 1 .. H = (N1 - 1) / 2
     const int* colperm;        // column pass: X0 column (position) i holds matrix column
                                //   colperm[i] (NULL: the identity; k_rowrader's order)
     float2* X0;                // tile pass-1 output [u_local][ldx]

@@ -28,7 +28,7 @@ import os
 
 import numpy as np
 
-from . import TRANSFORM_DIRECT, TRANSFORM_RADER, _check, _vp, describe_plan, load
+from . import TRANSFORM_DIRECT, TRANSFORM_RADER, TRANSFORM_SPLIT, _check, _vp, describe_plan, load
 
 N_FEATURES = 34
 # the network trained on the autotuner timings of the seeded training set (train_selector)
@@ -230,6 +230,10 @@ def choose_plan(m, max_m1: int = 2, fp32_only: bool = False) -> dict:
     fp32_only: among the plain FP32 plans only (no BF16x3 precision form)."""
     global _default
     from . import autotune
+    # the split transform (widths with no on-chip row pass of their own) has no timings in
+    # the training set: the planner's plan is returned as it is
+    if describe_plan(m.rows, m.cols)["row_transform"] == TRANSFORM_SPLIT:
+        return dict(length_rule=3)
     if _default is None:
         _default = Selector.load(DEFAULT_MODEL)
     rec = autotune.features_of(m)

@@ -99,19 +99,25 @@ enum { SPFFT_PAIR_AUTO = 0, SPFFT_PAIR_OFF = 1 };
  * row-pass work.  RADER: for cols = 2p, p an odd prime with a Rader shape for p - 1
  * (cols in spfft_rader_widths(); same restrictions): the two length-p halves of the
  * row DFT (even and odd columns) as cyclic convolutions of length p - 1 (fft_len =
- * p - 1), about half the row-pass work of BLUESTEIN.  AUTO: DIRECT, else RADER, when
- * possible, else BLUESTEIN.  The transform computed is the same; only the rounding
- * differs. */
+ * p - 1), about half the row-pass work of BLUESTEIN.  SPLIT: for cols = n1 * n2 with
+ * a length-n1 column kernel (n1 <= 64 with prime factors <= 31, or an odd prime <= 127)
+ * and an on-chip row pass of width n2 (DIRECT, or BLUESTEIN at a fused length): the
+ * row DFT as n1 DFTs of length n2 on chip, then n1-point DFTs across them, through a
+ * tile buffer of cols entries per row (m1 = n1, m2 = n2 in spfft_plan_info; FP32 form,
+ * no fft_len / conv_rows / four-step options).  AUTO: DIRECT, else RADER, when possible;
+ * else BLUESTEIN when its length has an on-chip shape; else SPLIT when possible; else
+ * BLUESTEIN with the four-step kernels.  The transform computed is the same; only the
+ * rounding differs. */
 enum { SPFFT_TRANSFORM_AUTO = 0, SPFFT_TRANSFORM_BLUESTEIN = 1, SPFFT_TRANSFORM_DIRECT = 2,
-       SPFFT_TRANSFORM_RADER = 3 };
+       SPFFT_TRANSFORM_RADER = 3, SPFFT_TRANSFORM_SPLIT = 4 };
 
 typedef struct spfft_plan_info {
     int struct_size;   /* sizeof(spfft_plan_info), set by the caller      */
     int rows, cols;
     int out_cols;      /* cols/2 + 1                                   */
     int pitch;         /* P, elements                                  */
-    int fft_len;       /* M                                            */
-    int m1, m2;        /* M = m1 * m2                                  */
+    int fft_len;       /* M (SPLIT: the length of the width-m2 row pass)   */
+    int m1, m2;        /* M = m1 * m2 (SPLIT: cols = m1 * m2)             */
     int packed;        /* 1 if CSC indices are stored as uint16        */
     int u_tile;
     int conv_rows;          /* rows per workgroup of the convolution kernel            */
@@ -126,7 +132,7 @@ typedef struct spfft_plan_info {
     int colpass_paired;     /* 1 if the column pass pairs rows u and R/2 - u            */
     int direct;             /* 1: direct length-cols FFT (fft_len = cols), 0: Bluestein  */
     int rader;              /* 1: Rader convolutions of length fft_len = cols/2 - 1      */
-    int row_transform;      /* SPFFT_TRANSFORM_BLUESTEIN, _DIRECT or _RADER (resolved)  */
+    int row_transform;      /* SPFFT_TRANSFORM_BLUESTEIN, _DIRECT, _RADER or _SPLIT (resolved) */
     int t_rows;             /* four-step kernels: rows of T per sub-tile (0: fused)      */
     int row_threads;        /* on-chip row kernels: threads per workgroup (0: four-step) */
 } spfft_plan_info;

@@ -53,6 +53,11 @@ def test_planner_covers_many_shapes():
         for rule in (3, 7):
             info = sp.describe_plan(17, cols, length_rule=rule)
             M = info["fft_len"]
+            if info["row_transform"] == sp.TRANSFORM_SPLIT:
+                # cols = m1 * m2, the width-m2 row pass at its direct or Bluestein length
+                assert info["m1"] * info["m2"] == cols and 2 <= info["m1"] <= 127
+                assert M == info["m2"] or M >= 2 * info["m2"] - 1
+                continue
             assert M >= 2 * cols - 1
             assert info["m1"] * info["m2"] == M
             assert 1 <= info["m1"] <= 64
@@ -111,7 +116,32 @@ def test_planner_direct_transform():
         with pytest.raises(sp.SpfftError):
             sp.describe_plan(300, cols, row_transform=sp.TRANSFORM_DIRECT)
     with pytest.raises(sp.SpfftError):
-        sp.describe_plan(300, 2048, row_transform=4)
+        sp.describe_plan(300, 2048, row_transform=5)
+
+
+def test_planner_split_transform():
+    """Widths without an on-chip row pass (no direct, Rader or fused Bluestein shape) split
+    as cols = n1 * n2 with the smallest n1 that has a column kernel and an on-chip width-n2
+    row pass; TRANSFORM_SPLIT forces it where it exists; the four-step options, the
+    precision form or an explicit length keep the four-step kernels."""
+    import cuspfft_amd as sp
+    c5 = sp.describe_plan(40421, 40421)  # 83 * 487: Bluestein M 82944 has no on-chip shape
+    assert c5["row_transform"] == sp.TRANSFORM_SPLIT
+    assert (c5["m1"], c5["m2"], c5["fft_len"]) == (83, 487, 1024)
+    assert c5["row_kernel"] == sp.ROW_FUSED and c5["t_rows"] == 0
+    f = sp.describe_plan(41, 7217, row_transform=sp.TRANSFORM_SPLIT)
+    assert (f["m1"], f["m2"], f["fft_len"]) == (7, 1031, 2304)
+    d = sp.describe_plan(64, 3072, row_transform=sp.TRANSFORM_SPLIT)
+    assert (d["m1"], d["m2"], d["fft_len"]) == (2, 1536, 1536)  # direct width-1536 pass
+    assert sp.describe_plan(41, 7217)["row_transform"] == sp.TRANSFORM_BLUESTEIN  # fused on chip
+    for kw in (dict(row_kernel=sp.ROW_FOUR_STEP), dict(precision=sp.PREC_BF16X3), dict(fft_len=82944),
+               dict(t_rows=1024)):
+        info = sp.describe_plan(40421, 40421, **kw)
+        assert info["row_transform"] == sp.TRANSFORM_BLUESTEIN and info["row_kernel"] == sp.ROW_FOUR_STEP
+    for cols in (40427, 7919):  # primes: no split
+        with pytest.raises(sp.SpfftError):
+            sp.describe_plan(30, cols, row_transform=sp.TRANSFORM_SPLIT)
+    assert sp.describe_plan(30, 40427)["row_transform"] == sp.TRANSFORM_BLUESTEIN
 
 
 def test_planner_rader_transform():
@@ -339,7 +369,7 @@ def test_automatic_tile():
         assert info["u_tile"] == n // 2 + 1
     assert sp.describe_plan(16706, 16706)["colpass_paired"] == 1
     assert sp.describe_plan(16706, 16706)["t_rows"] == 0  # on-chip row kernel: no T
-    c5 = sp.describe_plan(40421, 40421)
+    c5 = sp.describe_plan(40421, 40421, row_transform=sp.TRANSFORM_BLUESTEIN)
     assert c5["row_kernel"] == sp.ROW_FOUR_STEP and c5["fft_len"] == 82944
     assert c5["t_rows"] == c5["u_tile"] == 20211
     assert sp.describe_plan(40421, 40421, t_rows=7)["t_rows"] == 7

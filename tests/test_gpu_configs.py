@@ -1,9 +1,10 @@
 """Full-size GPU runs of the two largest workloads, checked against the FP64 oracle.
 
 c5: the 40421 x 40421 Chung-Lu matrix (~351,400 nonzeros) with its default plan, the
-four-step Bluestein row pass at M = 82944 (k_fft1<64> + k_conv_fast<36,36> +
-k_ifft1<64>); spot rows against the FP64 oracle plus
-the size-independent invariants.
+split row transform 40421 = 83 * 487 (the width-487 Bluestein row kernel at M = 1024 over
+the 83 runs of each row, then k_splitcols<83>), and with the four-step Bluestein row pass
+at M = 82944 (k_fft1<64> + k_conv_fast<36,36> + k_ifft1<64>); spot rows against the FP64
+oracle plus the size-independent invariants.
 
 c4: the 512-matrix batch (n log-uniform in [500, 12000]) with the shipped selector's
 plans, eager against hipGraph replays bit for bit, and sampled rows of every 16th
@@ -16,16 +17,22 @@ from test_gpu_path import REL_L2_PLAIN, _dev, _invariants
 
 
 @pytest.mark.gpu
-def test_c5_default_plan_full_size(gpu, oracle_mod):
+@pytest.mark.parametrize("transform", ["default", "bluestein"])
+def test_c5_full_size(gpu, oracle_mod, transform):
     import torch
     sp = gpu
     m = sp.config_matrix("c5")
-    plan = sp.Plan(m.rows, m.cols)
+    kw = {} if transform == "default" else dict(row_transform=sp.TRANSFORM_BLUESTEIN)
+    plan = sp.Plan(m.rows, m.cols, **kw)
     info = plan.info
-    print(f"[c5] {m.rows}x{m.cols} nnz={m.nnz} M={info['fft_len']} m1={info['m1']} "
-          f"t_rows={info['t_rows']} kernel={info['row_kernel']}", flush=True)
-    assert info["fft_len"] == 82944 and info["row_kernel"] == sp.ROW_FOUR_STEP
-    assert info["u_tile"] == m.rows // 2 + 1 and info["t_rows"] == info["u_tile"]
+    print(f"[c5 {transform}] {m.rows}x{m.cols} nnz={m.nnz} M={info['fft_len']} m1={info['m1']} "
+          f"m2={info['m2']} t_rows={info['t_rows']} kernel={info['row_kernel']}", flush=True)
+    if transform == "default":
+        assert info["row_transform"] == sp.TRANSFORM_SPLIT and (info["m1"], info["m2"]) == (83, 487)
+    else:
+        assert info["fft_len"] == 82944 and info["row_kernel"] == sp.ROW_FOUR_STEP
+        assert info["t_rows"] == info["u_tile"]
+    assert info["u_tile"] == m.rows // 2 + 1
     out = plan.alloc_output()
     plan.execute(_dev(m.row_idx), _dev(m.col_idx), out)
     torch.cuda.synchronize()
@@ -53,7 +60,7 @@ def test_t_rows_subtiles_bit_identical(gpu):
     m = sp.COOMatrix(600, m.cols, m.row_idx[m.row_idx < 600].copy(), m.col_idx[m.row_idx < 600].copy())
     outs = []
     for t_rows in (0, 301, 7):
-        plan = sp.Plan(m.rows, m.cols, t_rows=t_rows)
+        plan = sp.Plan(m.rows, m.cols, t_rows=t_rows, row_transform=sp.TRANSFORM_BLUESTEIN)
         print(f"[t_rows] request {t_rows} -> {plan.info['t_rows']}", flush=True)
         out = plan.alloc_output()
         plan.execute(_dev(m.row_idx), _dev(m.col_idx), out)
