@@ -19,7 +19,7 @@ import time
 import numpy as np
 
 from . import (Plan, SpfftError, describe_plan, fused_lengths, rader_widths, PREC_BF16X3, ROW_FOUR_STEP,
-               ROW_FUSED, TRANSFORM_BLUESTEIN, TRANSFORM_DIRECT, TRANSFORM_RADER)
+               ROW_FUSED, TRANSFORM_BLUESTEIN, TRANSFORM_DIRECT, TRANSFORM_RADER, TRANSFORM_SPLIT)
 
 # M1 lengths tried as alternatives to the planner's choice (register codelets
 # with a fast convolution kernel for M / M1).
@@ -52,9 +52,12 @@ def candidates(rows: int, cols: int, max_m1: int = 3) -> list[dict]:
     rule's M when it has a shape; plus the fused row kernel at the fused lengths
     M in [2*cols-1, 1.12 (2*cols-1)] (at most four; else the smallest above)."""
     out = []
-    # Bluestein candidates say so where the width has another transform (AUTO would take it)
-    auto = describe_plan(rows, cols)
-    blue = dict(row_transform=TRANSFORM_BLUESTEIN) if auto["direct"] or auto["rader"] else {}
+    # Bluestein candidates say so where the width has another transform (AUTO would take
+    # it: the direct or Rader transform, or for a length rule without an on-chip shape the
+    # split transform)
+    autos = [describe_plan(rows, cols, length_rule=rule) for rule in (3, 7)]
+    other = any(a["direct"] or a["rader"] or a["row_transform"] == TRANSFORM_SPLIT for a in autos)
+    blue = dict(row_transform=TRANSFORM_BLUESTEIN) if other else {}
     for rule in (3, 7):
         base = describe_plan(rows, cols, length_rule=rule, row_kernel=ROW_FOUR_STEP, **blue)
         M = base["fft_len"]
