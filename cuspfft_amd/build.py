@@ -34,7 +34,7 @@ LIB_SOURCES = ["csc_build.hip", "csc_noalloc.hip", "rowpass.hip", "rowpass_conv_
                "rowpass_fft1_a.hip", "rowpass_fft1_b.hip", "rowpass_fft1_c.hip", "rowpass_mfma.hip",
                "rowpass_fused_a.hip", "rowpass_fused_b.hip", "rowpass_fused_c.hip",
                "rowpass_fused_d.hip", "rowpass_fused_e.hip", "rowpass_fused_f.hip",
-               "rowpass_fused_g.hip", "rowpass_fused_h.hip", "rowpass_rader.hip",
+               "rowpass_fused_g.hip", "rowpass_fused_h.hip", "rowpass_rader.hip", "rowpass_raderruns.hip",
                "rowpass_split.hip", "rowpass_split_a.hip", "rowpass_split_b.hip", "rowpass_split_c.hip",
                "spfft_plan.cpp", "spfft_batch.cpp", "mlp.hip",
                "spfft_gen.cpp", "sparse_fft_shim.cpp"]
@@ -48,7 +48,7 @@ CLI_SOURCES = ["cli_main.cpp"]
 # and negation forms).
 NO_SIGNED_ZEROS = {"rowpass_fused_a.hip", "rowpass_fused_b.hip", "rowpass_fused_c.hip",
                    "rowpass_fused_d.hip", "rowpass_fused_e.hip", "rowpass_fused_f.hip",
-                   "rowpass_fused_g.hip", "rowpass_fused_h.hip", "rowpass_rader.hip",
+                   "rowpass_fused_g.hip", "rowpass_fused_h.hip", "rowpass_rader.hip", "rowpass_raderruns.hip",
                    "rowpass_fft1_a.hip", "rowpass_fft1_b.hip", "rowpass_fft1_c.hip"}
 
 

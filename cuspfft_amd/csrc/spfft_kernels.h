@@ -98,6 +98,12 @@ hipError_t launch_colpass_tile(const RowPassArgs& a, bool packed, hipStream_t s)
 // Split row transform (rowpass_split*.hip): pass B, k_splitcols<a.split_n1> over the
 // tile buffer a.T (C entries per row) into the output rows of the tile.
 bool split_supports_n1(int n1, bool* direct);
+// Pass A of the split transform for an odd prime run length p (rowpass_raderruns.hip):
+// Rader's convolution of length L = p - 1 for a.fused_rpw runs per workgroup.  shape:
+// M1, runs per workgroup, threads and stage radices r[3] for L, or false.
+bool raderruns_shape(int L, int* m1, int* ng, int* nt, int* r);
+size_t raderruns_lds_bytes(int L);  // of that shape's kernel (0: none)
+hipError_t launch_raderruns(const RowPassArgs& a, hipStream_t s, bool prepare_only);
 size_t split_lds_bytes(int cols);
 hipError_t launch_split_cols(const RowPassArgs& a, hipStream_t s);
 

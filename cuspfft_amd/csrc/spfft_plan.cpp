@@ -226,6 +226,44 @@ long long primitive_root(long long p) {
     }
 }
 
+// Rader tables for a prime p with L = p - 1 = m1 m2 (gcd(m1, m2) = 1) in the prime-factor
+// order of the kernels (rowpass_rader.hip, rowpass_raderruns.hip): with g a primitive
+// root of p, entry (n1, n2) of a convolution operand (position n2 + m2 n1) is a[q], q =
+// (m2 n1 + m1 n2) mod L; gpos[n2 + m2 n1] = g^q mod p (the input index it holds),
+// kt[n2 + m2 n1] = g^-q mod p (the output index it gives); b[q] = W_p^{g^-q}, and
+// frequency (k1, k2) of B^ = DFT_L(b) / L, at bh[k1 m2 + k2], is (c1 k1 + c2 k2) mod L
+// with c1 = m2 (m2^-1 mod m1), c2 = m1 (m1^-1 mod m2).
+void rader_tables(long long pr, long long m1, long long m2, std::vector<int>& gpos, std::vector<int>& kt,
+                  std::vector<float2>& bh) {
+    const long long L = pr - 1, g = primitive_root(pr), ginv = pow_mod(g, pr - 2, pr);
+    std::vector<int> gpow((size_t)L), gneg((size_t)L);
+    std::vector<cd> b((size_t)L);
+    long long gm = 1, gq = 1;
+    for (long long q = 0; q < L; q++) {
+        gneg[q] = (int)gm;
+        gpow[q] = (int)gq;
+        b[q] = unit_root(gm, pr, -1);
+        gm = gm * ginv % pr;
+        gq = gq * g % pr;
+    }
+    gpos.assign((size_t)L, 0);
+    kt.assign((size_t)L, 0);
+    for (long long n1 = 0; n1 < m1; n1++)
+        for (long long n2 = 0; n2 < m2; n2++) {
+            const size_t pos = (size_t)(n2 + m2 * n1);
+            const long long q = (m2 * n1 + m1 * n2) % L;
+            kt[pos] = gneg[q];
+            gpos[pos] = gpow[q];
+        }
+    fft64(b, -1);
+    const long long c1 = m2 * inv_mod(m2, m1) % L, c2 = m1 * inv_mod(m1, m2) % L;
+    const double inv_l = 1.0 / (double)L;
+    bh.assign((size_t)L, make_float2(0.f, 0.f));
+    for (long long k1 = 0; k1 < m1; k1++)
+        for (long long k2 = 0; k2 < m2; k2++)
+            bh[(size_t)(k1 * m2 + k2)] = to_f2(b[(size_t)((c1 * k1 + c2 * k2) % L)] * inv_l);
+}
+
 size_t g_lds_limit = 0;
 int g_cu_count = 0;  // compute units of the device (persistent fused launches)
 constexpr size_t kGfx950Lds = 163840;
@@ -510,7 +548,7 @@ static spfft_status rowpass_tile(spfft_plan_s& p, RowPassArgs& a, int u0, int n,
     // spectra into T at a pitch of split_p2, no mirror rows: R = 0), pass B
     // (k_splitcols).  Events: column pass, pass A (as the "fft1" stage), nothing, pass B
     // (as "ifft1_finalize").
-    RowPassArgs b = p.split_sub->base;
+    RowPassArgs b = p.split_rr ? p.split_rr_args : p.split_sub->base;
     b.X0 = a.X0;
     b.vrow_n1 = a.split_n1;  // tile row u N1 + n1 is run n1 of X0 row u
     b.vrow_ld = a.ldx;
@@ -524,7 +562,8 @@ static spfft_status rowpass_tile(spfft_plan_s& p, RowPassArgs& a, int u0, int n,
     if (ev) SPFFT_HIP(hipEventRecord(ev[0], s), "hipEventRecord");
     SPFFT_HIP(spfft::launch_colpass_tile(a, p.packed, s), "column pass launch");
     if (ev) SPFFT_HIP(hipEventRecord(ev[1], s), "hipEventRecord");
-    SPFFT_HIP(spfft::launch_fused(b, s, false), "split row kernel launch");
+    SPFFT_HIP(p.split_rr ? spfft::launch_raderruns(b, s, false) : spfft::launch_fused(b, s, false),
+              "split row kernel launch");
     if (ev) SPFFT_HIP(hipEventRecord(ev[2], s), "hipEventRecord");
     if (ev) SPFFT_HIP(hipEventRecord(ev[3], s), "hipEventRecord");
     SPFFT_HIP(spfft::launch_split_cols(a, s), "split column kernel launch");
@@ -740,15 +779,31 @@ bool split_sub_opts(int n2, size_t lds_limit, spfft_opts* so, int* len) {
     return true;
 }
 
+// Does an odd prime run length n2 have a k_raderruns shape (Rader over n2 - 1)?
+bool split_rr_ok(int n2, size_t lds_limit) {
+    int m1, ng, nt, r[3];
+    return is_odd_prime(n2) && spfft::raderruns_shape(n2 - 1, &m1, &ng, &nt, r) &&
+           spfft::raderruns_lds_bytes(n2 - 1) <= lds_limit;
+}
+
 // The split C = n1 * n2 with the smallest n1 that has a column kernel and leaves an
-// on-chip row pass of width n2, or 0.
-int find_split(int cols, int tile_rows, size_t lds_limit, spfft_opts* so, int* n2_out, int* len) {
+// on-chip row pass of width n2, or 0.  *rr: that pass is k_raderruns (an odd prime n2
+// with a shape: a length-(n2 - 1) convolution instead of Bluestein's >= 2 n2 - 1), else
+// the width-n2 plan of *so.
+int find_split(int cols, int tile_rows, size_t lds_limit, spfft_opts* so, int* n2_out, int* len, bool* rr) {
     for (int n1 = 2; n1 <= 127 && n1 <= cols / 2; n1++) {
         bool direct = false;
         if (cols % n1 || !spfft::split_supports_n1(n1, &direct)) continue;
         if ((long long)tile_rows * n1 > (1 << 22)) continue;  // rows of the width-n2 plan
-        if (split_sub_opts(cols / n1, lds_limit, so, len)) {
-            *n2_out = cols / n1;
+        const int n2 = cols / n1;
+        *rr = split_rr_ok(n2, lds_limit);
+        if (*rr) {
+            *n2_out = n2;
+            *len = n2 - 1;
+            return n1;
+        }
+        if (split_sub_opts(n2, lds_limit, so, len)) {
+            *n2_out = n2;
             return n1;
         }
     }
@@ -854,7 +909,8 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
                           (o.row_transform == SPFFT_TRANSFORM_AUTO && !fused_ok);
         const int tile_rows = std::min(o.u_tile > 0 ? o.u_tile : rows / 2 + 1, rows / 2 + 1);
         if (split_opts_ok && want)
-            split_n1 = find_split(cols, tile_rows, lds_limit, &p->split_opts, &split_n2, &p->split_len);
+            split_n1 = find_split(cols, tile_rows, lds_limit, &p->split_opts, &split_n2, &p->split_len,
+                                  &p->split_rr);
         if (o.row_transform == SPFFT_TRANSFORM_SPLIT && !split_n1)
             return fail(SPFFT_ERR_UNSUPPORTED, "no split transform for cols = " + std::to_string(cols) +
                                                    " with these options");
@@ -998,6 +1054,8 @@ void fill_info(const spfft_plan_s* p, spfft_plan_info* info) {
         info->m2 = p->base.split_n2;
         info->row_kernel = SPFFT_ROW_FUSED;
         info->row_threads = p->split_sub ? p->split_sub->base.fused_nt : 0;
+        int m1, ng, nt, r[3];
+        if (p->split_rr && spfft::raderruns_shape(p->base.split_n2 - 1, &m1, &ng, &nt, r)) info->row_threads = nt;
     }
 }
 
@@ -1072,23 +1130,67 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
         // its direct transform); W_C and, for an odd prime N1 > 31, the DFT_N1 table of
         // k_splitcols
         const int n1s = a.split_n1, n2s = a.split_n2;
-        const int tile_rows = std::min(p->opts.u_tile, rows / 2 + 1);
-        spfft_plan_s* sub = nullptr;
-        st = plan_create_ex(tile_rows * n1s, n2s, &p->split_opts, false, &sub);
-        if (st != SPFFT_OK) return st;
-        p->split_sub.reset(sub);
-        if (!sub->base.fused || sub->base.rader)
-            return fail(SPFFT_ERR_UNSUPPORTED, "split transform: the width-" + std::to_string(n2s) +
-                                                   " row pass is not on chip");
-        p->split_len = sub->M;
         std::vector<int> perm((size_t)cols);
-        for (int i1 = 0; i1 < n1s; i1++)
-            for (int i2 = 0; i2 < n2s; i2++) {
-                const size_t pos = (size_t)i1 * n2s + i2;
-                perm[pos] = i1 + n1s * i2;
-                const long long q = (long long)i2 * i2 % (2LL * n2s);
-                h[pos] = sub->base.direct ? make_float2(1.f, 0.f) : to_f2(unit_root(q, 2LL * n2s, -1));
+        if (p->split_rr) {
+            // Rader over each run (k_raderruns): run position j = n2 + M2 n1 < L holds
+            // x[g^q] of the run (column i1 + N1 g^q), position L holds x[0] (rader_tables)
+            int m1, ng, nt, r[3];
+            spfft::raderruns_shape(n2s - 1, &m1, &ng, &nt, r);
+            const int L = n2s - 1, m2 = r[0] * r[1] * r[2];
+            std::vector<int> gpos, kt;
+            std::vector<float2> bhr, t2r((size_t)m2);
+            rader_tables(n2s, m1, m2, gpos, kt, bhr);
+            for (int i1 = 0; i1 < n1s; i1++) {
+                for (int j = 0; j < L; j++) perm[(size_t)i1 * n2s + j] = i1 + n1s * gpos[j];
+                perm[(size_t)i1 * n2s + L] = i1;
             }
+            for (int k = 0; k < cols; k++) h[k] = make_float2(1.f, 0.f);
+            for (int i = 0; i < m2; i++) t2r[i] = to_f2(unit_root(i, m2, -1));
+            SPFFT_HIP(upload(p->rader_k, kt), "upload Rader indices");
+            SPFFT_HIP(upload(p->rr_bhat, bhr), "upload B^");
+            SPFFT_HIP(upload(p->rr_tw2, t2r), "upload twiddles");
+            spfft::RowPassArgs& b = p->split_rr_args;
+            b = spfft::RowPassArgs{};
+            b.C = n2s;
+            b.M = L;
+            b.M1 = m1;
+            b.M2 = m2;
+            b.rader = 1;
+            b.rader_p = n2s;
+            b.fused = 1;
+            b.fused_rpw = ng;
+            b.fused_nt = nt;
+            b.f2.L = m2;
+            b.f2.nstages = 0;
+            for (int i = 0, ns = 1; i < 3; i++) {
+                if (r[i] == 1) continue;
+                b.f2.radix[b.f2.nstages] = r[i];
+                b.f2.ns_before[b.f2.nstages++] = ns;
+                ns *= r[i];
+            }
+            b.rader_k = p->rader_k.as<int>();
+            b.bhat = p->rr_bhat.as<float2>();
+            b.tw2 = p->rr_tw2.as<float2>();
+            SPFFT_HIP(spfft::launch_raderruns(b, nullptr, true), "Rader run kernel LDS limit");
+            p->split_len = L;
+        } else {
+            const int tile_rows = std::min(p->opts.u_tile, rows / 2 + 1);
+            spfft_plan_s* sub = nullptr;
+            st = plan_create_ex(tile_rows * n1s, n2s, &p->split_opts, false, &sub);
+            if (st != SPFFT_OK) return st;
+            p->split_sub.reset(sub);
+            if (!sub->base.fused || sub->base.rader)
+                return fail(SPFFT_ERR_UNSUPPORTED, "split transform: the width-" + std::to_string(n2s) +
+                                                       " row pass is not on chip");
+            p->split_len = sub->M;
+            for (int i1 = 0; i1 < n1s; i1++)
+                for (int i2 = 0; i2 < n2s; i2++) {
+                    const size_t pos = (size_t)i1 * n2s + i2;
+                    perm[pos] = i1 + n1s * i2;
+                    const long long q = (long long)i2 * i2 % (2LL * n2s);
+                    h[pos] = sub->base.direct ? make_float2(1.f, 0.f) : to_f2(unit_root(q, 2LL * n2s, -1));
+                }
+        }
         std::vector<float2> w((size_t)256 + (cols + 255) / 256);
         for (int e = 0; e < 256; e++) w[e] = to_f2(unit_root(e, cols, -1));
         for (int i = 0; i < (cols + 255) / 256; i++) w[256 + i] = to_f2(unit_root(256LL * i, cols, -1));
@@ -1118,42 +1220,18 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
         for (int n2 = 0; n2 < p->M2; n2++)
             tm[(size_t)k1 * p->M2 + n2] = to_f2(unit_root((long long)k1 * n2, M, -1));
     if (a.rader) {
-        // g a primitive root of p, L = p - 1 = M = M1 M2 (coprime): the prime-factor
-        // orders of the kernel (rowpass_rader.hip).  Entry (n1, n2) of a convolution
-        // operand, X0 position s L + n2 + M2 n1, is a_s[q], q = (M2 n1 + M1 n2) mod L,
-        // i.e. column (p s + 2 g^q) mod 2p; positions 2L, 2L + 1 hold columns 0 and p.
-        // kt[n2 + M2 n1] = g^-q mod p for the same q; b[q] = W_p^{g^-q}; frequency
-        // (k1, k2) of B^ is (c1 k1 + c2 k2) mod L, c1 = M2 (M2^-1 mod M1),
-        // c2 = M1 (M1^-1 mod M2).
-        const long long pr = a.rader_p, g = primitive_root(pr), ginv = pow_mod(g, pr - 2, pr);
-        const long long m1 = p->M1, m2 = p->M2;
+        // X0 position s L + n2 + M2 n1 holds the column of a_s at (n1, n2), (p s + 2 g^q)
+        // mod 2p; positions 2L, 2L + 1 hold columns 0 and p (rader_tables)
+        const long long pr = a.rader_p, m1 = p->M1, m2 = p->M2;
         if (std::gcd(m1, m2) != 1) return fail(SPFFT_ERR_UNSUPPORTED, "Rader shape with non-coprime M1, M2");
-        std::vector<int> kt((size_t)M), perm((size_t)cols), gpow((size_t)M), gneg((size_t)M);
-        std::vector<cd> b((size_t)M);
-        long long gm = 1, gq = 1;
-        for (int q = 0; q < M; q++) {
-            gneg[q] = (int)gm;
-            gpow[q] = (int)gq;
-            b[q] = unit_root(gm, pr, -1);
-            gm = gm * ginv % pr;
-            gq = gq * g % pr;
+        std::vector<int> gpos, kt, perm((size_t)cols);
+        rader_tables(pr, m1, m2, gpos, kt, bh);
+        for (int pos = 0; pos < M; pos++) {
+            perm[pos] = 2 * gpos[pos];
+            perm[(size_t)M + pos] = (int)((pr + 2LL * gpos[pos]) % (2 * pr));
         }
-        for (long long n1 = 0; n1 < m1; n1++)
-            for (long long n2 = 0; n2 < m2; n2++) {
-                const size_t pos = (size_t)(n2 + m2 * n1);
-                const long long q = (m2 * n1 + m1 * n2) % M;
-                kt[pos] = gneg[q];
-                perm[pos] = (int)(2LL * gpow[q]);
-                perm[(size_t)M + pos] = (int)((pr + 2LL * gpow[q]) % (2 * pr));
-            }
         perm[2 * (size_t)M] = 0;
         perm[2 * (size_t)M + 1] = (int)pr;
-        fft64(b, -1);
-        const long long c1 = m2 * inv_mod(m2, m1) % M, c2 = m1 * inv_mod(m1, m2) % M;
-        const double inv_m = 1.0 / (double)M;
-        for (long long k1 = 0; k1 < m1; k1++)
-            for (long long k2 = 0; k2 < m2; k2++)
-                bh[(size_t)(k1 * m2 + k2)] = to_f2(b[(size_t)((c1 * k1 + c2 * k2) % M)] * inv_m);
         SPFFT_HIP(upload(p->rader_k, kt), "upload Rader indices");
         SPFFT_HIP(upload(p->colperm, perm), "upload Rader column order");
         a.rader_k = p->rader_k.as<int>();
@@ -1273,7 +1351,8 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
     p->table_bytes = p->h.bytes + p->wr_lo.bytes + p->wr_hi.bytes + p->wr_full.bytes + p->tw2.bytes + p->twM.bytes +
                      p->wm2.bytes + p->rader_k.bytes + p->colperm.bytes +
                      p->bhat.bytes + p->mfma_dft.bytes + p->mfma_frag.bytes + p->T.bytes + p->X0.bytes +
-                     p->split_w.bytes + p->split_dft.bytes + (p->split_sub ? p->split_sub->table_bytes : 0);
+                     p->split_w.bytes + p->split_dft.bytes + p->rr_bhat.bytes + p->rr_tw2.bytes +
+                     (p->split_sub ? p->split_sub->table_bytes : 0);
     *out = guard.release();
     return SPFFT_OK;
 }

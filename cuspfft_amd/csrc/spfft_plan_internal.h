@@ -88,8 +88,13 @@ struct spfft_plan_s {
     // X0 row (its tables and shape; its own workspace is unused), W_C and DFT_N1 tables
     std::unique_ptr<spfft_plan_s> split_sub;
     spfft_opts split_opts{};  // the options of that plan
-    int split_len = 0;        // its row-pass length (Bluestein M, or N2 when direct)
+    int split_len = 0;        // its row-pass length (Bluestein M, N2 when direct, N2 - 1 by Rader)
     DevBuf split_w, split_dft;
+    // or, for an odd prime N2 with a k_raderruns shape, Rader's algorithm over the runs:
+    // its kernel arguments (tables: rader_k, rr_bhat, rr_tw2) in place of split_sub
+    bool split_rr = false;
+    spfft::RowPassArgs split_rr_args{};
+    DevBuf rr_bhat, rr_tw2;
     // CSC workspace
     int nnz_cap = -1;
     int last_nnz = 0;

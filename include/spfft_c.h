@@ -101,7 +101,8 @@ enum { SPFFT_PAIR_AUTO = 0, SPFFT_PAIR_OFF = 1 };
  * row DFT (even and odd columns) as cyclic convolutions of length p - 1 (fft_len =
  * p - 1), about half the row-pass work of BLUESTEIN.  SPLIT: for cols = n1 * n2 with
  * a length-n1 column kernel (n1 <= 64 with prime factors <= 31, or an odd prime <= 127)
- * and an on-chip row pass of width n2 (DIRECT, or BLUESTEIN at a fused length): the
+ * and an on-chip row pass of width n2 (Rader's convolution of length n2 - 1 for an odd
+ * prime n2 with a run-kernel shape, else DIRECT, or BLUESTEIN at a fused length): the
  * row DFT as n1 DFTs of length n2 on chip, then n1-point DFTs across them, through a
  * tile buffer of cols entries per row (m1 = n1, m2 = n2 in spfft_plan_info; FP32 form,
  * no fft_len / conv_rows / four-step options).  AUTO: DIRECT, else RADER, when possible;
@@ -116,7 +117,7 @@ typedef struct spfft_plan_info {
     int rows, cols;
     int out_cols;      /* cols/2 + 1                                   */
     int pitch;         /* P, elements                                  */
-    int fft_len;       /* M (SPLIT: the length of the width-m2 row pass)   */
+    int fft_len;       /* M (SPLIT: the width-m2 pass's length: m2, m2 - 1 by Rader, or Bluestein's) */
     int m1, m2;        /* M = m1 * m2 (SPLIT: cols = m1 * m2)             */
     int packed;        /* 1 if CSC indices are stored as uint16        */
     int u_tile;

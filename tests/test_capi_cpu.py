@@ -127,10 +127,13 @@ def test_planner_split_transform():
     import cuspfft_amd as sp
     c5 = sp.describe_plan(40421, 40421)  # 83 * 487: Bluestein M 82944 has no on-chip shape
     assert c5["row_transform"] == sp.TRANSFORM_SPLIT
-    assert (c5["m1"], c5["m2"], c5["fft_len"]) == (83, 487, 1024)
+    # 487 is an odd prime with a run-kernel shape for 486: Rader's length 486
+    assert (c5["m1"], c5["m2"], c5["fft_len"], c5["row_threads"]) == (83, 487, 486, 256)
+    r = sp.describe_plan(41, 3409, row_transform=sp.TRANSFORM_SPLIT)
+    assert (r["m1"], r["m2"], r["fft_len"]) == (7, 487, 486)
     assert c5["row_kernel"] == sp.ROW_FUSED and c5["t_rows"] == 0
     f = sp.describe_plan(41, 7217, row_transform=sp.TRANSFORM_SPLIT)
-    assert (f["m1"], f["m2"], f["fft_len"]) == (7, 1031, 2304)
+    assert (f["m1"], f["m2"], f["fft_len"]) == (7, 1031, 2304)  # 1030 = 2 * 5 * 103: Bluestein
     d = sp.describe_plan(64, 3072, row_transform=sp.TRANSFORM_SPLIT)
     assert (d["m1"], d["m2"], d["fft_len"]) == (2, 1536, 1536)  # direct width-1536 pass
     assert sp.describe_plan(41, 7217)["row_transform"] == sp.TRANSFORM_BLUESTEIN  # fused on chip

@@ -1,6 +1,7 @@
 """GPU parity tests of the split row transform (cols = n1 * n2: an on-chip row pass of
-width n2 over the n1 runs of each row, then k_splitcols' length-n1 DFTs across them),
-through the C ABI, against the FP64 oracle.
+width n2 over the n1 runs of each row -- the width-n2 fused plan, or k_raderruns for an
+odd prime n2 -- then k_splitcols' length-n1 DFTs across them), through the C ABI,
+against the FP64 oracle.
 
 Bar: relative L2 error <= 9e-6 (the plain form's), every output row compared for the
 small cases; the streamed output and graph replays bit-identical to the eager run.
@@ -22,6 +23,12 @@ SPLIT_CASES = [
     (8, 66167, 900, 0, False, (127, 521)),     # n1 = 127, cols > 65536: int32 CSC indices
     (2, 43243, 50, 0, False, (83, 521)),       # rows 0 and 1 only: no mirror rows but row 1
     (1, 43243, 20, 0, False, (83, 521)),       # one row
+    # run length 487 (odd prime, 486 = 2 * 243): Rader's convolution over the runs
+    (40, 40421, 4000, 0, False, (83, 487)),    # the 40421 x 40421 workload's width
+    (100, 40421, 5000, 16, False, (83, 487)),  # tiles of 16 rows, the last one of 3
+    (41, 3409, 3000, 0, True, (7, 487)),       # 147 runs: the last workgroup's 3 of 8
+    (35, 974, 1500, 0, True, (2, 487)),        # 36 runs
+    (1, 974, 30, 0, True, (2, 487)),           # one row, two runs
 ]
 
 
@@ -59,6 +66,21 @@ def test_split_single_nonzero_closed_form(gpu, c0):
     rows, cols, r0 = 12, 43243, 5
     plan, X = _run(sp, rows, cols, np.array([r0], np.int32), np.array([c0], np.int32))
     assert plan.info["row_transform"] == sp.TRANSFORM_SPLIT
+    u = np.arange(rows)[:, None]
+    v = np.arange(cols // 2 + 1)[None, :]
+    ph = ((u * r0) % rows) / rows + ((v * c0) % cols) / cols
+    assert np.abs(X - np.exp(-2j * np.pi * ph)).max() < 2e-5
+
+
+@pytest.mark.parametrize("c0", [0, 1, 82, 83, 83 * 485 + 7, 40420])
+def test_split_rader_runs_single_nonzero(gpu, c0):
+    """Width 83 * 487 (Rader over the runs): one nonzero at (r0, c0) against the closed
+    form; c0 = n1 + 83 n2 covers the runs' x[0] (n2 = 0), which enters each output
+    through the inverse codelet, and their last entries."""
+    sp = gpu
+    rows, cols, r0 = 10, 40421, 3
+    plan, X = _run(sp, rows, cols, np.array([r0], np.int32), np.array([c0], np.int32))
+    assert plan.info["row_transform"] == sp.TRANSFORM_SPLIT and plan.info["fft_len"] == 486
     u = np.arange(rows)[:, None]
     v = np.arange(cols // 2 + 1)[None, :]
     ph = ((u * r0) % rows) / rows + ((v * c0) % cols) / cols
