@@ -214,7 +214,7 @@ constexpr size_t fused_launch_lds(int m1, int nb, int rpw, int r1, int r2, int r
 // run r % vrow_n1 of X0 row r / vrow_n1
 __device__ __forceinline__ size_t x0_row(const RowPassArgs& a, int r) {
     if (a.vrow_n1 == 0) return (size_t)r * a.ldx;
-    const int q = r / a.vrow_n1;
+    const int q = (int)__umulhi((uint32_t)r, a.vrow_magic);  // r / vrow_n1
     return (size_t)q * a.vrow_ld + (size_t)(r - q * a.vrow_n1) * a.C;
 }
 

@@ -119,16 +119,28 @@ __global__ __launch_bounds__(64) void k_colsched_batch(const RowPassArgs* __rest
 }
 
 // acc[0] += w, acc[t] += w s^t (t < N): a lane's rows 64 apart share one phase
-// lookup; each further row costs one complex product
+// lookup.  w s is one complex product; the further rows follow the three-term
+// recurrence w s^(t+1) = 2 Re(s) w s^t - w s^(t-1) (|s| = 1), two FMAs a row instead
+// of a complex product's four operations.  Its float32 error grows faster with t than
+// the products' (8 rows: RMS 2.1e-7 against 1.0e-7 per term, at most 2e-6; 16 rows:
+// 6.1e-7, at most 8e-6, over 20000 random phases of R = 40421).
 template <int N>
 __device__ __forceinline__ void accumulate_rows(float2 (&acc)[N], float2 w, float2 s) {
     acc[0].x += w.x;
     acc[0].y += w.y;
+    if constexpr (N > 1) {
+        float2 wp = w, wc = cmul(w, s);
+        acc[1].x += wc.x;
+        acc[1].y += wc.y;
+        const float c2 = s.x + s.x;
 #pragma unroll
-    for (int t = 1; t < N; t++) {
-        w = cmul(w, s);
-        acc[t].x += w.x;
-        acc[t].y += w.y;
+        for (int t = 2; t < N; t++) {
+            const float2 wn = make_float2(fmaf(c2, wc.x, -wp.x), fmaf(c2, wc.y, -wp.y));
+            acc[t].x += wn.x;
+            acc[t].y += wn.y;
+            wp = wc;
+            wc = wn;
+        }
     }
 }
 

@@ -551,6 +551,7 @@ static spfft_status rowpass_tile(spfft_plan_s& p, RowPassArgs& a, int u0, int n,
     RowPassArgs b = p.split_rr ? p.split_rr_args : p.split_sub->base;
     b.X0 = a.X0;
     b.vrow_n1 = a.split_n1;  // tile row u N1 + n1 is run n1 of X0 row u
+    b.vrow_magic = (uint32_t)(((1ULL << 32) + a.split_n1 - 1) / a.split_n1);
     b.vrow_ld = a.ldx;
     b.out = b.out_m = a.T;
     b.P = a.split_p2;

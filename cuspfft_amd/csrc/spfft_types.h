@@ -77,6 +77,8 @@ struct RowPassArgs {
     int vrow_n1, vrow_ld;      // row kernels, vrow_n1 > 0 (pass A of the split transform): tile row
                                //   r is run r % vrow_n1 of X0 row r / vrow_n1 (X0 rows of vrow_ld
                                //   entries, runs of C); 0: tile row r is X0 row r (ldx entries)
+    uint32_t vrow_magic;       // ceil(2^32 / vrow_n1): r / vrow_n1 = umulhi(r, vrow_magic) for
+                               //   r (vrow_n1 - 1) < 2^32 (tile rows r < 2^22, vrow_n1 <= 127)
     const float2* split_w;     // k_splitcols: W_C^e = split_w[256 + (e >> 8)] * split_w[e & 255]
     const float2* split_dft;   // k_splitcols, odd prime N1 > 31: (cos, sin)(2 pi p q / N1) at
                                //   [(p - 1) H + q - 1], p, q = 1 .. H = (N1 - 1) / 2
