@@ -787,12 +787,14 @@ bool split_rr_ok(int n2, size_t lds_limit) {
            spfft::raderruns_lds_bytes(n2 - 1) <= lds_limit;
 }
 
-// The split C = n1 * n2 with the smallest n1 that has a column kernel and leaves an
-// on-chip row pass of width n2, or 0.  *rr: that pass is k_raderruns (an odd prime n2
-// with a shape: a length-(n2 - 1) convolution instead of Bluestein's >= 2 n2 - 1), else
-// the width-n2 plan of *so.
-int find_split(int cols, int tile_rows, size_t lds_limit, spfft_opts* so, int* n2_out, int* len, bool* rr) {
+// The split C = n1 * n2 with the smallest n1 (only n1 = n1_req when that is > 0) that
+// has a column kernel and leaves an on-chip row pass of width n2, or 0.  *rr: that pass
+// is k_raderruns (an odd prime n2 with a shape: a length-(n2 - 1) convolution instead of
+// Bluestein's >= 2 n2 - 1), else the width-n2 plan of *so.
+int find_split(int cols, int n1_req, int tile_rows, size_t lds_limit, spfft_opts* so, int* n2_out, int* len,
+               bool* rr) {
     for (int n1 = 2; n1 <= 127 && n1 <= cols / 2; n1++) {
+        if (n1_req > 0 && n1 != n1_req) continue;
         bool direct = false;
         if (cols % n1 || !spfft::split_supports_n1(n1, &direct)) continue;
         if ((long long)tile_rows * n1 > (1 << 22)) continue;  // rows of the width-n2 plan
@@ -904,13 +906,16 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
     // split (C = N1 N2): when asked for, or (AUTO) when no on-chip row pass exists for C
     int split_n1 = 0, split_n2 = 0;
     {
-        const bool split_opts_ok = !mfma && o.fft_len == 0 && o.conv_rows == 0 && o.m1 == 0 && o.t_rows == 0 &&
+        // (m1: the split's N1 when SPLIT is asked for)
+        const bool split_opts_ok = !mfma && o.fft_len == 0 && o.conv_rows == 0 &&
+                                   (o.m1 == 0 || o.row_transform == SPFFT_TRANSFORM_SPLIT) && o.t_rows == 0 &&
                                    o.row_threads == 0 && o.row_kernel != SPFFT_ROW_FOUR_STEP;
         const bool want = o.row_transform == SPFFT_TRANSFORM_SPLIT ||
                           (o.row_transform == SPFFT_TRANSFORM_AUTO && !fused_ok);
         const int tile_rows = std::min(o.u_tile > 0 ? o.u_tile : rows / 2 + 1, rows / 2 + 1);
         if (split_opts_ok && want)
-            split_n1 = find_split(cols, tile_rows, lds_limit, &p->split_opts, &split_n2, &p->split_len,
+            split_n1 = find_split(cols, o.row_transform == SPFFT_TRANSFORM_SPLIT ? o.m1 : 0, tile_rows, lds_limit,
+                                  &p->split_opts, &split_n2, &p->split_len,
                                   &p->split_rr);
         if (o.row_transform == SPFFT_TRANSFORM_SPLIT && !split_n1)
             return fail(SPFFT_ERR_UNSUPPORTED, "no split transform for cols = " + std::to_string(cols) +

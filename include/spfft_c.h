@@ -57,7 +57,8 @@ typedef struct spfft_opts {
                         rows/2 + 1 computed rows                                   */
     int precision;   /* SPFFT_PREC_*                                           */
     int fft_len;     /* 0: from length_rule; else an explicit 7-smooth M >= 2*cols-1 */
-    int m1;          /* 0: automatic; else the first factor of M = m1 * m2 (a 7-smooth m1 <= 64) */
+    int m1;          /* 0: automatic; else the first factor of M = m1 * m2 (a 7-smooth m1 <= 64);
+                        row_transform SPLIT: the split's n1 (cols = n1 * n2)               */
     int conv_rows;   /* 0: automatic; else rows per workgroup of the convolution kernel (<= 64) */
     int colpass_rows;  /* 0: automatic; else 1, 2, 4, 8 or 16 output rows per lane in the column pass */
     int colpass_table; /* 0: automatic; 1: two-level W_R table; 2: whole W_R table in LDS      */

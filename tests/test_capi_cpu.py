@@ -147,6 +147,23 @@ def test_planner_split_transform():
     assert sp.describe_plan(30, 40427)["row_transform"] == sp.TRANSFORM_BLUESTEIN
 
 
+def test_planner_split_n1_option():
+    """With TRANSFORM_SPLIT, m1 names the split's n1: it must divide cols, have a column
+    kernel and leave an on-chip width-n2 pass; m1 = 0 keeps the smallest such n1."""
+    import cuspfft_amd as sp
+    S = sp.TRANSFORM_SPLIT
+    i = sp.describe_plan(300, 9490, row_transform=S, m1=10)
+    assert (i["m1"], i["m2"]) == (10, 949)
+    assert sp.describe_plan(300, 9490, row_transform=S)["m1"] == 2  # the smallest n1
+    i = sp.describe_plan(300, 9490, row_transform=S, m1=5)
+    assert (i["m1"], i["m2"]) == (5, 1898)
+    for bad in (3, 7, 949):  # 3, 7 do not divide 9490; 949 has no column kernel
+        with pytest.raises(sp.SpfftError):
+            sp.describe_plan(300, 9490, row_transform=S, m1=bad)
+    i = sp.describe_plan(100, 40421, row_transform=S, m1=83)
+    assert (i["m1"], i["m2"], i["fft_len"]) == (83, 487, 486)
+
+
 def test_planner_rader_transform():
     """Widths 2p (p an odd prime with a shape for p - 1) take Rader's convolutions of
     length p - 1 by default; Bluestein when asked, or when an option rules the on-chip
