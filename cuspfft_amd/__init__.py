@@ -217,10 +217,12 @@ def direct_lengths() -> list[int]:
 
 
 def rader_widths() -> list[int]:
-    """Widths cols = 2p with a Rader row transform (row_transform=TRANSFORM_RADER)."""
-    buf = (ctypes.c_int * 256)()
-    k = load().spfft_rader_widths(ctypes.cast(buf, _vp), 256)
-    return list(buf[:min(k, 256)])
+    """Widths with a Rader row transform (row_transform=TRANSFORM_RADER): cols = 2p or
+    cols = p, p an odd prime with a kernel shape for p - 1."""
+    k = load().spfft_rader_widths(None, 0)
+    buf = (ctypes.c_int * max(k, 1))()
+    k = load().spfft_rader_widths(ctypes.cast(buf, _vp), k)
+    return list(buf[:k])
 
 
 # ---------------------------------------------------------------------------

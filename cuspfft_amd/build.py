@@ -35,6 +35,8 @@ LIB_SOURCES = ["csc_build.hip", "csc_noalloc.hip", "rowpass.hip", "rowpass_conv_
                "rowpass_fused_a.hip", "rowpass_fused_b.hip", "rowpass_fused_c.hip",
                "rowpass_fused_d.hip", "rowpass_fused_e.hip", "rowpass_fused_f.hip",
                "rowpass_fused_g.hip", "rowpass_fused_h.hip", "rowpass_rader.hip", "rowpass_raderruns.hip",
+               "rowpass_rader_gen_a.hip", "rowpass_rader_gen_b.hip", "rowpass_rader_gen_c.hip",
+               "rowpass_rader_gen_d.hip",
                "rowpass_split.hip", "rowpass_split_a.hip", "rowpass_split_b.hip", "rowpass_split_c.hip",
                "spfft_plan.cpp", "spfft_batch.cpp", "mlp.hip",
                "spfft_gen.cpp", "sparse_fft_shim.cpp"]
@@ -49,6 +51,8 @@ CLI_SOURCES = ["cli_main.cpp"]
 NO_SIGNED_ZEROS = {"rowpass_fused_a.hip", "rowpass_fused_b.hip", "rowpass_fused_c.hip",
                    "rowpass_fused_d.hip", "rowpass_fused_e.hip", "rowpass_fused_f.hip",
                    "rowpass_fused_g.hip", "rowpass_fused_h.hip", "rowpass_rader.hip", "rowpass_raderruns.hip",
+                   "rowpass_rader_gen_a.hip", "rowpass_rader_gen_b.hip", "rowpass_rader_gen_c.hip",
+                   "rowpass_rader_gen_d.hip",
                    "rowpass_fft1_a.hip", "rowpass_fft1_b.hip", "rowpass_fft1_c.hip"}
 
 

@@ -81,12 +81,13 @@ struct FusedShape {
 bool fused_shape(int i, FusedShape* out);
 // prepare_only: raise the kernel's dynamic LDS limit (once per process, before launches)
 hipError_t launch_fused(const RowPassArgs& a, hipStream_t s, bool prepare_only);
-// Rader row kernel (k_rowrader, rowpass_rader.hip) shapes for widths C = 2p, p an odd
-// prime: M1, threads, stage radices r[3] with p - 1 = M1 r[0] r[1] r[2].
+// Rader row kernel (k_rowrader, rader_impl.h) shapes for widths C = 2p (ns = 2) or C = p
+// (ns = 1), p an odd prime: M1, threads, stage radices r[3] with p - 1 = M1 r[0] r[1] r[2].
 struct RaderShape {
     int m1, nt, r[3];
     size_t lds_bytes;
-    int mfma;  // 1: the precision form (k_rowrader_mfma), 0: FP32 codelets (k_rowrader)
+    int mfma;  // 1: the precision form (k_rowrader_mfma, ns = 2 only), 0: FP32 codelets (k_rowrader)
+    int ns;    // convolutions per row: 2 for C = 2p, 1 for C = p
 };
 bool rader_shape(int i, RaderShape* out);
 hipError_t launch_rader(const RowPassArgs& a, hipStream_t s, bool prepare_only);

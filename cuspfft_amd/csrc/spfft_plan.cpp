@@ -705,7 +705,7 @@ int spfft_rader_widths(int* out, int cap) {
     spfft::RaderShape r;
     for (int i = 0; spfft::rader_shape(i, &r); i++) {
         const int p = r.m1 * r.r[0] * r.r[1] * r.r[2] + 1;
-        if (is_odd_prime(p)) ws.push_back(2 * p);
+        if (is_odd_prime(p)) ws.push_back(r.ns * p);
     }
     std::sort(ws.begin(), ws.end());
     ws.erase(std::unique(ws.begin(), ws.end()), ws.end());
@@ -743,15 +743,16 @@ int find_fused(int M, int m1_req, size_t lds_limit, spfft::FusedShape* out, bool
     return -1;
 }
 
-// Rader shape for the convolution length L = p - 1 (as a fused shape: one batch, the
-// two row groups of k_rowrader), or -1.
-int find_rader(int L, int m1_req, size_t lds_limit, bool mfma, spfft::FusedShape* out, int nt_req = 0) {
+// Rader shape for the convolution length L = p - 1 with ns convolutions per row (2: width
+// 2p, 1: width p), as a fused shape (one batch, rpw = the ns row groups of k_rowrader),
+// or -1.
+int find_rader(int L, int ns, int m1_req, size_t lds_limit, bool mfma, spfft::FusedShape* out, int nt_req = 0) {
     spfft::RaderShape r;
     for (int i = 0; spfft::rader_shape(i, &r); i++) {
-        if (r.m1 * r.r[0] * r.r[1] * r.r[2] != L || (m1_req && r.m1 != m1_req) || r.lds_bytes > lds_limit ||
-            r.mfma != (mfma ? 1 : 0) || (nt_req && r.nt != nt_req))
+        if (r.m1 * r.r[0] * r.r[1] * r.r[2] != L || r.ns != ns || (m1_req && r.m1 != m1_req) ||
+            r.lds_bytes > lds_limit || r.mfma != (mfma ? 1 : 0) || (nt_req && r.nt != nt_req))
             continue;
-        *out = spfft::FusedShape{r.m1, 1, 2, r.nt, {r.r[0], r.r[1], r.r[2]}, r.lds_bytes};
+        *out = spfft::FusedShape{r.m1, 1, r.ns, r.nt, {r.r[0], r.r[1], r.r[2]}, r.lds_bytes};
         return i;
     }
     return -1;
@@ -874,11 +875,13 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
                                                " with these options");
     const bool direct =
         direct_ok && (o.row_transform == SPFFT_TRANSFORM_AUTO || o.row_transform == SPFFT_TRANSFORM_DIRECT);
-    // Rader: cols = 2p, p an odd prime, with a shape for the convolution length p - 1
-    // (the precision form has Rader shapes of its own: k_rowrader_mfma)
+    // Rader: cols = 2p or cols = p, p an odd prime, with a shape for the convolution
+    // length p - 1 (the precision form has Rader shapes of its own: k_rowrader_mfma)
     const bool rader_chip_ok = o.fft_len == 0 && o.conv_rows == 0 && o.row_kernel != SPFFT_ROW_FOUR_STEP;
-    const bool rader_ok = rader_chip_ok && cols % 2 == 0 && is_odd_prime(cols / 2) &&
-                          find_rader(cols / 2 - 1, o.m1, lds_limit, mfma, &fd, o.row_threads) >= 0;
+    const int rader_ns = cols % 2 == 0 ? 2 : 1;
+    const int rader_pr = cols / rader_ns;
+    const bool rader_ok = rader_chip_ok && is_odd_prime(rader_pr) &&
+                          find_rader(rader_pr - 1, rader_ns, o.m1, lds_limit, mfma, &fd, o.row_threads) >= 0;
     if (o.row_transform == SPFFT_TRANSFORM_RADER && !rader_ok)
         return fail(SPFFT_ERR_UNSUPPORTED, "no Rader transform for cols = " + std::to_string(cols) +
                                                " with these options");
@@ -888,7 +891,7 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
         find_fused(cols, o.m1, lds_limit, &fd, true, o.row_threads);
         M = cols;
     }
-    if (rader) M = cols / 2 - 1;
+    if (rader) M = rader_pr - 1;
     p->R = rows;
     p->C = cols;
     p->V = cols / 2 + 1;
@@ -944,7 +947,7 @@ spfft_status plan_shape(int rows, int cols, const spfft_opts* opts_in, size_t bu
         a.fused = 1;
         a.direct = direct ? 1 : 0;
         a.rader = rader ? 1 : 0;
-        a.rader_p = rader ? cols / 2 : 0;
+        a.rader_p = rader ? rader_pr : 0;
         a.fused_nb = fs.nb;
         a.fused_rpw = fs.rpw;
         a.fused_nt = fs.nt;
@@ -1226,18 +1229,24 @@ spfft_status plan_create_ex(int rows, int cols, const spfft_opts* opts_in, bool 
         for (int n2 = 0; n2 < p->M2; n2++)
             tm[(size_t)k1 * p->M2 + n2] = to_f2(unit_root((long long)k1 * n2, M, -1));
     if (a.rader) {
-        // X0 position s L + n2 + M2 n1 holds the column of a_s at (n1, n2), (p s + 2 g^q)
-        // mod 2p; positions 2L, 2L + 1 hold columns 0 and p (rader_tables)
+        // width 2p: X0 position s L + n2 + M2 n1 holds the column of a_s at (n1, n2),
+        // (p s + 2 g^q) mod 2p; positions 2L, 2L + 1 hold columns 0 and p.  Width p:
+        // position n2 + M2 n1 holds column g^q, position L column 0 (rader_tables)
         const long long pr = a.rader_p, m1 = p->M1, m2 = p->M2;
         if (std::gcd(m1, m2) != 1) return fail(SPFFT_ERR_UNSUPPORTED, "Rader shape with non-coprime M1, M2");
         std::vector<int> gpos, kt, perm((size_t)cols);
         rader_tables(pr, m1, m2, gpos, kt, bh);
-        for (int pos = 0; pos < M; pos++) {
-            perm[pos] = 2 * gpos[pos];
-            perm[(size_t)M + pos] = (int)((pr + 2LL * gpos[pos]) % (2 * pr));
+        if (cols == 2 * pr) {
+            for (int pos = 0; pos < M; pos++) {
+                perm[pos] = 2 * gpos[pos];
+                perm[(size_t)M + pos] = (int)((pr + 2LL * gpos[pos]) % (2 * pr));
+            }
+            perm[2 * (size_t)M] = 0;
+            perm[2 * (size_t)M + 1] = (int)pr;
+        } else {
+            for (int pos = 0; pos < M; pos++) perm[pos] = gpos[pos];
+            perm[(size_t)M] = 0;
         }
-        perm[2 * (size_t)M] = 0;
-        perm[2 * (size_t)M + 1] = (int)pr;
         SPFFT_HIP(upload(p->rader_k, kt), "upload Rader indices");
         SPFFT_HIP(upload(p->colperm, perm), "upload Rader column order");
         a.rader_k = p->rader_k.as<int>();

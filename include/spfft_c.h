@@ -97,10 +97,10 @@ enum { SPFFT_PAIR_AUTO = 0, SPFFT_PAIR_OFF = 1 };
  * of the reference's variants.  DIRECT: one FFT of length cols itself when cols has a
  * fused-kernel shape holding the whole row on chip (cols in spfft_direct_lengths();
  * FP32 form, no fft_len / conv_rows / four-step options): no chirps, a quarter of the
- * row-pass work.  RADER: for cols = 2p, p an odd prime with a Rader shape for p - 1
- * (cols in spfft_rader_widths(); same restrictions): the two length-p halves of the
- * row DFT (even and odd columns) as cyclic convolutions of length p - 1 (fft_len =
- * p - 1), about half the row-pass work of BLUESTEIN.  SPLIT: for cols = n1 * n2 with
+ * row-pass work.  RADER: for cols = 2p or cols = p, p an odd prime with a Rader shape
+ * for p - 1 (cols in spfft_rader_widths(); same restrictions): the length-p DFTs of the
+ * row (for 2p: its even and its odd columns) as cyclic convolutions of length p - 1
+ * (fft_len = p - 1), about half the row-pass work of BLUESTEIN.  SPLIT: for cols = n1 * n2 with
  * a length-n1 column kernel (n1 <= 64 with prime factors <= 31, or an odd prime <= 127)
  * and an on-chip row pass of width n2 (Rader's convolution of length n2 - 1 for an odd
  * prime n2 with a run-kernel shape, else DIRECT, or BLUESTEIN at a fused length): the
@@ -361,9 +361,11 @@ int spfft_fused_lengths(int* out, int cap);
  * Replaces the reference's power-of-two-only plain FFT paths for such widths
  * (README.md:653-673: Bluestein doubles the work of a power-of-two width). */
 int spfft_direct_lengths(int* out, int cap);
-/* Widths cols = 2p with a Rader row transform (SPFFT_TRANSFORM_RADER), increasing; same
- * convention.  No counterpart in the reference, whose every non-power-of-two width
- * goes through Bluestein (src/sparse_fft_bluestein.cu:2327-2370). */
+/* Widths with a Rader row transform (SPFFT_TRANSFORM_RADER), increasing; same
+ * convention: cols = 2p for p - 1 = 29 m (1858, 4178, 16706), and cols = p or 2p
+ * from 1024 to 16384 for every prime p whose p - 1 is 7-smooth and has a kernel shape.
+ * No counterpart in the reference, whose every non-power-of-two width goes through
+ * Bluestein (src/sparse_fft_bluestein.cu:2327-2370). */
 int spfft_rader_widths(int* out, int cap);
 
 /* Seeded synthetic binary matrices (host memory, canonical COO: deduplicated,

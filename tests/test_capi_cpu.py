@@ -164,16 +164,57 @@ def test_planner_split_n1_option():
     assert (i["m1"], i["m2"], i["fft_len"]) == (83, 487, 486)
 
 
+CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cuspfft_amd", "csrc")
+
+
+def _load_generator():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("gen_rader_shapes", os.path.join(CSRC, "gen_rader_shapes.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _smooth7(n):
+    for d in (2, 3, 5, 7):
+        while n % d == 0:
+            n //= d
+    return n == 1
+
+
+def _prime(n):
+    return n > 2 and all(n % d for d in range(2, int(n ** 0.5) + 1))
+
+
 def test_planner_rader_transform():
-    """Widths 2p (p an odd prime with a shape for p - 1) take Rader's convolutions of
-    length p - 1 by default; Bluestein when asked, or when an option rules the on-chip
-    kernels out; other widths reject TRANSFORM_RADER."""
+    """Widths 2p and odd prime widths p (p with a shape for p - 1) take Rader's
+    convolutions of length p - 1 by default; Bluestein when asked, or when an option
+    rules the on-chip kernels out; other widths reject TRANSFORM_RADER.  The widths are
+    the hand-written shapes' 2p with p - 1 = 29 m (1858, 4178, 16706) and every prime p
+    with 7-smooth p - 1 in the generated range (widths p and 2p from 1024 to 16384)."""
     import cuspfft_amd as sp
     widths = sp.rader_widths()
     assert widths == sorted(widths) and 16706 in widths
+    assert {1858, 4178, 16706} <= set(widths)
+    # the generated shapes: rader_shapes.h is what gen_rader_shapes.py writes, and the
+    # library has exactly its widths besides the hand-written ones, all of them primes p
+    # or 2p with 7-smooth p - 1 in the generated range (the others of the range have no
+    # M1 <= 32 made of whole prime-power blocks of p - 1 within the thread and register
+    # limits, e.g. 3137: 3136 = 2^6 7^2)
+    gen = _load_generator()
+    with open(os.path.join(CSRC, "rader_shapes.h")) as f:
+        assert f.read() == gen.render()
+    gw = {w for w, _ in gen.shapes()}
+    assert gw == set(widths) - {1858, 4178, 16706} and len(gw) >= 100
+    want = {w for w in range(1024, 16385)
+            if (_prime(w) and _smooth7(w - 1)) or (w % 2 == 0 and _prime(w // 2) and _smooth7(w // 2 - 1))}
+    assert gw <= want and 3137 in want - gw
+    odd = [w for w in widths if w % 2]
+    assert len(odd) >= 50 and 12097 in odd
     for cols in widths:
-        p = cols // 2
-        assert cols % 2 == 0 and all(p % d for d in range(2, int(p ** 0.5) + 1))
+        ns = 1 if cols % 2 else 2
+        p = cols // ns
+        assert _prime(p)
         for rows in (7, 64, cols):
             info = sp.describe_plan(rows, cols)
             assert info["rader"] == 1 and info["direct"] == 0 and info["fft_len"] == p - 1
@@ -181,16 +222,21 @@ def test_planner_rader_transform():
             assert sp.describe_plan(rows, cols, row_transform=sp.TRANSFORM_RADER)["rader"] == 1
             b = sp.describe_plan(rows, cols, row_transform=sp.TRANSFORM_BLUESTEIN)
             assert b["rader"] == 0 and b["fft_len"] >= 2 * cols - 1
-        # the precision form has Rader shapes of its own (k_rowrader_mfma, M1 = 29)
-        mf = sp.describe_plan(64, cols, precision=sp.PREC_BF16X3)
-        assert mf["rader"] == 1 and mf["precision"] == 1 and mf["m1"] == 29 and mf["fft_len"] == p - 1
+        # the precision form has Rader shapes of its own (k_rowrader_mfma, M1 = 29) at the
+        # hand-written widths only
+        if cols in (1858, 4178, 16706):
+            mf = sp.describe_plan(64, cols, precision=sp.PREC_BF16X3)
+            assert mf["rader"] == 1 and mf["precision"] == 1 and mf["m1"] == 29 and mf["fft_len"] == p - 1
+        else:
+            assert sp.describe_plan(64, cols, precision=sp.PREC_BF16X3)["rader"] == 0
         for kw in (dict(row_kernel=sp.ROW_FOUR_STEP),
                    dict(fft_len=sp.next_3_smooth(2 * cols - 1)), dict(conv_rows=4)):
             assert sp.describe_plan(64, cols, **kw)["rader"] == 0
             with pytest.raises(sp.SpfftError):
                 sp.describe_plan(64, cols, row_transform=sp.TRANSFORM_RADER, **kw)
-    # prime halves without a shape, odd widths, smooth widths
-    for cols in (2 * 8363, 16707, 2310, 2048, 6):
+    # prime halves or primes without a shape (8362 = 2 * 37 * 113; c2's 8219: 8218 =
+    # 2 * 7 * 587), odd composite widths, smooth widths
+    for cols in (2 * 8363, 8219, 16707, 2310, 2048, 6):
         assert sp.describe_plan(64, cols)["rader"] == 0
         with pytest.raises(sp.SpfftError):
             sp.describe_plan(64, cols, row_transform=sp.TRANSFORM_RADER)
