@@ -60,7 +60,9 @@ GEN_BIG = [(1153, 1153, 12000, 1153), (1153, 400, 6000, 64), (12097, 300, 9000, 
 def test_rader_generated_large(gpu, oracle_mod, cols, rows, nnz, tile):
     """Odd prime widths (1153, 12097, 15361) and widths 2p (2402, 8402, 16202) at larger
     sizes: the FP64 oracle on every row (sampled rows above 1000 rows), Rader against the
-    Bluestein plan of the same matrix."""
+    Bluestein plan of the same matrix; the streamed output (rows to pinned host memory)
+    bit-identical to the device output up to 2402 rows."""
+    import torch
     sp = gpu
     assert cols in sp.rader_widths()
     rng = np.random.default_rng(cols + 3 * rows)
@@ -78,6 +80,12 @@ def test_rader_generated_large(gpu, oracle_mod, cols, rows, nnz, tile):
     err, errb = oracle_mod.rel_l2(X[u], ref), oracle_mod.rel_l2(Xb[u], ref)
     print(f"  rel_l2 rader={err:.3e} bluestein={errb:.3e}")
     assert err <= REL_L2_PLAIN and errb <= REL_L2_PLAIN
+    if rows <= 2402:
+        host = plan.alloc_host_output()
+        host.fill_(complex(float("nan"), float("nan")))
+        plan.execute_to_host(_dev(r), _dev(c), host)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(host[:, : cols // 2 + 1].numpy(), X)
 
 
 @pytest.mark.parametrize("r0,c0", [(0, 0), (1, 1), (5, 1152), (37, 576), (100, 577)])
@@ -96,8 +104,8 @@ def test_rader_odd_width_single_nonzero(gpu, r0, c0):
 
 
 def test_batch_graph_with_generated_rader_widths(gpu, oracle_mod):
-    """A batch mixing odd prime widths, generated widths 2p, c3's width, a direct width and
-    a Bluestein one (shared launches: one row-kernel launch per shape): eager against the
+    """A batch mixing odd prime widths, generated widths 2p, a direct width and a Bluestein
+    one (shared launches: one row-kernel launch per shape): eager against the
     FP64 oracle, two graph replays bit-identical to the eager run, and the per-matrix
     launch mode bit-identical to the shared one."""
     import torch
