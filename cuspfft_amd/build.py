@@ -40,7 +40,7 @@ LIB_SOURCES = ["csc_build.hip", "csc_noalloc.hip", "rowpass.hip", "rowpass_conv_
                "rowpass_split.hip", "rowpass_split_a.hip", "rowpass_split_b.hip", "rowpass_split_c.hip",
                "spfft_plan.cpp", "spfft_batch.cpp", "mlp.hip",
                "spfft_gen.cpp", "sparse_fft_shim.cpp"]
-CLI_SOURCES = ["cli_main.cpp"]
+CLI_SOURCES = ["cli_main.cpp", "cpu_fft.cpp"]
 
 
 # The row-FFT files prune their codelets to the inputs and outputs that can be

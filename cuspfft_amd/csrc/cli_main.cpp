@@ -14,14 +14,20 @@
 //     --sparse-only       accepted for compatibility (no dense baseline is built)
 //     --csc-auto          this build's native plan (direct or Rader row transform
 //                         where the width has one, else Bluestein)
-//     --cpu-reference     FFTW3 CPU reference; reported as unavailable when this
-//                         build has no FFTW (the reference compiles it out then)
+//     --cpu-reference     the CPU reference row: a dense FP32 2-D R2C FFT on the host,
+//                         single-threaded, only the transform timed (the reference runs
+//                         FFTW3 here; this build has no FFTW, so the transform is its own,
+//                         cpu_fft.cpp, and the row says so); its output is the ground
+//                         truth of every variant's check line, as in the reference
 //     --check-ref FILE    after each variant, the reference's check line (max_abs,
 //                         max_rel, rms_abs) against a reference half spectrum
-//                         ("SPFFTREF", int32 rows, int32 cols, complex128 values)
+//                         ("SPFFTREF", int32 rows, int32 cols, complex128 values);
+//                         overrides --cpu-reference as the ground truth
+//     --write-ref FILE    write the --cpu-reference spectrum in that format
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -31,6 +37,7 @@
 
 #include "../../include/sparse_fft.h"
 #include "../../include/spfft_c.h"
+#include "cpu_fft.h"
 
 namespace {
 
@@ -122,6 +129,49 @@ RefSpectrum load_reference(const std::string& path) {
     return r;
 }
 
+void write_reference(const std::string& path, const RefSpectrum& r) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) throw std::runtime_error("cannot write reference spectrum " + path);
+    const int dims[2] = {r.rows, r.cols};
+    const bool ok = std::fwrite("SPFFTREF", 1, 8, f) == 8 && std::fwrite(dims, sizeof(int), 2, f) == 2 &&
+                    std::fwrite(r.re_im.data(), sizeof(double), r.re_im.size(), f) == r.re_im.size();
+    if (std::fclose(f) != 0 || !ok) throw std::runtime_error("cannot write reference spectrum " + path);
+}
+
+// The CPU reference row (reference src/cpu_reference.cpp:8-64): the binary matrix as a
+// dense R x C float array, then the timed 2-D R2C transform (spfft_cpu::rfft2).
+struct CpuResult {
+    RefSpectrum spectrum;
+    float ms = 0.f;
+    size_t mem_bytes = 0;
+};
+
+CpuResult cpu_reference(const COOMatrix& coo) {
+    const size_t R = coo.rows, C = coo.cols, V = C / 2 + 1;
+    std::vector<float> dense(R * C, 0.f);
+    for (int i = 0; i < coo.nnz; i++) {
+        const int r = coo.row_idx[i], c = coo.col_idx[i];
+        if (r < 0 || c < 0 || (size_t)r >= R || (size_t)c >= C)
+            throw std::runtime_error("COO index out of range");
+        dense[(size_t)r * C + c] = 1.f;
+    }
+    std::vector<spfft_cpu::cf> out(R * V);
+    const auto t0 = std::chrono::steady_clock::now();
+    spfft_cpu::rfft2((int)R, (int)C, dense.data(), out.data());
+    const auto t1 = std::chrono::steady_clock::now();
+    CpuResult res;
+    res.ms = (float)std::chrono::duration<double, std::milli>(t1 - t0).count();
+    res.mem_bytes = dense.size() * sizeof(float) + out.size() * sizeof(spfft_cpu::cf);
+    res.spectrum.rows = (int)R;
+    res.spectrum.cols = (int)C;
+    res.spectrum.re_im.resize(out.size() * 2);
+    for (size_t i = 0; i < out.size(); i++) {
+        res.spectrum.re_im[2 * i] = out[i].real();
+        res.spectrum.re_im[2 * i + 1] = out[i].imag();
+    }
+    return res;
+}
+
 // The reference driver's error triple (src/main.cu:168-215): over every entry of the
 // half spectrum, max |x - ref|, max |x - ref| / max(1, |ref|), and the root mean square
 // of |x - ref|; printed in the same line format.
@@ -173,7 +223,7 @@ void usage(const char* p) {
             "Usage: %s <matrix.mtx | gen:c1|c2|c3|c5|p14> [--csc-mixed-radix] [--csc-mixed-radix-graph]\n"
             "          [--csc-cufft-smooth] [--csc-stockham-stream] [--csc-stockham-cufft-stream]\n"
             "          [--csc-auto] [--csc-tile N] [--repeat N] [--sparse-only] [--cpu-reference]\n"
-            "          [--check-ref FILE]\n",
+            "          [--check-ref FILE] [--write-ref FILE]\n",
             p);
 }
 
@@ -185,7 +235,7 @@ int main(int argc, char** argv) {
     bool mixed = false, mixed_graph = false, smooth = false, cpu_ref = false;
     bool stream3 = false, stream7 = false, native = false;
     int tile = 1024, repeat = 1;
-    std::string ref_path;
+    std::string ref_path, write_path;
     try {
         for (int i = 2; i < argc; i++) {
             const std::string f(argv[i]);
@@ -197,6 +247,10 @@ int main(int argc, char** argv) {
             else if (f == "--csc-stockham-cufft-stream") stream7 = true;
             else if (f == "--csc-auto") native = true;
             else if (f == "--cpu-reference") cpu_ref = true;
+            else if (f == "--write-ref") {
+                if (++i >= argc) throw std::runtime_error("--write-ref requires a file name");
+                write_path = argv[i];
+            }
             else if (f == "--check-ref") {
                 if (++i >= argc) throw std::runtime_error("--check-ref requires a file name");
                 ref_path = argv[i];
@@ -219,10 +273,20 @@ int main(int argc, char** argv) {
         printf("nnz:    %d  (density %.4f%%)\n\n", coo.nnz, 100.0 * coo.nnz / ((double)coo.rows * coo.cols));
         printf("%-26s  %10s  %12s\n", "Method", "Time (ms)", "Mem (MB)");
         printf("%-26s  %10s  %12s\n", "--------------------------", "----------", "------------");
-        if (cpu_ref)
-            printf("%-26s  %10s  %12s  [%s]\n", "CPU reference (FFTW3)", "FAIL", "FAIL",
-                   "FFTW3 not available in this build");
         RefSpectrum ref;
+        if (cpu_ref) {
+            // the reference's CPU row (src/main.cu:324-342): dense scatter untimed, only the
+            // transform timed; its output becomes the ground truth of the check lines
+            try {
+                CpuResult cpu = cpu_reference(coo);
+                printf("%-26s  %10.3f  %12.2f  [CPU, own FP32 FFT single-thread; no FFTW3 in this build]\n",
+                       "CPU reference (FP32 FFT)", cpu.ms, cpu.mem_bytes / 1e6);
+                ref = std::move(cpu.spectrum);
+                if (!write_path.empty()) write_reference(write_path, ref);
+            } catch (const std::exception& e) {
+                printf("%-26s  %10s  %12s  [%s]\n", "CPU reference (FP32 FFT)", "FAIL", "FAIL", e.what());
+            }
+        }
         if (!ref_path.empty()) {
             ref = load_reference(ref_path);
             if (ref.rows != coo.rows || ref.cols != coo.cols)
@@ -255,7 +319,7 @@ int main(int argc, char** argv) {
                 print_row(v.label, t, r.mem_bytes, repeat);
                 printf("  rate %-20s  %.2f FFTs/s (tile loop%s)\n", v.label, 1000.0 / t.median,
                        v.kind == 3 || v.kind == 4 ? " with device-to-host copies" : "");
-                if (!ref_path.empty()) print_check(v.label, ref, r);
+                if (ref.rows) print_check(v.label, ref, r);
                 (void)hipFree(r.d_output);
             } catch (const std::exception& e) {
                 printf("%-26s  %10s  %12s  [%s]\n", v.label, "OOM", "OOM", e.what());

@@ -442,3 +442,47 @@ def test_automatic_tile():
     assert sp.describe_plan(40421, 40421, u_tile=5, t_rows=7)["t_rows"] == 5
     with pytest.raises(sp.SpfftError):
         sp.describe_plan(40421, 40421, t_rows=-1)
+
+
+def _read_spfftref(path):
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"SPFFTREF"
+    R, C = (int(x) for x in np.frombuffer(raw[8:16], np.int32))
+    X = np.frombuffer(raw[16:], np.float64).reshape(R, C // 2 + 1, 2)
+    return X[..., 0] + 1j * X[..., 1]
+
+
+@pytest.mark.parametrize("rows,cols,nnz", [(2310, 2310, 0), (97, 131, 900), (64, 60, 500), (1, 5, 3),
+                                           (7, 1, 4), (33, 2 * 8353, 400), (128, 1024, 2000)])
+def test_cli_cpu_reference(tmp_path, rows, cols, nnz):
+    """The CLI's --cpu-reference transform (host FP32 FFT, no GPU): mixed radix (c1's
+    2310 = 2 3 5 7 11, 64 x 60, powers of two), Bluestein for prime factors above 31
+    (97 x 131, c3's width 2 x 8353), 1-row and 1-column matrices; its spectrum, written
+    with --write-ref, against numpy.fft.rfft2 of the dense matrix in FP64."""
+    import cuspfft_amd as sp
+    cli = os.path.join(os.path.dirname(sp.LIB_PATH), "spfft_cli")
+    if nnz == 0:
+        m = sp.config_matrix("c1")
+        spec, r, c = "gen:c1", m.row_idx, m.col_idx
+    else:
+        rng = np.random.default_rng(rows * 1000 + cols)
+        idx = rng.choice(rows * cols, min(nnz, rows * cols), replace=False)
+        r, c = idx // cols, idx % cols
+        spec = os.path.join(str(tmp_path), "m.mtx")
+        with open(spec, "w") as f:
+            f.write("%%MatrixMarket matrix coordinate pattern general\n")
+            f.write(f"{rows} {cols} {len(idx)}\n")
+            for a, b in zip(r, c):
+                f.write(f"{a + 1} {b + 1}\n")
+    out = os.path.join(str(tmp_path), "ref.bin")
+    res = subprocess.run([cli, spec, "--cpu-reference", "--write-ref", out], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "CPU reference (FP32 FFT)" in res.stdout and "FAIL" not in res.stdout
+    X = _read_spfftref(out)
+    d = np.zeros((rows, cols))
+    d[r, c] = 1.0
+    ref = np.fft.rfft2(d)
+    assert X.shape == ref.shape
+    err = np.linalg.norm(X - ref) / np.linalg.norm(ref)
+    assert err < 1e-6, err

@@ -590,6 +590,18 @@ def test_cli_reference_flags(gpu, oracle_mod, tmp_path):
         # entries; max_rel divides by max(1, |ref|), so small entries count absolutely
         assert max_abs < 2e-2 and max_rel < 1e-3 and rms < 1e-3, ln
     assert "Sparse CSC (auto)" in res.stdout
+    # --cpu-reference: the host FP32 transform's row, and its output as the ground truth of
+    # every variant's check line (the reference's behaviour, src/main.cu:324-342)
+    res = subprocess.run([cli, "gen:c1", "--cpu-reference", "--csc-mixed-radix", "--csc-cufft-smooth",
+                          "--csc-auto"], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "CPU reference (FP32 FFT)" in res.stdout and "FAIL" not in res.stdout
+    checks = [ln for ln in res.stdout.splitlines() if ln.startswith("  check ")]
+    assert len(checks) == 3, res.stdout
+    for ln in checks:
+        f = ln.split()
+        max_abs, max_rel = float(f[f.index("max_abs") + 1]), float(f[f.index("max_rel") + 1])
+        assert max_abs < 2e-2 and max_rel < 1e-3, ln
     bad = subprocess.run([cli, "gen:c1", "--csc-mixed-radix", "--csc-tile", "0"],
                          capture_output=True, text=True, timeout=60)
     assert bad.returncode == 1 and "--csc-tile must be positive" in bad.stderr
