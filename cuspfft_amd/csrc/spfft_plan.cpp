@@ -465,15 +465,19 @@ spfft_status do_csc_build(spfft_plan_s& p, const int* d_row, const int* d_col, i
 size_t device_lds_limit() { return g_lds_limit; }
 
 // Column-pass kernel of the automatic plan for nnz nonzeros: rows per lane from the
-// number of computed rows in a tile (the extra rows cost one complex product per nonzero
-// instead of a phase reduction and a table lookup): 8 (8-column tiles of 512 rows) once
-// a tile has at least 2048 rows, else 4; and at that size the matrix-core kernel when
-// columns average at least kAutoMfmaDegree nonzeros (measured: 14-30% faster at average
-// degree 24 for R = 3000-16706, 10-20% slower at degree 8).
+// number of computed rows in a tile (a lane's further rows follow a three-term recurrence,
+// two FMAs per component, instead of a phase reduction and a table lookup): 4 below 2048
+// rows in the tile; at 2048 or more, 16 rows per lane (8-column tiles of 1024 rows) when
+// columns average at least kAutoRows16Degree nonzeros, else 8.  Measured (one MI355X,
+// whole transform, U = 8 / 16 / matrix cores): uniform matrices of degree 12-29 at
+// n = 6000-16001 and the banded 8219^2 one (degree 29.5) run fastest with 16 rows, 1-9%
+// ahead of 8 and of the matrix-core kernel (which 16 rows replaced here; it stays a
+// candidate for the autotuner); at degree 7-9 (the power-law 16706^2 and 40421^2) 16 rows
+// lose 19-22% to 8.
 int auto_colpass_rows(const spfft_plan_s& p, int64_t nnz) {
     const int rows = std::min(p.opts.u_tile, p.R / 2 + 1);
     if (rows < 2048) return 4;
-    return nnz >= (int64_t)kAutoMfmaDegree * p.C ? spfft::kColpassMfma : 8;
+    return nnz >= (int64_t)kAutoRows16Degree * p.C ? 16 : 8;
 }
 
 // The one-tile row-pass arguments of plan p for the batched form: p.base with the

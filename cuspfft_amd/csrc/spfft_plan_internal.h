@@ -160,6 +160,6 @@ int auto_colpass_rows(const spfft_plan_s& p, int64_t nnz);
 // automatic W_R table choice: the whole table in LDS only up to this many rows
 constexpr int kAutoFullTableRows = 2048;
 // automatic column pass: the matrix-core kernel from this average column degree
-constexpr int kAutoMfmaDegree = 16;
+constexpr int kAutoRows16Degree = 12;  // average column degree for 16 rows per lane (auto_colpass_rows)
 
 }  // namespace spfft_internal
