@@ -267,6 +267,7 @@ int main(int argc, char** argv) {
                 throw std::runtime_error("Unknown option: " + f);
             }
         }
+        if (!write_path.empty() && !cpu_ref) throw std::runtime_error("--write-ref needs --cpu-reference");
         printf("Matrix: %s\n", argv[1]);
         COOMatrix coo = load(argv[1]);
         printf("Size:   %d x %d\n", coo.rows, coo.cols);

@@ -486,3 +486,6 @@ def test_cli_cpu_reference(tmp_path, rows, cols, nnz):
     assert X.shape == ref.shape
     err = np.linalg.norm(X - ref) / np.linalg.norm(ref)
     assert err < 1e-6, err
+    if nnz == 0:
+        bad = subprocess.run([cli, spec, "--write-ref", out], capture_output=True, text=True, timeout=60)
+        assert bad.returncode == 1 and "--write-ref needs --cpu-reference" in bad.stderr
