@@ -464,20 +464,20 @@ spfft_status do_csc_build(spfft_plan_s& p, const int* d_row, const int* d_col, i
 
 size_t device_lds_limit() { return g_lds_limit; }
 
-// Column-pass kernel of the automatic plan for nnz nonzeros: rows per lane from the
-// number of computed rows in a tile (a lane's further rows follow a three-term recurrence,
-// two FMAs per component, instead of a phase reduction and a table lookup): 4 below 2048
-// rows in the tile; at 2048 or more, 16 rows per lane (8-column tiles of 1024 rows) when
-// columns average at least kAutoRows16Degree nonzeros, else 8.  Measured (one MI355X,
-// whole transform, U = 8 / 16 / matrix cores): uniform matrices of degree 12-29 at
-// n = 6000-16001 and the banded 8219^2 one (degree 29.5) run fastest with 16 rows, 1-9%
-// ahead of 8 and of the matrix-core kernel (which 16 rows replaced here; it stays a
-// candidate for the autotuner); at degree 7-9 (the power-law 16706^2 and 40421^2) 16 rows
-// lose 19-22% to 8.
+// Column-pass kernel of the automatic plan for nnz nonzeros: rows per lane (a lane's
+// further rows follow a three-term recurrence, two FMAs per component, instead of a phase
+// reduction and a table lookup): 16 rows per lane (8-column tiles of 1024 rows) when a
+// tile has at least 2048 rows and the columns average at least kAutoRows16Degree
+// nonzeros, else 8 (tiles of 512 rows).  Measured (one MI355X, whole transforms):
+// * uniform matrices of degree 12-29 at n = 6000-16001 and the banded 8219^2 one (degree
+//   29.5): 16 rows fastest, 1-9% ahead of 8 and of the matrix-core kernel (which 16 rows
+//   replaced here; it stays a candidate for the autotuner);
+// * degree 7-9 (the power-law 16706^2 and 40421^2): 16 rows lose 19-22% to 8;
+// * tiles below 2048 rows (n = 1000-3500, degree 6-26, and the 2310^2 matrix): 8 rows
+//   3-10% ahead of 4 (the earlier choice there) and of 16.
 int auto_colpass_rows(const spfft_plan_s& p, int64_t nnz) {
     const int rows = std::min(p.opts.u_tile, p.R / 2 + 1);
-    if (rows < 2048) return 4;
-    return nnz >= (int64_t)kAutoRows16Degree * p.C ? 16 : 8;
+    return rows >= 2048 && nnz >= (int64_t)kAutoRows16Degree * p.C ? 16 : 8;
 }
 
 // The one-tile row-pass arguments of plan p for the batched form: p.base with the
