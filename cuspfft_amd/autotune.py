@@ -167,7 +167,8 @@ def tune_matrix(m, reps: int = 3, max_m1: int = 3, record: dict | None = None, r
         ms, st, info = time_plan(m.rows, m.cols, d_row, d_col, o, reps, row_limit)
         o = dict(o, row_kernel=info["row_kernel"])  # the kernel the plan resolved to
         t = dict(opts=o, ms=ms, stages=st, fft_len=info["fft_len"], m1=info["m1"], m2=info["m2"],
-                 direct=info["direct"], rader=info["rader"], row_threads=info["row_threads"])
+                 direct=info["direct"], rader=info["rader"], row_threads=info["row_threads"],
+                 split=int(info["row_transform"] == TRANSFORM_SPLIT))
         if default:
             t["default"] = True
         trials.append(t)

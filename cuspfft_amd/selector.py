@@ -79,15 +79,20 @@ def features(rec: dict, opts: dict, info: dict | None = None, trial: bool = Fals
 
 def trial_opts(t: dict, rec: dict | None = None) -> dict:
     """The options of a timed trial with its Bluestein length pinned to the one it ran;
-    a trial that ran the direct length-cols transform or Rader's convolutions (flagged,
-    or, in records written before the flags, a length of cols or of p - 1 for cols = 2p
-    or p) is planned as that again."""
+    a trial that ran the direct length-cols transform, Rader's convolutions or the split
+    transform (flagged, or, in records written before the flags, a length of cols, of
+    p - 1 for cols = 2p or p, or below 2 cols - 1) is planned as that again."""
     o = dict(t["opts"])
     n = t.get("fft_len")
     cols = rec["cols"] if rec is not None else None
     direct = t.get("direct") or (cols is not None and n == cols and n < 2 * cols - 1)
     rader = t.get("rader") or (cols is not None and n == (cols // 2 if cols % 2 == 0 else cols) - 1)
-    if direct or rader:
+    # the split transform reports the length of its width-n2 pass, below any Bluestein length
+    split = t.get("split") or (not direct and not rader and cols is not None and n and n < 2 * cols - 1)
+    if split:
+        o.pop("fft_len", None)
+        o["row_transform"] = TRANSFORM_SPLIT
+    elif direct or rader:
         o.pop("fft_len", None)
         o["row_transform"] = TRANSFORM_DIRECT if direct else TRANSFORM_RADER
     elif n:

@@ -1,9 +1,10 @@
 """Trains and evaluates the learned plan selector.
 
     python -m cuspfft_amd.train_selector --dir OUT [--count 2000] [--heldout 256] [--epochs 300]
+                                         [--per-rader-width 16]
 
 1. Tunes the seeded training set (autotune.training_specs: COUNT random matrices
-   plus 16 at each width with a Rader transform) exhaustively, timing each plan in
+   plus PER_RADER_WIDTH (default 16) at each width with a Rader transform) exhaustively, timing each plan in
    full (tile-sampled above an estimated 16 ms per transform), appending to
    OUT/train.jsonl (resumes where it stopped).  Matrices are
    generated 8 at a time, as they are tuned.
@@ -88,6 +89,8 @@ def main(argv=None):
     ap.add_argument("--row-limit", type=int, default=512)
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--skip-c5", action="store_true")
+    ap.add_argument("--per-rader-width", type=int, default=16,
+                    help="training matrices at each width with a Rader transform (4 per width held out)")
     ap.add_argument("--max-seconds", type=float, default=0,
                     help="tuning budget of this run; the DB resumes in the next run")
     args = ap.parse_args(argv)
@@ -96,7 +99,7 @@ def main(argv=None):
     from . import autotune, selector
     t0 = time.time()
     deadline = t0 + args.max_seconds if args.max_seconds > 0 else None
-    specs = autotune.training_specs(args.count, per_rader_width=16)
+    specs = autotune.training_specs(args.count, per_rader_width=args.per_rader_width)
     train = tune_set(specs, os.path.join(args.dir, "train.jsonl"), args.row_limit, args.reps, deadline)
     if len(train) < len(specs):
         log(f"[selector] tuning budget spent with {len(train)}/{len(specs)} matrices; run again to resume")
