@@ -10,8 +10,9 @@
 //   packed      = rows and columns stored as uint16 iff rows, cols <= 65536
 //
 // GPU method (integer work only, no floating point):
-//   1. k_count_cols      per-column counts with LDS-free global atomics (order
-//                        free, so deterministic), range validation -> err flag
+//   1. k_count_hist0     per-column counts with LDS-free global atomics (order
+//                        free, so deterministic), range validation -> err flag;
+//                        the same launch makes the first radix pass's histograms
 //   2. k_scan_*          exclusive scan of (count, count > 0) over the C
 //                        columns -> dense column starts (used by the column
 //                        pass), compact slot of every active column ->
@@ -44,18 +45,6 @@ __global__ void k_zero_counts(int* __restrict__ p, long n, int* __restrict__ q) 
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
     if (i == 0) q[0] = 0;
-}
-
-__global__ void k_count_cols(const int* __restrict__ row, const int* __restrict__ col, int nnz,
-                             int R, int C, int* __restrict__ counts, int* __restrict__ err) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nnz) return;
-    const int r = row[i], c = col[i];
-    if (r < 0 || r >= R || c < 0 || c >= C) {
-        atomicOr(err, 1);
-        return;
-    }
-    atomicAdd(&counts[c], 1);
 }
 
 // Block-wide exclusive scan of one int per thread; returns the block total.
@@ -217,18 +206,45 @@ __global__ __launch_bounds__(kOneThreads) void k_scan_single(const int* __restri
 
 __device__ __forceinline__ int digit_of(int key, int shift) { return (int)(((uint32_t)key >> shift) & 255u); }
 
-__global__ void k_radix_hist(const int* __restrict__ keys, int nnz, int shift, int* __restrict__ hist,
-                             int nblk) {
+__device__ __forceinline__ void radix_hist_block(const int* __restrict__ keys, int nnz, int shift,
+                                                 int* __restrict__ hist, int nblk, int b) {
     __shared__ int s_h[256];
     s_h[threadIdx.x] = 0;
     __syncthreads();
-    const int base = blockIdx.x * kSortBlock;
+    const int base = b * kSortBlock;
     for (int k = 0; k < kSortIters; k++) {
         const int i = base + k * kSortThreads + threadIdx.x;
         if (i < nnz) atomicAdd(&s_h[digit_of(keys[i], shift)], 1);
     }
     __syncthreads();
-    hist[threadIdx.x * nblk + blockIdx.x] = s_h[threadIdx.x];
+    hist[threadIdx.x * nblk + b] = s_h[threadIdx.x];
+}
+
+__global__ void k_radix_hist(const int* __restrict__ keys, int nnz, int shift, int* __restrict__ hist,
+                             int nblk) {
+    radix_hist_block(keys, nnz, shift, hist, nblk, blockIdx.x);
+}
+
+// The column counts (blocks [0, nb_count)) and the first radix pass's
+// digit histograms (k_radix_hist of the input columns, blocks from nb_count on) in one
+// launch: both only read the COO, so one launch does the work of two (small inputs are
+// bound by launch latency).
+__global__ __launch_bounds__(kSortThreads) void k_count_hist0(const int* __restrict__ row,
+                                                              const int* __restrict__ col, int nnz, int R, int C,
+                                                              int* __restrict__ counts, int* __restrict__ err,
+                                                              int nb_count, int* __restrict__ hist, int nblk) {
+    if ((int)blockIdx.x >= nb_count) {
+        radix_hist_block(col, nnz, 0, hist, nblk, blockIdx.x - nb_count);
+        return;
+    }
+    const int i = blockIdx.x * kSortThreads + threadIdx.x;
+    if (i >= nnz) return;
+    const int r = row[i], c = col[i];
+    if (r < 0 || r >= R || c < 0 || c >= C) {
+        atomicOr(err, 1);
+        return;
+    }
+    atomicAdd(&counts[c], 1);
 }
 
 // Stable scatter.  Wave w of block b owns elements
@@ -391,10 +407,14 @@ hipError_t launch_batch_coo(const int* row, const int* col, const long long* coo
 hipError_t launch_csc_build(const CscBuildArgs& a, hipStream_t s) {
     const int R = a.R, C = a.C, nnz = a.nnz;
     // 1. counts + validation
+    // (with the first radix pass's histograms of the input columns)
     hipLaunchKernelGGL(k_zero_counts, dim3(cdiv((long)C + 1, 256)), dim3(256), 0, s, a.counts, (long)C + 1, a.err);
-    if (nnz > 0)
-        hipLaunchKernelGGL(k_count_cols, dim3(cdiv(nnz, 256)), dim3(256), 0, s, a.row, a.col, nnz, R, C,
-                           a.counts, a.err);
+    const int nblk = cdiv(nnz, kSortBlock);
+    if (nnz > 0) {
+        const int nb_count = cdiv(nnz, kSortThreads);
+        hipLaunchKernelGGL(k_count_hist0, dim3(nb_count + nblk), dim3(kSortThreads), 0, s, a.row, a.col, nnz, R, C,
+                           a.counts, a.err, nb_count, a.hist, nblk);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // 2. column scan + compaction
@@ -405,7 +425,6 @@ hipError_t launch_csc_build(const CscBuildArgs& a, hipStream_t s) {
     if (e != hipSuccess || nnz == 0) return e;
     // 3. stable radix sort of rows by column
     const int passes = csc_sort_passes(C);
-    const int nblk = cdiv(nnz, kSortBlock);
     const int* kin = a.col;
     const int* vin = a.row;
     int* kout = a.keys[0];
@@ -413,7 +432,8 @@ hipError_t launch_csc_build(const CscBuildArgs& a, hipStream_t s) {
     for (int p = 0; p < passes; p++) {
         const int shift = 8 * p;
         const bool fin = (p == passes - 1);
-        hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kSortThreads), 0, s, kin, nnz, shift, a.hist, nblk);
+        if (p > 0)  // pass 0's histograms came with the column counts
+            hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kSortThreads), 0, s, kin, nnz, shift, a.hist, nblk);
         e = run_scan<0, false>(a.hist, 256 * nblk, a.scan_tmp, nullptr, a.hist_off, nullptr, nullptr, s);
         if (e != hipSuccess) return e;
         if (fin) {
