@@ -163,10 +163,9 @@ constexpr int kOneItems = 4;
 constexpr int kOneBlock = kOneThreads * kOneItems;
 
 template <int MODE, bool PACKED>
-__global__ __launch_bounds__(kOneThreads) void k_scan_single(const int* __restrict__ in, int n,
-                                                             int* __restrict__ totals, int* __restrict__ out,
-                                                             void* __restrict__ sparse_cols,
-                                                             int* __restrict__ col_ptr) {
+__device__ __forceinline__ void scan_single_body(const int* __restrict__ in, int n, int* __restrict__ totals,
+                                                 int* __restrict__ out, void* __restrict__ sparse_cols,
+                                                 int* __restrict__ col_ptr) {
     __shared__ int s_w[kOneThreads / 64];
     const int base = threadIdx.x * kOneItems;
     int v[kOneItems];
@@ -202,6 +201,28 @@ __global__ __launch_bounds__(kOneThreads) void k_scan_single(const int* __restri
         totals[0] = tx;
         totals[1] = ty;
     }
+}
+
+template <int MODE, bool PACKED>
+__global__ __launch_bounds__(kOneThreads) void k_scan_single(const int* __restrict__ in, int n,
+                                                             int* __restrict__ totals, int* __restrict__ out,
+                                                             void* __restrict__ sparse_cols,
+                                                             int* __restrict__ col_ptr) {
+    scan_single_body<MODE, PACKED>(in, n, totals, out, sparse_cols, col_ptr);
+}
+
+// The column scan (block 0, as k_scan_single<1, PACKED>) and the scan of the first radix
+// pass's histograms (block 1, as k_scan_single<0, false>) in one launch, when both fit
+// one block.
+template <bool PACKED>
+__global__ __launch_bounds__(kOneThreads) void k_scan_single_pair(const int* __restrict__ counts, int C,
+                                                                  int* __restrict__ totals, int* __restrict__ colstart,
+                                                                  void* __restrict__ sparse_cols,
+                                                                  int* __restrict__ col_ptr,
+                                                                  const int* __restrict__ hist, int nh,
+                                                                  int* __restrict__ hist_off) {
+    if (blockIdx.x == 0) scan_single_body<1, PACKED>(counts, C, totals, colstart, sparse_cols, col_ptr);
+    else scan_single_body<0, false>(hist, nh, nullptr, hist_off, nullptr, nullptr);
 }
 
 __device__ __forceinline__ int digit_of(int key, int shift) { return (int)(((uint32_t)key >> shift) & 255u); }
@@ -417,11 +438,23 @@ hipError_t launch_csc_build(const CscBuildArgs& a, hipStream_t s) {
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // 2. column scan + compaction
-    e = a.packed ? run_scan<1, true>(a.counts, C, a.scan_tmp, a.totals, a.colstart, a.sparse_cols,
-                                     a.col_ptr, s)
-                 : run_scan<1, false>(a.counts, C, a.scan_tmp, a.totals, a.colstart, a.sparse_cols,
-                                      a.col_ptr, s);
+    // 2. column scan + compaction (with the first radix pass's histogram scan when both
+    // fit one block)
+    const bool pair = nnz > 0 && (long)C + 1 <= kOneBlock && 256L * nblk + 1 <= kOneBlock;
+    if (pair) {
+        if (a.packed)
+            hipLaunchKernelGGL((k_scan_single_pair<true>), dim3(2), dim3(kOneThreads), 0, s, a.counts, C, a.totals,
+                               a.colstart, a.sparse_cols, a.col_ptr, a.hist, 256 * nblk, a.hist_off);
+        else
+            hipLaunchKernelGGL((k_scan_single_pair<false>), dim3(2), dim3(kOneThreads), 0, s, a.counts, C, a.totals,
+                               a.colstart, a.sparse_cols, a.col_ptr, a.hist, 256 * nblk, a.hist_off);
+        e = hipGetLastError();
+    } else {
+        e = a.packed ? run_scan<1, true>(a.counts, C, a.scan_tmp, a.totals, a.colstart, a.sparse_cols,
+                                         a.col_ptr, s)
+                     : run_scan<1, false>(a.counts, C, a.scan_tmp, a.totals, a.colstart, a.sparse_cols,
+                                          a.col_ptr, s);
+    }
     if (e != hipSuccess || nnz == 0) return e;
     // 3. stable radix sort of rows by column
     const int passes = csc_sort_passes(C);
@@ -434,8 +467,10 @@ hipError_t launch_csc_build(const CscBuildArgs& a, hipStream_t s) {
         const bool fin = (p == passes - 1);
         if (p > 0)  // pass 0's histograms came with the column counts
             hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kSortThreads), 0, s, kin, nnz, shift, a.hist, nblk);
-        e = run_scan<0, false>(a.hist, 256 * nblk, a.scan_tmp, nullptr, a.hist_off, nullptr, nullptr, s);
-        if (e != hipSuccess) return e;
+        if (p > 0 || !pair) {  // pass 0's scan ran with the column scan when both fit one block
+            e = run_scan<0, false>(a.hist, 256 * nblk, a.scan_tmp, nullptr, a.hist_off, nullptr, nullptr, s);
+            if (e != hipSuccess) return e;
+        }
         if (fin) {
             if (a.col_rbase)  // batched: uint16 rows of each column's own matrix (all of R <= 65536)
                 hipLaunchKernelGGL((k_radix_scatter<true, true>), dim3(nblk), dim3(kSortThreads), 0, s, kin, vin,
